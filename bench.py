@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X solver core.
+
+Default workload: SISe3 on 1600 herds over 1460 days with scheduled enter,
+exit, internal- and external-transfer events (the second benchmark
+configuration).  A step is one complete simulation of that model with all
+inputs and the trajectory resident in HBM (SimInf_hip_plan_run); the seed
+changes every step.
+
+Prints one JSON line: node-days/s (value), transitions/s, p50/p99 step
+latency, the roofline of the solver kernel (algorithmic bytes over the
+kernel time measured with HIP events on the plan's stream) and a CPU
+baseline (the C checker, one core, on the same workload).
+"""
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(model):
+    """U/V writeback plus one read of u0, v0, ldata and the events, plus a
+    read-modify-write of source and destination per external transfer."""
+    Nn, Nc, Nd, Nld, Nrep = model.Nn, model.Nc, model.Nd, model.Nld, model.replicates
+    tlen = len(model.tspan)
+    ev = model.events["event"]
+    len_e2 = int((ev == 3).sum())
+    return (4 * Nc * Nn * Nrep * tlen + 8 * Nd * Nn * Nrep * tlen
+            + 4 * Nc * Nn * Nrep + 8 * Nd * Nn * Nrep + 8 * Nld * Nn
+            + 36 * len(ev) + 2 * (2 * 4 * Nc) * len_e2)
+
+
+def make_workload(name):
+    from siminf_amd import workloads as W
+    if name == "sise3_herds":
+        return W.sise3_herds(), {"workload": "SISe3 1600 herds x 1460 days, events",
+                                 "solver": "ssm"}
+    if name == "mparse_sir":
+        return W.mparse_sir_readme(), {"workload": "mparse SIR 1000 nodes x 150 days",
+                                       "solver": "ssm"}
+    if name.startswith("seir_million"):
+        solver = "aem" if name.endswith("aem") else "ssm"
+        return W.seir_million(), {"workload": "SEIR 1e6 nodes x 365 days",
+                                  "solver": solver}
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(model, solver, budget_s=20.0):
+    """The C checker on one host core: full runs of the same workload,
+    as many as fit in about budget_s seconds (at least one)."""
+    from oracle import bridge
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        r = bridge.run(model, solver=solver, seed=1000 + len(times))
+        times.append(time.perf_counter() - t0)
+        if r.err:
+            raise SystemExit("checker failed with %d" % r.err)
+        if time.perf_counter() - t_start > budget_s or len(times) >= 5:
+            break
+    t = statistics.median(times)
+    node_days = model.Nn * model.replicates * _ndays(model)
+    return {"value": node_days / t, "unit": "node-days/s", "cores": 1,
+            "kind": "port",
+            "sample": "%d full run(s) of the same workload, median %.3f s, "
+                      "%d transitions in the last" % (len(times), t, r.transitions)}
+
+
+def _ndays(model):
+    ts = model.tspan
+    f0 = np.floor(ts[0])
+    k = 0
+    while not (f0 + k + 1 > ts[-1]):
+        k += 1
+    return k + 1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sise3_herds")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design, not a launch path)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from siminf_amd import _native
+    model, config = make_workload(args.workload)
+    solver = config["solver"]
+    lib = _native.load()
+    dev = lib.SimInf_hip_model_builtin(model.plugin.encode())
+    bundle = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1, seed=1)
+    plan = ctypes.c_void_p()
+    err = lib.SimInf_hip_plan_create(ctypes.byref(plan), ctypes.byref(bundle.args),
+                                     ctypes.c_void_p(dev))
+    if err:
+        raise SystemExit("plan_create failed: %d" % err)
+
+    def step(seed):
+        e = lib.SimInf_hip_plan_run(plan, seed)
+        if e:
+            raise SystemExit("run failed: %d %s" % (e, lib.SimInf_hip_error_message(e)))
+        tr, nd, ms = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        lib.SimInf_hip_plan_stats(plan, ctypes.byref(tr), ctypes.byref(nd), ctypes.byref(ms))
+        return tr.value, nd.value, ms.value
+
+    for i in range(args.warmup):
+        step(100 + i)
+    lat, kms, trans = [], [], 0
+    node_days = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        tr, nd, ms = step(1 + i)
+        lat.append(time.perf_counter() - s0)
+        kms.append(ms)
+        trans += tr
+        node_days += nd
+    wall = time.perf_counter() - t0
+    lib.SimInf_hip_plan_destroy(plan)
+
+    bytes_per_launch = algorithmic_bytes(model)
+    kernel_s = statistics.mean(kms) / 1e3
+    achieved = bytes_per_launch / kernel_s / 1e9
+    out = {
+        "metric": "node-days/sec (SSA transitions/sec and p50/p99 simulation latency alongside)",
+        "value": node_days / wall,
+        "unit": "node-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 state, f64 rates/time",
+        "data": "synthetic (seeded generator of the configuration's shape)",
+        "config": dict(config, nodes=model.Nn, replicates=model.replicates,
+                       days=_ndays(model), events=int(len(model.events["event"]))),
+        "transitions_per_s": trans / wall,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "kernel_ms_mean": statistics.mean(kms),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(model, solver)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,211 @@
+/*
+ * siminf_hip.h -- C ABI of the MI355X (gfx950) solver core.
+ *
+ * The library replaces the reference's solver entry points
+ *
+ *     int SimInf_run_solver_ssm(SimInf_solver_args *args);
+ *     int SimInf_run_solver_mssm(SimInf_solver_args *args);
+ *     int SimInf_run_solver_mssm_crn(SimInf_solver_args *args);
+ *     int SimInf_run_solver_aem(SimInf_solver_args *args);
+ *                                    (src/SimInf_internal.h:406-420)
+ *
+ * and the dispatch that SimInf_run performs on them (src/SimInf.c:383-399).
+ * SimInf_solver_args below has the reference's field order and meaning
+ * (src/SimInf_internal.h:44-200), so the struct the reference's SimInf_run
+ * fills can be passed as is.  Host function pointers cannot run on the
+ * GPU, so every entry point takes a device plugin (SimInf_hip_model) that
+ * holds the same TRFun/PTSFun bodies compiled for gfx950; tr_fun and
+ * pts_fun in the struct are not called.
+ *
+ * Return values are 0 or a negative SimInf error code (SimInf.h:66-89);
+ * SimInf_hip_error_message gives the reference's message for a code
+ * (src/SimInf.c:28-97).
+ *
+ * All pointers in SimInf_solver_args are host pointers.  The entry points
+ * are blocking: they stage inputs in device memory, run, and copy U/V (or
+ * the sparse prU/prV) back before returning.  The SimInf_hip_plan_* calls
+ * keep inputs and outputs resident in device memory across runs.
+ */
+#ifndef SIMINF_HIP_H
+#define SIMINF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Event types (SimInf.h:56-59). */
+enum {
+    SIMINF_EXIT_EVENT = 0,
+    SIMINF_ENTER_EVENT = 1,
+    SIMINF_INTERNAL_TRANSFER_EVENT = 2,
+    SIMINF_EXTERNAL_TRANSFER_EVENT = 3
+};
+
+/* Error codes (SimInf.h:66-89). */
+typedef enum {
+    SIMINF_ERR_NEGATIVE_STATE       = -1,
+    SIMINF_ERR_ALLOC_MEMORY_BUFFER  = -2,
+    SIMINF_UNDEFINED_EVENT          = -3,
+    SIMINF_INVALID_EDGE_PROBABILITY = -4,
+    SIMINF_ERR_V_IS_NOT_FINITE      = -7,
+    SIMINF_ERR_SAMPLE_SELECT        = -8,
+    SIMINF_ERR_INVALID_MODEL        = -9,
+    SIMINF_ERR_V_IS_NEGATIVE        = -10,
+    SIMINF_ERR_INVALID_RATE         = -11,
+    SIMINF_ERR_UNKNOWN_SOLVER       = -12,
+    SIMINF_ERR_DEST_OUT_OF_BOUNDS   = -13,
+    SIMINF_ERR_NODE_OUT_OF_BOUNDS   = -14,
+    SIMINF_ERR_EVENTS_N             = -15,
+    SIMINF_ERR_EVENT_SHIFT          = -16,
+    SIMINF_ERR_SHIFT_OUT_OF_BOUNDS  = -17,
+    SIMINF_ERR_INVALID_PROPORTION   = -18,
+    SIMINF_ERR_AEM_REPLICATED_MODEL = -19,
+    SIMINF_ERR_SPARSE_MODEL         = -20,
+    SIMINF_ERR_INVALID_OPTIONS      = -21,
+    SIMINF_ERR_INVALID_SEED         = -22,
+    SIMINF_ERR_INVALID_CRN          = -23,
+    SIMINF_ERR_INVALID_RNG_TYPE     = -24,
+    /* Not in the reference: the GPU runtime failed (allocation, launch). */
+    SIMINF_ERR_DEVICE               = -100
+} SimInf_error_code;
+
+/* Plugin signatures (SimInf.h:92-107). */
+typedef double (*TRFun)(const int *u, const double *v, const double *ldata,
+                        const double *gdata, double t);
+typedef int (*PTSFun)(double *v_new, const int *u, const double *v,
+                      const double *ldata, const double *gdata, int node,
+                      double t);
+
+/* Solver arguments; field order and meaning of SimInf_internal.h:44-200. */
+typedef struct SimInf_solver_args {
+    const int *u0;          /* int Nc x (Nn*Nrep), column per node      */
+    const double *v0;       /* double Nd x (Nn*Nrep)                    */
+    const int *irG;         /* G (Nt x Nt) CSC row indices              */
+    const int *jcG;         /* G column pointers                        */
+    const int *irS;         /* S (Nc x Nt) CSC row indices              */
+    const int *jcS;         /* S column pointers                        */
+    const int *prS;         /* S values (integers)                      */
+    const double *tspan;    /* output times                             */
+    int tlen;
+    int *U;                 /* dense (Nn*Nc) x (Nrep*tlen) or NULL      */
+    const int *irU;         /* sparse U pattern when U == NULL          */
+    const int *jcU;
+    double *prU;
+    double *V;              /* dense (Nn*Nd) x (Nrep*tlen) or NULL      */
+    const int *irV;
+    const int *jcV;
+    double *prV;
+    const double *ldata;    /* double Nld x Nn (shared by replicates)   */
+    const double *gdata;
+    int Nn;
+    int Nc;
+    int Nt;
+    int Nd;
+    int Nld;
+    int Nrep;
+    const int *irE;         /* E (Nc x Nselect) CSC                     */
+    const int *jcE;
+    double *prE;
+    const int *N;           /* shift matrix Nc x Nshift or NULL         */
+    int len;                /* number of scheduled events               */
+    const int *event;
+    const int *time;
+    const int *node;        /* one-based                                */
+    const int *dest;        /* one-based                                */
+    const int *n;
+    const double *proportion;
+    const int *select;      /* one-based                                */
+    const int *shift;       /* one-based                                */
+    int Nthread;
+    int solver;             /* 0 = ssm, 1 = aem                         */
+    int crn;
+    int rng_type;
+    unsigned long int seed;
+    TRFun *tr_fun;
+    PTSFun pts_fun;
+} SimInf_solver_args;
+
+/* ------------------------------------------------------------------ */
+/* Device plugins.                                                     */
+/* ------------------------------------------------------------------ */
+
+/* Opaque handle of a model compiled for gfx950. */
+typedef struct SimInf_hip_model SimInf_hip_model;
+
+/* Built-in device plugins, by the reference's model names: "SIR",
+ * "SEIR", "SIS", "SISe", "SISe3" (src/models/SIR.c etc.) and "SIR_mparse" (the
+ * code mparse emits for S -> beta*S*I/N -> I, I -> gamma*I -> R,
+ * N <- S+I+R).  NULL if the name is unknown. */
+const SimInf_hip_model *SimInf_hip_model_builtin(const char *name);
+
+/* Dimensions the plugin was compiled for (Nc, Nt, Nd). */
+int SimInf_hip_model_dims(const SimInf_hip_model *model,
+                          int *Nc, int *Nt, int *Nd);
+
+/* ------------------------------------------------------------------ */
+/* Blocking entry points (SimInf_internal.h:406-420, SimInf.c:383-399). */
+/* ------------------------------------------------------------------ */
+
+/* ssm: one replicate (Nrep == 1); replaces SimInf_run_solver_ssm. */
+int SimInf_hip_run_solver_ssm(SimInf_solver_args *args,
+                              const SimInf_hip_model *model);
+
+/* mssm: Nrep >= 1 replicates; replaces SimInf_run_solver_mssm. */
+int SimInf_hip_run_solver_mssm(SimInf_solver_args *args,
+                               const SimInf_hip_model *model);
+
+/* mssm with common random numbers; replaces SimInf_run_solver_mssm_crn.
+ * Every (replicate, node) has its own stream in this library, which is
+ * the property crn asks for, so it runs the same code as mssm. */
+int SimInf_hip_run_solver_mssm_crn(SimInf_solver_args *args,
+                                   const SimInf_hip_model *model);
+
+/* aem: one replicate; replaces SimInf_run_solver_aem. */
+int SimInf_hip_run_solver_aem(SimInf_solver_args *args,
+                              const SimInf_hip_model *model);
+
+/* Dispatch of SimInf.c:383-399: ssm -> mssm_crn / mssm / ssm by Nrep and
+ * crn; aem -> aem, or SIMINF_ERR_AEM_REPLICATED_MODEL if Nrep > 1. */
+int SimInf_hip_run(SimInf_solver_args *args, const SimInf_hip_model *model);
+
+/* Reference error message for a code (SimInf.c:28-97). */
+const char *SimInf_hip_error_message(int code);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident plans (no counterpart in the reference: they keep   */
+/* the inputs and trajectories in HBM between runs, for callers that   */
+/* run one model many times).                                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct SimInf_hip_plan SimInf_hip_plan;
+
+/* Validate args, build the event schedules and stage every input in
+ * device memory.  args->solver/crn select the solver as SimInf_hip_run
+ * does.  The host arrays may be freed after the call. */
+int SimInf_hip_plan_create(SimInf_hip_plan **plan,
+                           const SimInf_solver_args *args,
+                           const SimInf_hip_model *model);
+
+/* Run the plan with a seed; blocks until the kernels have finished.  The
+ * trajectory stays in device memory. */
+int SimInf_hip_plan_run(SimInf_hip_plan *plan, unsigned long int seed);
+
+/* Copy the last run's trajectory to host buffers shaped like
+ * args->U / args->V (or prU / prV for sparse output).  NULL skips. */
+int SimInf_hip_plan_download(SimInf_hip_plan *plan, int *U, double *V,
+                             double *prU, double *prV);
+
+/* Statistics of the last run: fired transitions (SSA events and aem
+ * firings), node-days simulated and the device time of the solver
+ * kernels in milliseconds (HIP events on the plan's stream). */
+int SimInf_hip_plan_stats(const SimInf_hip_plan *plan,
+                          long long *transitions, long long *node_days,
+                          double *kernel_ms);
+
+void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMINF_HIP_H */

@@ -1,0 +1,140 @@
+"""ctypes bridge to the CPU checker.  TEST INFRASTRUCTURE ONLY: imported
+by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg."""
+import ctypes
+import os
+
+import numpy as np
+
+from siminf_amd import _native
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libsiminf_oracle.so")
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from . import build
+            build.build()
+        lib = ctypes.CDLL(LIB)
+        ap = ctypes.POINTER(_native.SolverArgs)
+        for name in ("oracle_run", "oracle_run_solver_ssm", "oracle_run_solver_mssm",
+                     "oracle_run_solver_mssm_crn", "oracle_run_solver_aem"):
+            f = getattr(lib, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [ap]
+        lib.oracle_set_model.restype = ctypes.c_int
+        lib.oracle_set_model.argtypes = [ap, ctypes.c_char_p]
+        lib.oracle_last_transitions.restype = ctypes.c_longlong
+        lib.oracle_aem_ties.restype = ctypes.c_longlong
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        lib.oracle_philox.argtypes = [u32p, u32p, u32p]
+        lib.oracle_uniforms.argtypes = [ctypes.c_ulonglong, ctypes.c_uint,
+                                        ctypes.c_ulonglong, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.oracle_log.argtypes = [dp, dp, ctypes.c_longlong]
+        lib.oracle_sqrt.argtypes = [dp, dp, ctypes.c_longlong]
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib.oracle_binomial.argtypes = [ctypes.c_ulonglong, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, ip]
+        lib.oracle_hypergeometric.argtypes = [ctypes.c_ulonglong, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ip]
+        lib.oracle_sample.restype = ctypes.c_int
+        lib.oracle_sample.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ip,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_ulonglong, ctypes.c_int, ip]
+        _lib = lib
+    return _lib
+
+
+class OracleResult:
+    def __init__(self, err, bundle, transitions, ties):
+        self.err = err
+        self.U = bundle.U
+        self.V = bundle.V
+        self.prU = bundle.prU
+        self.prV = bundle.prV
+        self.transitions = transitions
+        self.aem_ties = ties
+
+
+def run(model, solver="ssm", seed=0, crn=False):
+    """Run the checker on the same SimInf_solver_args the GPU path gets."""
+    lib = load()
+    b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
+                           crn=1 if crn else 0, seed=int(seed))
+    if lib.oracle_set_model(ctypes.byref(b.args), model.plugin.encode()):
+        raise ValueError("no host plugin for %s" % model.plugin)
+    err = lib.oracle_run(ctypes.byref(b.args))
+    return OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
+
+
+def uniforms(seed, domain, sid, count):
+    out = np.zeros(count, dtype=np.float64)
+    load().oracle_uniforms(seed, domain, sid, count,
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    load().oracle_philox(c, k, o)
+    return list(o)
+
+
+def log(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    dp = ctypes.POINTER(ctypes.c_double)
+    load().oracle_log(x.ctypes.data_as(dp), y.ctypes.data_as(dp), x.size)
+    return y
+
+
+def sqrt(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    dp = ctypes.POINTER(ctypes.c_double)
+    load().oracle_sqrt(x.ctypes.data_as(dp), y.ctypes.data_as(dp), x.size)
+    return y
+
+
+def binomial(seed, p, n, count):
+    out = np.zeros(count, dtype=np.int32)
+    load().oracle_binomial(seed, p, n, count,
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out
+
+
+def hypergeometric(seed, n1, n2, t, count):
+    out = np.zeros(count, dtype=np.int32)
+    load().oracle_hypergeometric(seed, n1, n2, t, count,
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out
+
+
+def sample(E, u_node, select, n, proportion, seed, count, enter=False):
+    """Repeated sample_select / sample_select_enter on one node; returns
+    (error code, count x Nc array).  select is zero-based."""
+    import scipy.sparse as sp
+    E = sp.csc_matrix(E)
+    E.sort_indices()
+    ir = np.ascontiguousarray(E.indices, dtype=np.int32)
+    jc = np.ascontiguousarray(E.indptr, dtype=np.int32)
+    pr = np.ascontiguousarray(E.data, dtype=np.float64)
+    u = np.ascontiguousarray(u_node, dtype=np.int32)
+    Nc = E.shape[0]
+    out = np.zeros((count, Nc), dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+    err = load().oracle_sample(int(enter), ir.ctypes.data_as(ip), jc.ctypes.data_as(ip),
+                               pr.ctypes.data_as(dp), Nc, u.ctypes.data_as(ip),
+                               select, n, proportion, seed, count,
+                               out.ctypes.data_as(ip))
+    return err, out

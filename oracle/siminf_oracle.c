@@ -1,0 +1,1047 @@
+/*
+ * siminf_oracle.c -- CPU checker for the MI355X solver core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (siminf_amd/, the HIP
+ * library, bench.py's GPU leg) links, loads or calls this file; only
+ * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py do,
+ * and only as the checker.
+ *
+ * A plain-C restatement of the reference's solver core, following
+ *   src/solvers/SimInf_solver_ssm.c:74-373       (ssm day loop)
+ *   src/solvers/SimInf_solver_mssm.c:81-402      (mssm replicate loop)
+ *   src/solvers/SimInf_solver_mssm_crn.c:92-416  (crn variant)
+ *   src/solvers/SimInf_solver_aem.c:61-459       (aem + calcTimes)
+ *   src/misc/binheap.c:15-105                    (indexed binary heap)
+ *   src/solvers/SimInf_solver.c:62-324, :658-950 (sampling, events,
+ *                                                 sparse output)
+ * with one thread.  Random numbers come from the injected streams of
+ * siminf_rand.h instead of GSL:
+ *   - SSA draws and E1-event draws of (replicate r, node i) use the node
+ *     stream (SIMINF_DOMAIN_NODE, r*Nn + i), in the order the reference
+ *     draws them from its thread stream when that thread owns node i;
+ *   - each external-transfer (E2) event k of replicate r uses its own
+ *     stream (SIMINF_DOMAIN_E2, r*len + k);
+ *   - aem channel (r, i, tr) uses (SIMINF_DOMAIN_AEM, (r*Nn + i)*Nt + tr)
+ *     like the reference's per-(node, transition) GSL streams
+ *     (SimInf_solver_aem.c:555-566).
+ * Per day the order is: SSA (or aem) for every node, the E1 events with
+ * time <= tt in list order, the E2 events with time <= tt in list order,
+ * the post-time-step function, then writeback; for sorted events this is
+ * the reference's order (SimInf_solver_ssm.c:144-362).
+ *
+ * Parity status: the loops are pinned by the reference's RNG-free golden
+ * vectors (tests/SEIR.R, tests/phi.R, tests/sample_select.R), invariants
+ * and closed-form distributions in tests/; the GSL samplers it replaces are
+ * "parity unpinned" against GSL itself (GSL is not in this image).
+ */
+#include "../include/siminf_hip.h"
+#include "../siminf_amd/csrc/siminf_rand.h"
+#include "../siminf_amd/csrc/siminf_models.h"
+
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+static long long oracle_transitions_count;
+
+/* ---------------------------------------------------------------- */
+/* Sampling (SimInf_solver.c:62-324).                                */
+/* ---------------------------------------------------------------- */
+
+/* Individuals to remove/move for an event; SimInf_solver.c:62-214. */
+static int oracle_sample_select(
+    const int *irE, const int *jcE, const double *prE, int Nc,
+    const int *u_node, int select, int n, double proportion,
+    int *individuals, siminf_stream *rng)
+{
+    const int k0 = jcE[select], k1 = jcE[select + 1];
+    const int Nstates = k1 - k0;
+    int Nindividuals = 0, Nkinds = 0, i;
+
+    memset(individuals, 0, (size_t)Nc * sizeof(int));
+
+    for (i = k0; i < k1; i++) {
+        const int nk = u_node[irE[i]];
+        if (nk > 0)
+            Nkinds++;
+        Nindividuals += nk;
+    }
+
+    if (n == 0) {
+        if (proportion < 0 || proportion > 1)
+            return SIMINF_ERR_INVALID_PROPORTION;
+        n = siminf_binomial(rng, proportion, Nindividuals);
+    }
+
+    if (Nstates <= 0 || n > Nindividuals || n < 0)
+        return SIMINF_ERR_SAMPLE_SELECT;
+
+    if (n == 0)
+        return 0;
+    if (Nindividuals == n) {
+        for (i = k0; i < k1; i++)
+            individuals[irE[i]] = u_node[irE[i]];
+        return 0;
+    }
+    if (Nstates == 1) {
+        individuals[irE[k0]] = n;
+        return 0;
+    }
+    if (Nkinds == 1) {
+        for (i = k0; i < k1; i++) {
+            if (u_node[irE[i]] > 0) {
+                individuals[irE[i]] = n;
+                break;
+            }
+        }
+        return 0;
+    }
+
+    /* Equal weights: multivariate hypergeometric by conditional
+     * univariate draws over the column's rows in order. */
+    {
+        int equal = 1;
+        for (i = k0 + 1; i < k1; i++) {
+            if (prE[i] != prE[i - 1]) {
+                equal = 0;
+                break;
+            }
+        }
+        if (equal) {
+            for (i = k0; i < k1 - 1; i++) {
+                const int nk = u_node[irE[i]];
+                if (n == 0)
+                    break;
+                individuals[irE[i]] =
+                    siminf_hypergeometric(rng, nk, Nindividuals - nk, n);
+                Nindividuals -= nk;
+                n -= individuals[irE[i]];
+            }
+            individuals[irE[i]] = n;
+            return 0;
+        }
+    }
+
+    /* Unequal weights: Wallenius urn, one individual per draw. */
+    while (n > 0) {
+        double cum = 0.0, rnd;
+        for (i = k0; i < k1; i++)
+            cum += prE[i] * (u_node[irE[i]] - individuals[irE[i]]);
+        rnd = siminf_uniform(rng) * cum;
+        i = k0;
+        cum = prE[i] * (u_node[irE[i]] - individuals[irE[i]]);
+        while (i < k1 && rnd > cum) {
+            i++;
+            if (i < k1)
+                cum += prE[i] * (u_node[irE[i]] - individuals[irE[i]]);
+        }
+        if (i >= k1)
+            i = k1 - 1;
+        if (prE[i] * (u_node[irE[i]] - individuals[irE[i]]) == 0.0) {
+            while (i > k0 &&
+                   prE[i] * (u_node[irE[i]] - individuals[irE[i]]) == 0.0)
+                i--;
+            if (prE[i] * (u_node[irE[i]] - individuals[irE[i]]) == 0.0)
+                return SIMINF_ERR_SAMPLE_SELECT;
+        }
+        individuals[irE[i]] += 1;
+        n--;
+    }
+    return 0;
+}
+
+/* Individuals to enter a node; SimInf_solver.c:243-324. */
+static int oracle_sample_select_enter(
+    const int *irE, const int *jcE, const double *prE, int Nc,
+    const int *u_node, int select, int n, double proportion,
+    int *individuals, siminf_stream *rng)
+{
+    const int k0 = jcE[select], k1 = jcE[select + 1];
+    const int Nstates = k1 - k0;
+    double w_cum = 0.0;
+    int i;
+
+    memset(individuals, 0, (size_t)Nc * sizeof(int));
+
+    if (n == 0) {
+        if (proportion < 0 || proportion > 1)
+            return SIMINF_ERR_INVALID_PROPORTION;
+        for (i = k0; i < k1; i++)
+            n += u_node[irE[i]];
+        n = siminf_binomial(rng, proportion, n);
+    }
+
+    if (Nstates <= 0 || n < 0)
+        return SIMINF_ERR_SAMPLE_SELECT;
+
+    if (n == 0)
+        return 0;
+    if (Nstates == 1) {
+        individuals[irE[k0]] = n;
+        return 0;
+    }
+
+    for (i = k0; i < k1; i++)
+        w_cum += prE[i];
+
+    while (n > 0) {
+        const double rnd = siminf_uniform(rng) * w_cum;
+        double cum;
+        i = k0;
+        cum = prE[i];
+        while (i < k1 && rnd > cum) {
+            i++;
+            if (i < k1)
+                cum += prE[i];
+        }
+        if (i >= k1)
+            i = k1 - 1;
+        if (prE[i] == 0.0) {
+            while (i > k0 && prE[i] == 0.0)
+                i--;
+            if (prE[i] == 0.0)
+                return SIMINF_ERR_SAMPLE_SELECT;
+        }
+        individuals[irE[i]] += 1;
+        n--;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- */
+/* Event processing (SimInf_solver.c:658-919).                       */
+/* ---------------------------------------------------------------- */
+
+typedef struct oracle_ctx {
+    SimInf_solver_args *a;
+    int Nn, Nc, Nt, Nd, Nld;
+    int *individuals;
+} oracle_ctx;
+
+/* Apply event k (zero-based fields derived on the fly) to the replicate
+ * state u (Nn x Nc).  Returns 0 or an error code. */
+static int oracle_apply_event(
+    oracle_ctx *c, int k, int *u, int *update_node, siminf_stream *rng)
+{
+    const SimInf_solver_args *a = c->a;
+    const int Nc = c->Nc;
+    const int type = a->event[k];
+    const int node = a->node[k] - 1;
+    const int dest = a->dest[k] - 1;
+    const int select = a->select[k] - 1;
+    const int shift = a->shift[k] - 1;
+    const int n = a->n[k];
+    const double proportion = a->proportion[k];
+    const int *irE = a->irE, *jcE = a->jcE;
+    const double *prE = a->prE;
+    int *ind = c->individuals;
+    int err, i;
+
+    if (node < 0 || node >= c->Nn)
+        return SIMINF_ERR_NODE_OUT_OF_BOUNDS;
+
+    switch (type) {
+    case SIMINF_EXIT_EVENT:
+        err = oracle_sample_select(irE, jcE, prE, Nc, &u[node * Nc], select,
+                                   n, proportion, ind, rng);
+        if (err)
+            return err;
+        for (i = jcE[select]; i < jcE[select + 1]; i++) {
+            const int jj = irE[i];
+            u[node * Nc + jj] -= ind[jj];
+            if (u[node * Nc + jj] < 0)
+                return SIMINF_ERR_NEGATIVE_STATE;
+        }
+        break;
+
+    case SIMINF_ENTER_EVENT:
+        err = oracle_sample_select_enter(irE, jcE, prE, Nc, &u[node * Nc],
+                                         select, n, proportion, ind, rng);
+        if (err)
+            return err;
+        for (i = jcE[select]; i < jcE[select + 1]; i++) {
+            const int jj = irE[i];
+            const int kn = node * Nc + jj;
+            if (shift < 0) {
+                u[kn] += ind[jj];
+                if (u[kn] < 0)
+                    return SIMINF_ERR_NEGATIVE_STATE;
+            } else if (!a->N) {
+                return SIMINF_ERR_EVENTS_N;
+            } else {
+                const int ll = a->N[shift * Nc + jj];
+                if (jj + ll < 0 || jj + ll >= Nc)
+                    return SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
+                u[kn + ll] += ind[jj];
+                if (u[kn + ll] < 0)
+                    return SIMINF_ERR_NEGATIVE_STATE;
+            }
+        }
+        break;
+
+    case SIMINF_INTERNAL_TRANSFER_EVENT:
+        if (!a->N)
+            return SIMINF_ERR_EVENTS_N;
+        if (shift < 0)
+            return SIMINF_ERR_EVENT_SHIFT;
+        err = oracle_sample_select(irE, jcE, prE, Nc, &u[node * Nc], select,
+                                   n, proportion, ind, rng);
+        if (err)
+            return err;
+        for (i = jcE[select]; i < jcE[select + 1]; i++) {
+            const int jj = irE[i];
+            const int kn = node * Nc + jj;
+            const int ll = a->N[shift * Nc + jj];
+            if (jj + ll < 0 || jj + ll >= Nc)
+                return SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
+            u[kn + ll] += ind[jj];
+            if (u[kn + ll] < 0)
+                return SIMINF_ERR_NEGATIVE_STATE;
+            u[kn] -= ind[jj];
+            if (u[kn] < 0)
+                return SIMINF_ERR_NEGATIVE_STATE;
+        }
+        break;
+
+    case SIMINF_EXTERNAL_TRANSFER_EVENT:
+        if (dest < 0 || dest >= c->Nn)
+            return SIMINF_ERR_DEST_OUT_OF_BOUNDS;
+        err = oracle_sample_select(irE, jcE, prE, Nc, &u[node * Nc], select,
+                                   n, proportion, ind, rng);
+        if (err)
+            return err;
+        for (i = jcE[select]; i < jcE[select + 1]; i++) {
+            const int jj = irE[i];
+            const int kd = dest * Nc + jj;
+            const int kn = node * Nc + jj;
+            if (shift < 0) {
+                u[kd] += ind[jj];
+                if (u[kd] < 0)
+                    return SIMINF_ERR_NEGATIVE_STATE;
+            } else if (!a->N) {
+                return SIMINF_ERR_EVENTS_N;
+            } else {
+                const int ll = a->N[shift * Nc + jj];
+                if (jj + ll < 0 || jj + ll >= Nc)
+                    return SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
+                u[kd + ll] += ind[jj];
+                if (u[kd + ll] < 0)
+                    return SIMINF_ERR_NEGATIVE_STATE;
+            }
+            u[kn] -= ind[jj];
+            if (u[kn] < 0)
+                return SIMINF_ERR_NEGATIVE_STATE;
+        }
+        update_node[dest] = 1;
+        break;
+
+    default:
+        return SIMINF_UNDEFINED_EVENT;
+    }
+
+    update_node[node] = 1;
+    return 0;
+}
+
+/* Process the events of replicate rep due at tt: first the E1 events in
+ * list order (each drawing from its node's stream), then the E2 events
+ * in list order (each drawing from its own stream).  *cursor1 / *cursor2
+ * are the next unprocessed positions in the E1 / E2 subsequences. */
+static int oracle_process_events(
+    oracle_ctx *c, int rep, double tt, int *u, int *update_node,
+    siminf_stream *node_rng, int *cursor1, int *cursor2)
+{
+    const SimInf_solver_args *a = c->a;
+    int k, err;
+
+    for (k = *cursor1; k < a->len; k++) {
+        if (a->event[k] == SIMINF_EXTERNAL_TRANSFER_EVENT)
+            continue;
+        if (a->time[k] > tt)
+            break;
+        {
+            const int node = a->node[k] - 1;
+            siminf_stream *rng = (node >= 0 && node < c->Nn) ?
+                &node_rng[node] : &node_rng[0];
+            err = oracle_apply_event(c, k, u, update_node, rng);
+            if (err) {
+                *cursor1 = k;
+                return err;
+            }
+        }
+    }
+    *cursor1 = k;
+
+    for (k = *cursor2; k < a->len; k++) {
+        siminf_stream rng;
+        if (a->event[k] != SIMINF_EXTERNAL_TRANSFER_EVENT)
+            continue;
+        if (a->time[k] > tt)
+            break;
+        rng = siminf_stream_make(a->seed, SIMINF_DOMAIN_E2,
+                                 (uint64_t)rep * (uint64_t)a->len +
+                                 (uint64_t)k);
+        err = oracle_apply_event(c, k, u, update_node, &rng);
+        if (err) {
+            *cursor2 = k;
+            return err;
+        }
+    }
+    *cursor2 = k;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- */
+/* Writeback (SimInf_solver_ssm.c:318-334, mssm.c:323-371,            */
+/* SimInf_solver.c:930-950).                                          */
+/* ---------------------------------------------------------------- */
+
+static void oracle_store(
+    oracle_ctx *c, int rep, double tt, const int *u, const double *v_new,
+    int *U_it, int *V_it)
+{
+    SimInf_solver_args *a = c->a;
+    const ptrdiff_t Nn = c->Nn, Nc = c->Nc, Nd = c->Nd, tlen = a->tlen;
+
+    while (*U_it < a->tlen && tt > a->tspan[*U_it]) {
+        if (a->U) {
+            memcpy(&a->U[(ptrdiff_t)(rep * tlen + *U_it) * Nn * Nc], u,
+                   (size_t)(Nn * Nc) * sizeof(int));
+        } else {
+            const ptrdiff_t col = rep * tlen + *U_it;
+            for (int j = a->jcU[col]; j < a->jcU[col + 1]; j++)
+                a->prU[j] = u[a->irU[j]];
+        }
+        (*U_it)++;
+    }
+
+    while (*V_it < a->tlen && tt > a->tspan[*V_it]) {
+        if (a->V) {
+            if (Nn * Nd > 0)
+                memcpy(&a->V[(ptrdiff_t)(rep * tlen + *V_it) * Nn * Nd], v_new,
+                       (size_t)(Nn * Nd) * sizeof(double));
+        } else {
+            const ptrdiff_t col = rep * tlen + *V_it;
+            for (int j = a->jcV[col]; j < a->jcV[col + 1]; j++)
+                a->prV[j] = v_new[a->irV[j]];
+        }
+        (*V_it)++;
+    }
+}
+
+static int oracle_rate_ok(double rate)
+{
+    return siminf_isfinite(rate) && !(rate < 0.0);
+}
+
+/* ---------------------------------------------------------------- */
+/* Direct-method day loop for one replicate (ssm.c:74-373 with one     */
+/* thread, mssm.c:105-383 for one replicate).                          */
+/* ---------------------------------------------------------------- */
+
+static int oracle_direct_replicate(oracle_ctx *c, int rep, int is_mssm)
+{
+    SimInf_solver_args *a = c->a;
+    const int Nn = c->Nn, Nc = c->Nc, Nt = c->Nt, Nd = c->Nd, Nld = c->Nld;
+    int *u = malloc((size_t)Nn * Nc * sizeof(int) + 1);
+    double *v = malloc((size_t)Nn * Nd * sizeof(double) + 8);
+    double *v_new = malloc((size_t)Nn * Nd * sizeof(double) + 8);
+    double *t_rate = malloc((size_t)Nn * Nt * sizeof(double) + 8);
+    double *sum_t_rate = malloc((size_t)Nn * sizeof(double) + 8);
+    double *t_time = malloc((size_t)Nn * sizeof(double) + 8);
+    int *update_node = calloc((size_t)Nn + 1, sizeof(int));
+    siminf_stream *rng = malloc((size_t)Nn * sizeof(siminf_stream) + 1);
+    int error = 0, U_it = 0, V_it = 0, cursor1 = 0, cursor2 = 0;
+    double tt = a->tspan[0];
+    double next_unit_of_time = floor(tt) + 1.0;
+
+    memcpy(u, &a->u0[(ptrdiff_t)rep * Nn * Nc], (size_t)Nn * Nc * sizeof(int));
+    if (Nd > 0) {
+        memcpy(v, &a->v0[(ptrdiff_t)rep * Nn * Nd], (size_t)Nn * Nd * sizeof(double));
+        memcpy(v_new, v, (size_t)Nn * Nd * sizeof(double));
+    }
+    for (int node = 0; node < Nn; node++)
+        rng[node] = siminf_stream_make(a->seed, SIMINF_DOMAIN_NODE,
+                                       (uint64_t)rep * (uint64_t)Nn +
+                                       (uint64_t)node);
+
+    /* Initial rates (ssm.c:96-117). */
+    for (int node = 0; node < Nn && (!is_mssm || !error); node++) {
+        sum_t_rate[node] = 0.0;
+        for (int j = 0; j < Nt; j++) {
+            const double rate = a->tr_fun[j](&u[node * Nc], &v[node * Nd],
+                                             &a->ldata[node * Nld], a->gdata,
+                                             tt);
+            t_rate[node * Nt + j] = rate;
+            sum_t_rate[node] += rate;
+            if (!oracle_rate_ok(rate))
+                error = SIMINF_ERR_INVALID_RATE;
+        }
+        t_time[node] = tt;
+    }
+
+    while (!error) {
+        /* (1) SSA per node (ssm.c:144-235). */
+        for (int node = 0; node < Nn && !error; node++) {
+            for (;;) {
+                double cum, rnd, tau, delta = 0.0;
+                int tr;
+
+                if (sum_t_rate[node] <= 0.0) {
+                    t_time[node] = next_unit_of_time;
+                    break;
+                }
+                tau = -siminf_log(siminf_uniform(&rng[node])) / sum_t_rate[node];
+                if ((tau + t_time[node]) >= next_unit_of_time) {
+                    t_time[node] = next_unit_of_time;
+                    break;
+                }
+                t_time[node] += tau;
+
+                rnd = siminf_uniform(&rng[node]) * sum_t_rate[node];
+                tr = 0;
+                cum = t_rate[node * Nt];
+                while (tr < Nt && rnd > cum) {
+                    tr++;
+                    if (tr < Nt)
+                        cum += t_rate[node * Nt + tr];
+                }
+
+                if (tr >= Nt)
+                    tr = Nt - 1;
+                if (t_rate[node * Nt + tr] == 0.0) {
+                    while (tr > 0 && t_rate[node * Nt + tr] == 0.0)
+                        tr--;
+                    if (t_rate[node * Nt + tr] == 0.0) {
+                        sum_t_rate[node] = 0.0;
+                        break;
+                    }
+                }
+
+                oracle_transitions_count++;
+
+                for (int j = a->jcS[tr]; j < a->jcS[tr + 1]; j++) {
+                    u[node * Nc + a->irS[j]] += a->prS[j];
+                    if (u[node * Nc + a->irS[j]] < 0)
+                        error = SIMINF_ERR_NEGATIVE_STATE;
+                }
+
+                for (int j = a->jcG[tr]; j < a->jcG[tr + 1]; j++) {
+                    const int k = a->irG[j];
+                    const double old = t_rate[node * Nt + k];
+                    const double rate = a->tr_fun[k](&u[node * Nc], &v[node * Nd],
+                                                     &a->ldata[node * Nld],
+                                                     a->gdata, t_time[node]);
+                    t_rate[node * Nt + k] = rate;
+                    delta += rate - old;
+                    if (!oracle_rate_ok(rate))
+                        error = SIMINF_ERR_INVALID_RATE;
+                }
+                sum_t_rate[node] += delta;
+
+                /* The node stops at its first failing step; the
+                 * reference keeps drawing until the day ends. */
+                if (error)
+                    break;
+            }
+        }
+
+        /* (2, 3) Scheduled events. */
+        if (!error)
+            error = oracle_process_events(c, rep, tt, u, update_node, rng,
+                                          &cursor1, &cursor2);
+
+        /* (4) Post time step and rate refresh (ssm.c:262-302). */
+        for (int node = 0; node < Nn && (!is_mssm || !error); node++) {
+            const int rc = a->pts_fun(&v_new[node * Nd], &u[node * Nc],
+                                      &v[node * Nd], &a->ldata[node * Nld],
+                                      a->gdata, node, tt);
+            if (rc < 0) {
+                error = rc;
+                break;
+            } else if (rc > 0 || update_node[node]) {
+                double delta = 0.0;
+                for (int j = 0; j < Nt; j++) {
+                    const double old = t_rate[node * Nt + j];
+                    const double rate = a->tr_fun[j](&u[node * Nc],
+                                                     &v_new[node * Nd],
+                                                     &a->ldata[node * Nld],
+                                                     a->gdata, tt);
+                    t_rate[node * Nt + j] = rate;
+                    delta += rate - old;
+                    if (!oracle_rate_ok(rate))
+                        error = SIMINF_ERR_INVALID_RATE;
+                }
+                sum_t_rate[node] += delta;
+                update_node[node] = 0;
+            }
+        }
+
+        /* (5) Advance time. */
+        tt = next_unit_of_time;
+        next_unit_of_time += 1.0;
+
+        /* (6) Writeback, then swap v and v_new. */
+        oracle_store(c, rep, tt, u, v_new, &U_it, &V_it);
+        {
+            double *tmp = v;
+            v = v_new;
+            v_new = tmp;
+        }
+
+        if (U_it >= a->tlen)
+            break;
+    }
+
+    free(u); free(v); free(v_new); free(t_rate); free(sum_t_rate);
+    free(t_time); free(update_node); free(rng);
+    return error;
+}
+
+/* ---------------------------------------------------------------- */
+/* aem (SimInf_solver_aem.c:61-459, binheap.c:15-105).                */
+/* ---------------------------------------------------------------- */
+
+static long long oracle_aem_finite_ties;
+
+static void heap_percolate_down(int n1, double *data, int *INDEX, int *INDEX2,
+                                int N)
+{
+    int child, node = n1;
+    const double key = data[node];
+    const int j = INDEX[node];
+    while ((child = (node << 1) + 1) < N) {
+        if (child != N - 1 && data[child + 1] < data[child])
+            child++;
+        if (data[child] < key) {
+            const int rxn = INDEX[child];
+            data[node] = data[child];
+            INDEX2[rxn] = node;
+            INDEX[node] = rxn;
+        } else {
+            break;
+        }
+        node = child;
+    }
+    data[node] = key;
+    INDEX[node] = j;
+    INDEX2[j] = node;
+}
+
+static void heap_percolate_up(int node, double *data, int *INDEX, int *INDEX2)
+{
+    int parent;
+    const double key = data[node];
+    const int j = INDEX[node];
+    do {
+        parent = (node - 1) >> 1;
+        if (key < data[parent]) {
+            const int rxn = INDEX[parent];
+            data[node] = data[parent];
+            INDEX2[rxn] = node;
+            INDEX[node] = rxn;
+        } else {
+            break;
+        }
+        node = parent;
+    } while (parent > 0);
+    data[node] = key;
+    INDEX[node] = j;
+    INDEX2[j] = node;
+}
+
+static void heap_initialize(double *data, int *INDEX, int *INDEX2, int N)
+{
+    for (int i = (N - 1) >> 1; i >= 0; i--)
+        heap_percolate_down(i, data, INDEX, INDEX2, N);
+}
+
+static void heap_update(int node, double *data, int *INDEX, int *INDEX2, int N)
+{
+    const int parent = (node - 1) >> 1;
+    if (node > 0 && data[node] < data[parent])
+        heap_percolate_up(node, data, INDEX, INDEX2);
+    else
+        heap_percolate_down(node, data, INDEX, INDEX2, N);
+}
+
+/* calcTimes (SimInf_solver_aem.c:61-86). */
+static void oracle_calc_times(double *time, double *infTime, double tt,
+                              double old_rate, double new_rate,
+                              siminf_stream *rng)
+{
+    const double oldtime = time[0];
+    if (isinf(oldtime)) {
+        if (infTime[0] == 0.0)
+            time[0] = -siminf_log(siminf_uniform(rng)) / new_rate + tt;
+        else if (new_rate > 0.0)
+            time[0] = tt + (infTime[0] / new_rate);
+    } else if (new_rate >= 2.2250738585072014e-308) {
+        if (oldtime == tt)
+            time[0] = -siminf_log(siminf_uniform(rng)) / new_rate + tt;
+        else
+            time[0] = ((old_rate / new_rate) * (oldtime - tt)) + tt;
+    } else {
+        infTime[0] = (oldtime - tt) * old_rate;
+        time[0] = INFINITY;
+    }
+}
+
+/* Count exact ties among the finite next-reaction times of a node: the
+ * heap and a first-index argmin may then pick different channels. */
+static void oracle_count_ties(const double *times, int Nt)
+{
+    for (int i = 0; i < Nt; i++)
+        for (int j = i + 1; j < Nt; j++)
+            if (!isinf(times[i]) && times[i] == times[j])
+                oracle_aem_finite_ties++;
+}
+
+static int oracle_aem(oracle_ctx *c)
+{
+    SimInf_solver_args *a = c->a;
+    const int Nn = c->Nn, Nc = c->Nc, Nt = c->Nt, Nd = c->Nd, Nld = c->Nld;
+    int *u = malloc((size_t)Nn * Nc * sizeof(int) + 1);
+    double *v = malloc((size_t)Nn * Nd * sizeof(double) + 8);
+    double *v_new = malloc((size_t)Nn * Nd * sizeof(double) + 8);
+    double *t_rate = malloc((size_t)Nn * Nt * sizeof(double) + 8);
+    double *t_time = malloc((size_t)Nn * sizeof(double) + 8);
+    int *update_node = calloc((size_t)Nn + 1, sizeof(int));
+    siminf_stream *node_rng = malloc((size_t)Nn * sizeof(siminf_stream) + 1);
+    siminf_stream *rng_vec = malloc((size_t)Nn * Nt * sizeof(siminf_stream) + 1);
+    int *reactHeap = malloc((size_t)Nn * Nt * sizeof(int) + 1);
+    int *reactNode = malloc((size_t)Nn * Nt * sizeof(int) + 1);
+    double *reactTimes = malloc((size_t)Nn * Nt * sizeof(double) + 8);
+    double *reactInf = calloc((size_t)Nn * Nt + 1, sizeof(double));
+    int error = 0, U_it = 0, V_it = 0, cursor1 = 0, cursor2 = 0;
+    double tt = a->tspan[0];
+    double next_unit_of_time = floor(tt) + 1.0;
+
+    memcpy(u, a->u0, (size_t)Nn * Nc * sizeof(int));
+    if (Nd > 0) {
+        memcpy(v, a->v0, (size_t)Nn * Nd * sizeof(double));
+        memcpy(v_new, v, (size_t)Nn * Nd * sizeof(double));
+    }
+    for (int node = 0; node < Nn; node++) {
+        node_rng[node] = siminf_stream_make(a->seed, SIMINF_DOMAIN_NODE,
+                                            (uint64_t)node);
+        for (int j = 0; j < Nt; j++)
+            rng_vec[node * Nt + j] =
+                siminf_stream_make(a->seed, SIMINF_DOMAIN_AEM,
+                                   (uint64_t)node * (uint64_t)Nt + (uint64_t)j);
+    }
+
+    /* Initial rates and reaction times (aem.c:117-153). */
+    for (int node = 0; node < Nn; node++) {
+        for (int j = 0; j < Nt; j++) {
+            const double rate = a->tr_fun[j](&u[node * Nc], &v[node * Nd],
+                                             &a->ldata[node * Nld], a->gdata, tt);
+            t_rate[node * Nt + j] = rate;
+            if (!oracle_rate_ok(rate))
+                error = SIMINF_ERR_INVALID_RATE;
+            reactTimes[Nt * node + j] =
+                -siminf_log(siminf_uniform(&rng_vec[Nt * node + j])) / rate + tt;
+            if (reactTimes[Nt * node + j] <= 0.0)
+                reactTimes[Nt * node + j] = INFINITY;
+            reactHeap[Nt * node + j] = reactNode[Nt * node + j] = j;
+        }
+        heap_initialize(&reactTimes[Nt * node], &reactNode[Nt * node],
+                        &reactHeap[Nt * node], Nt);
+        t_time[node] = tt;
+    }
+
+    while (!error) {
+        for (int node = 0; node < Nn && !error; node++) {
+            for (;;) {
+                int tr;
+                double old_t_rate, rate;
+
+                oracle_count_ties(&reactTimes[Nt * node], Nt);
+                t_time[node] = reactTimes[Nt * node];
+                if (isinf(t_time[node]) || t_time[node] >= next_unit_of_time) {
+                    t_time[node] = next_unit_of_time;
+                    break;
+                }
+
+                tr = reactNode[Nt * node] % Nt;
+                oracle_transitions_count++;
+
+                for (int j = a->jcS[tr]; j < a->jcS[tr + 1]; j++) {
+                    u[node * Nc + a->irS[j]] += a->prS[j];
+                    if (u[node * Nc + a->irS[j]] < 0)
+                        error = SIMINF_ERR_NEGATIVE_STATE;
+                }
+
+                for (int ii = a->jcG[tr]; ii < a->jcG[tr + 1]; ii++) {
+                    const int j = a->irG[ii];
+                    if (j == tr)
+                        continue;
+                    old_t_rate = t_rate[node * Nt + j];
+                    rate = a->tr_fun[j](&u[node * Nc], &v[node * Nd],
+                                        &a->ldata[node * Nld], a->gdata,
+                                        t_time[node]);
+                    t_rate[node * Nt + j] = rate;
+                    if (!oracle_rate_ok(rate))
+                        error = SIMINF_ERR_INVALID_RATE;
+                    oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + j]],
+                                      &reactInf[Nt * node + j], t_time[node],
+                                      old_t_rate, t_rate[node * Nt + j],
+                                      &rng_vec[Nt * node + j]);
+                    heap_update(reactHeap[Nt * node + j], &reactTimes[Nt * node],
+                                &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+                }
+
+                old_t_rate = t_rate[node * Nt + tr];
+                rate = a->tr_fun[tr](&u[node * Nc], &v[node * Nd],
+                                     &a->ldata[node * Nld], a->gdata,
+                                     t_time[node]);
+                t_rate[node * Nt + tr] = rate;
+                if (!oracle_rate_ok(rate))
+                    error = SIMINF_ERR_INVALID_RATE;
+                oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + tr]],
+                                  &reactInf[Nt * node + tr], t_time[node],
+                                  old_t_rate, t_rate[node * Nt + tr],
+                                  &rng_vec[Nt * node + tr]);
+                heap_update(reactHeap[Nt * node + tr], &reactTimes[Nt * node],
+                            &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+                if (error)
+                    break;
+            }
+        }
+
+        if (!error)
+            error = oracle_process_events(c, 0, tt, u, update_node, node_rng,
+                                          &cursor1, &cursor2);
+
+        for (int node = 0; node < Nn; node++) {
+            const int rc = a->pts_fun(&v_new[node * Nd], &u[node * Nc],
+                                      &v[node * Nd], &a->ldata[node * Nld],
+                                      a->gdata, node, tt);
+            if (rc < 0) {
+                error = rc;
+                break;
+            } else if (rc > 0 || update_node[node]) {
+                for (int j = 0; j < Nt; j++) {
+                    const double old = t_rate[node * Nt + j];
+                    const double rate = a->tr_fun[j](&u[node * Nc],
+                                                     &v_new[node * Nd],
+                                                     &a->ldata[node * Nld],
+                                                     a->gdata, tt);
+                    t_rate[node * Nt + j] = rate;
+                    if (!oracle_rate_ok(rate))
+                        error = SIMINF_ERR_INVALID_RATE;
+                    oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + j]],
+                                      &reactInf[Nt * node + j], t_time[node],
+                                      old, t_rate[node * Nt + j],
+                                      &rng_vec[Nt * node + j]);
+                    heap_update(reactHeap[Nt * node + j], &reactTimes[Nt * node],
+                                &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+                }
+                update_node[node] = 0;
+            }
+        }
+
+        tt = next_unit_of_time;
+        next_unit_of_time += 1.0;
+
+        oracle_store(c, 0, tt, u, v_new, &U_it, &V_it);
+        {
+            double *tmp = v;
+            v = v_new;
+            v_new = tmp;
+        }
+        if (U_it >= a->tlen)
+            break;
+    }
+
+    free(u); free(v); free(v_new); free(t_rate); free(t_time);
+    free(update_node); free(node_rng); free(rng_vec); free(reactHeap);
+    free(reactNode); free(reactTimes); free(reactInf);
+    return error;
+}
+
+/* ---------------------------------------------------------------- */
+/* Entry points.                                                     */
+/* ---------------------------------------------------------------- */
+
+static int oracle_ctx_init(oracle_ctx *c, SimInf_solver_args *a)
+{
+    c->a = a;
+    c->Nn = a->Nn;
+    c->Nc = a->Nc;
+    c->Nt = a->Nt;
+    c->Nd = a->Nd;
+    c->Nld = a->Nld;
+    c->individuals = calloc((size_t)a->Nc + 1, sizeof(int));
+    return c->individuals ? 0 : SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+}
+
+int oracle_run_solver_ssm(SimInf_solver_args *a)
+{
+    oracle_ctx c;
+    int err = oracle_ctx_init(&c, a);
+    oracle_transitions_count = 0;
+    if (!err)
+        err = oracle_direct_replicate(&c, 0, 0);
+    free(c.individuals);
+    return err;
+}
+
+int oracle_run_solver_mssm(SimInf_solver_args *a)
+{
+    oracle_ctx c;
+    int err = oracle_ctx_init(&c, a);
+    oracle_transitions_count = 0;
+    for (int rep = 0; rep < a->Nrep && !err; rep++)
+        err = oracle_direct_replicate(&c, rep, 1);
+    free(c.individuals);
+    return err;
+}
+
+int oracle_run_solver_mssm_crn(SimInf_solver_args *a)
+{
+    return oracle_run_solver_mssm(a);
+}
+
+int oracle_run_solver_aem(SimInf_solver_args *a)
+{
+    oracle_ctx c;
+    int err = oracle_ctx_init(&c, a);
+    oracle_transitions_count = 0;
+    oracle_aem_finite_ties = 0;
+    if (!err)
+        err = oracle_aem(&c);
+    free(c.individuals);
+    return err;
+}
+
+/* Dispatch of SimInf.c:383-399. */
+int oracle_run(SimInf_solver_args *a)
+{
+    if (a->solver == 0) {
+        if (a->Nrep > 1)
+            return a->crn == 1 ? oracle_run_solver_mssm_crn(a)
+                               : oracle_run_solver_mssm(a);
+        return oracle_run_solver_ssm(a);
+    }
+    if (a->Nrep > 1)
+        return SIMINF_ERR_AEM_REPLICATED_MODEL;
+    return oracle_run_solver_aem(a);
+}
+
+long long oracle_last_transitions(void)
+{
+    return oracle_transitions_count;
+}
+
+long long oracle_aem_ties(void)
+{
+    return oracle_aem_finite_ties;
+}
+
+/* ---------------------------------------------------------------- */
+/* Host plugins of the built-in models (the TRFun/PTSFun tables the   */
+/* reference's <Model>_run functions pass to SimInf_run).             */
+/* ---------------------------------------------------------------- */
+
+static TRFun SIR_tr[] = {SIR_S_to_I, SIR_I_to_R};
+static TRFun SEIR_tr[] = {SEIR_S_to_E, SEIR_E_to_I, SEIR_I_to_R};
+static TRFun SIS_tr[] = {SIS_S_to_I, SIS_I_to_S};
+static TRFun SISe_tr[] = {SISe_S_to_I, SISe_I_to_S};
+static TRFun SISe3_tr[] = {SISe3_S_1_to_I_1, SISe3_I_1_to_S_1,
+                           SISe3_S_2_to_I_2, SISe3_I_2_to_S_2,
+                           SISe3_S_3_to_I_3, SISe3_I_3_to_S_3};
+static TRFun SIR_mparse_tr[] = {SIR_mparse_trFun1, SIR_mparse_trFun2};
+
+/* Fill tr_fun / pts_fun for a built-in model name; 0 if found. */
+int oracle_set_model(SimInf_solver_args *a, const char *name)
+{
+    if (strcmp(name, "SIR") == 0) {
+        a->tr_fun = SIR_tr;
+        a->pts_fun = siminf_pts_noop;
+    } else if (strcmp(name, "SEIR") == 0) {
+        a->tr_fun = SEIR_tr;
+        a->pts_fun = siminf_pts_noop;
+    } else if (strcmp(name, "SIS") == 0) {
+        a->tr_fun = SIS_tr;
+        a->pts_fun = siminf_pts_noop;
+    } else if (strcmp(name, "SISe") == 0) {
+        a->tr_fun = SISe_tr;
+        a->pts_fun = SISe_pts;
+    } else if (strcmp(name, "SISe3") == 0) {
+        a->tr_fun = SISe3_tr;
+        a->pts_fun = SISe3_pts;
+    } else if (strcmp(name, "SIR_mparse") == 0) {
+        a->tr_fun = SIR_mparse_tr;
+        a->pts_fun = siminf_pts_noop;
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- */
+/* Primitive probes for the tests.                                   */
+/* ---------------------------------------------------------------- */
+
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    siminf_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], out);
+}
+
+void oracle_uniforms(unsigned long long seed, unsigned domain,
+                     unsigned long long id, int count, double *out)
+{
+    siminf_stream s = siminf_stream_make(seed, domain, id);
+    for (int i = 0; i < count; i++)
+        out[i] = siminf_uniform(&s);
+}
+
+void oracle_log(const double *x, double *y, long long count)
+{
+    for (long long i = 0; i < count; i++)
+        y[i] = siminf_log(x[i]);
+}
+
+void oracle_sqrt(const double *x, double *y, long long count)
+{
+    for (long long i = 0; i < count; i++)
+        y[i] = siminf_sqrt(x[i]);
+}
+
+void oracle_binomial(unsigned long long seed, double p, int n, int count,
+                     int *out)
+{
+    siminf_stream s = siminf_stream_make(seed, SIMINF_DOMAIN_E2, 7);
+    for (int i = 0; i < count; i++)
+        out[i] = siminf_binomial(&s, p, n);
+}
+
+void oracle_hypergeometric(unsigned long long seed, int n1, int n2, int t,
+                           int count, int *out)
+{
+    siminf_stream s = siminf_stream_make(seed, SIMINF_DOMAIN_E2, 11);
+    for (int i = 0; i < count; i++)
+        out[i] = siminf_hypergeometric(&s, n1, n2, t);
+}
+
+/* sample_select / sample_select_enter on one node's state, repeated
+ * count times from one stream; out is count x Nc. */
+int oracle_sample(int enter, const int *irE, const int *jcE, const double *prE,
+                  int Nc, const int *u_node, int select, int n,
+                  double proportion, unsigned long long seed, int count,
+                  int *out)
+{
+    siminf_stream s = siminf_stream_make(seed, SIMINF_DOMAIN_E2, 3);
+    for (int i = 0; i < count; i++) {
+        const int err = enter ?
+            oracle_sample_select_enter(irE, jcE, prE, Nc, u_node, select, n,
+                                       proportion, &out[i * Nc], &s) :
+            oracle_sample_select(irE, jcE, prE, Nc, u_node, select, n,
+                                 proportion, &out[i * Nc], &s);
+        if (err)
+            return err;
+    }
+    return 0;
+}

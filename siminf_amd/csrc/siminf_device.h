@@ -1,0 +1,866 @@
+/*
+ * siminf_device.h -- gfx950 device code of the solver core: per-node
+ * direct-method SSA and all-events-method day steps, scheduled-event
+ * application and the kernel parameter block.
+ *
+ * One lane owns one (replicate, node).  Its compartment state u[NC], its
+ * continuous state v[ND], its propensities rate[NT] and the running sum a0
+ * live in VGPRs; every index into them is a compile-time constant, so the
+ * model's rate functions inline into straight-line code.  The selected
+ * transition is dispatched through a compile-time chain of cases.  The
+ * dependency graph G and the stoichiometry S arrive at run time as
+ * per-transition bit masks and a dense value table, read with uniform
+ * (scalar) loads.
+ *
+ * Reference: src/solvers/SimInf_solver_ssm.c:144-302 (SSA step, PTS and
+ * refresh), src/solvers/SimInf_solver_aem.c:61-384 (aem), and
+ * src/solvers/SimInf_solver.c:62-324, :658-919 (sampling, events).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+
+#include "siminf_rand.h"
+#include "siminf_models.h"
+#include "../../include/siminf_hip.h"
+
+namespace siminf {
+
+/* ------------------------------------------------------------------ */
+/* Compile-time iteration.                                             */
+/* ------------------------------------------------------------------ */
+
+template <int N, class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl<N>(f, std::make_integer_sequence<int, N>{});
+}
+
+/* ------------------------------------------------------------------ */
+/* Kernel parameters.                                                  */
+/* ------------------------------------------------------------------ */
+
+/* Error keys order the errors the way a one-thread reference run meets
+ * them: replicate, day, phase, position, with the code in the low bits.
+ * The smallest key over all lanes is the reported error. */
+enum Phase { PH_INIT = 0, PH_SSA = 1, PH_E1 = 2, PH_E2 = 3, PH_PTS = 4 };
+
+__host__ __device__ inline uint64_t error_key(int rep, int day, int phase,
+                                              long long pos, int code)
+{
+    const uint64_t r = rep < 0xFFFF ? (uint64_t)rep : 0xFFFFull;
+    const uint64_t d = day < 0x3FFFF ? (uint64_t)day : 0x3FFFFull;
+    const uint64_t p = pos < 0x1FFFFF ? (uint64_t)pos : 0x1FFFFFull;
+    return (r << 48) | (d << 30) | ((uint64_t)phase << 27) | (p << 6) |
+           (uint64_t)(-code & 63);
+}
+
+struct KParams {
+    /* Dimensions. */
+    int Nn, Nrep, Nld, tlen, len, Nshift;
+    long long lanes;                 /* Nn * Nrep */
+    unsigned long long seed;
+    double tt0, next0;               /* tspan[0], floor(tspan[0]) + 1 */
+
+    /* Model inputs. */
+    const int *u0;                   /* Nc x (Nn*Nrep)  */
+    const double *v0;                /* Nd x (Nn*Nrep)  */
+    const double *ldata;             /* Nld x Nn, or {INFINITY} */
+    const double *gdata;
+    const double *tspan;
+
+    /* S and G: Smask[k] has bit c set when S[c,k] is stored, Sval[k*NC+c]
+     * holds its value; Gmask[k] has bit j set when G[j,k] is stored. */
+    const uint32_t *Smask;
+    const int *Sval;
+    const uint32_t *Gmask;
+
+    /* E and N: Emask[s] has bit c set when E[c,s] is stored, Ew[s*NC+c]
+     * holds its weight; Nshift_mat is N (Nc x Nshift) or NULL. */
+    const uint32_t *Emask;
+    const double *Ew;
+    const int *Nmat;
+
+    /* Events (zero-based node/dest/select/shift), sorted by time. */
+    const int *ev_type, *ev_time, *ev_node, *ev_dest, *ev_n, *ev_select,
+        *ev_shift, *ev_rank, *ev_day;
+    const double *ev_prop;
+
+    /* E1 events grouped by node in list order: e1_idx[e1_ptr[i] ..
+     * e1_ptr[i+1]) are node i's events. */
+    const int *e1_ptr, *e1_idx;
+
+    /* E2 schedule: day d's conflict-free levels are e2_day[d] ..
+     * e2_day[d+1]; level l holds e2_idx[e2_lvl[l] .. e2_lvl[l+1]). */
+    const int *e2_day, *e2_lvl, *e2_idx;
+    int ndays;
+
+    /* Output: dense U/V or sparse (irU, jcU, prU) / (irV, jcV, prV). */
+    int *U;
+    double *V;
+    const int *irU, *jcU;
+    double *prU;
+    const int *irV, *jcV;
+    double *prV;
+
+    /* Work state for the replicate-per-workgroup kernel. */
+    int *u_work;                     /* Nc x (Nn*Nrep) when u is not in LDS */
+    double *v_work, *vnew_work;      /* Nd x (Nn*Nrep) */
+    double *rate_work;               /* Nt x (Nn*Nrep), SoA by transition */
+    double *a0_work;                 /* Nn*Nrep */
+    unsigned long long *rng_j;       /* Nn*Nrep */
+    double *rng_cache;               /* Nn*Nrep */
+    int *e1_cursor;                  /* Nn*Nrep */
+
+    /* Results. */
+    unsigned long long *err_key;
+    unsigned long long *transitions;
+};
+
+__device__ __forceinline__ void record_error(const KParams &p, int rep, int day,
+                                             int phase, long long pos, int code)
+{
+    atomicMin(p.err_key, (unsigned long long)error_key(rep, day, phase, pos, code));
+}
+
+__device__ __forceinline__ bool rate_ok(double r)
+{
+    return siminf_isfinite(r) && !(r < 0.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Register-array helpers with a run-time index.                       */
+/* ------------------------------------------------------------------ */
+
+template <int N, class T>
+__device__ __forceinline__ T sel(const T (&a)[N], int i)
+{
+    T r = a[0];
+    static_for<N>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if (k > 0 && i == k)
+            r = a[k];
+    });
+    return r;
+}
+
+template <int N, class T>
+__device__ __forceinline__ void add_at(T (&a)[N], int i, T x)
+{
+    static_for<N>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if (i == k)
+            a[k] += x;
+    });
+}
+
+/* ------------------------------------------------------------------ */
+/* Direct-method SSA for one node over [t, next) (ssm.c:144-235).      */
+/* ------------------------------------------------------------------ */
+
+template <class M>
+struct NodeState {
+    int u[M::NC];
+    double v[M::ND > 0 ? M::ND : 1];
+    double vnew[M::ND > 0 ? M::ND : 1];
+    double rate[M::NT];
+    double a0;
+    siminf_stream rng;
+};
+
+/* Fire transition K: apply S(:,K), then recompute the rates listed in
+ * G(:,K) in row order and add their changes to a0.  Returns an error
+ * code or 0; like the reference the step completes after an error. */
+template <class M, int K>
+__device__ __forceinline__ int fire(NodeState<M> &s, const KParams &p,
+                                    const double *ld, double t)
+{
+    int err = 0;
+    const uint32_t smask = p.Smask[K];
+    static_for<M::NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if (smask & (1u << c)) {
+            s.u[c] += p.Sval[K * M::NC + c];
+            if (s.u[c] < 0)
+                err = SIMINF_ERR_NEGATIVE_STATE;
+        }
+    });
+    const uint32_t gmask = p.Gmask[K];
+    double delta = 0.0;
+    static_for<M::NT>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if (gmask & (1u << j)) {
+            const double old = s.rate[j];
+            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, t);
+            s.rate[j] = r;
+            delta += r - old;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+        }
+    });
+    s.a0 += delta;
+    return err;
+}
+
+template <class M>
+__device__ __forceinline__ int ssa_day(NodeState<M> &s, const KParams &p,
+                                       const double *ld, double tt, double next,
+                                       unsigned long long &fired)
+{
+    constexpr int NT = M::NT;
+    int err = 0;
+    double t = tt;
+    for (;;) {
+        if (s.a0 <= 0.0)
+            break;
+        const double tau = -siminf_log(siminf_uniform(&s.rng)) / s.a0;
+        if ((tau + t) >= next)
+            break;
+        t += tau;
+
+        const double rnd = siminf_uniform(&s.rng) * s.a0;
+        int tr = 0;
+        double cum = s.rate[0];
+        bool searching = rnd > cum;
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if (k > 0 && searching) {
+                tr = k;
+                cum += s.rate[k];
+                searching = rnd > cum;
+            }
+        });
+        /* Floating point fix (ssm.c:170-189). */
+        if (searching)
+            tr = NT - 1;
+        if (sel(s.rate, tr) == 0.0) {
+            static_for<NT>([&](auto K) {
+                constexpr int k = NT - 1 - decltype(K)::value;
+                if (k > 0 && tr == k && s.rate[k] == 0.0)
+                    tr = k - 1;
+            });
+            if (sel(s.rate, tr) == 0.0) {
+                s.a0 = 0.0;
+                break;
+            }
+        }
+
+        fired++;
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if (tr == k) {
+                const int e = fire<M, k>(s, p, ld, t);
+                if (e)
+                    err = e;
+            }
+        });
+        /* The node stops at its first failing step. */
+        if (err)
+            break;
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------ */
+/* All-events method for one node over [tt, next) (aem.c:181-300).     */
+/* ------------------------------------------------------------------ */
+
+template <class M>
+struct AemState {
+    int u[M::NC];
+    double v[M::ND > 0 ? M::ND : 1];
+    double vnew[M::ND > 0 ? M::ND : 1];
+    double rate[M::NT];
+    double time[M::NT];     /* next firing time of each transition */
+    double inf[M::NT];      /* reactInf */
+    unsigned long long j[M::NT];
+    siminf_stream rng;      /* node stream: E1 events */
+};
+
+/* calcTimes (aem.c:61-86) for channel J. */
+template <class M, int J>
+__device__ __forceinline__ void calc_times(AemState<M> &s, const KParams &p,
+                                           uint64_t chan_base, double tt,
+                                           double old_rate, double new_rate)
+{
+    const double oldtime = s.time[J];
+    siminf_stream c = siminf_stream_make(p.seed, SIMINF_DOMAIN_AEM,
+                                         chan_base + (uint64_t)J);
+    c.j = s.j[J];
+    if (isinf(oldtime)) {
+        if (s.inf[J] == 0.0) {
+            s.time[J] = -siminf_log(siminf_uniform_at(&c, c.j)) / new_rate + tt;
+            s.j[J] = c.j + 1;
+        } else if (new_rate > 0.0) {
+            s.time[J] = tt + (s.inf[J] / new_rate);
+        }
+    } else if (new_rate >= 2.2250738585072014e-308) {
+        if (oldtime == tt) {
+            s.time[J] = -siminf_log(siminf_uniform_at(&c, c.j)) / new_rate + tt;
+            s.j[J] = c.j + 1;
+        } else {
+            s.time[J] = ((old_rate / new_rate) * (oldtime - tt)) + tt;
+        }
+    } else {
+        s.inf[J] = (oldtime - tt) * old_rate;
+        s.time[J] = INFINITY;
+    }
+}
+
+template <class M, int K>
+__device__ __forceinline__ int aem_fire(AemState<M> &s, const KParams &p,
+                                        const double *ld, double t,
+                                        uint64_t chan_base)
+{
+    int err = 0;
+    const uint32_t smask = p.Smask[K];
+    static_for<M::NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if (smask & (1u << c)) {
+            s.u[c] += p.Sval[K * M::NC + c];
+            if (s.u[c] < 0)
+                err = SIMINF_ERR_NEGATIVE_STATE;
+        }
+    });
+    const uint32_t gmask = p.Gmask[K];
+    static_for<M::NT>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if (j != K && (gmask & (1u << j))) {
+            const double old = s.rate[j];
+            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, t);
+            s.rate[j] = r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+            calc_times<M, j>(s, p, chan_base, t, old, r);
+        }
+    });
+    {
+        const double old = s.rate[K];
+        const double r = M::template rate<K>(s.u, s.v, ld, p.gdata, t);
+        s.rate[K] = r;
+        if (!rate_ok(r))
+            err = SIMINF_ERR_INVALID_RATE;
+        calc_times<M, K>(s, p, chan_base, t, old, r);
+    }
+    return err;
+}
+
+template <class M>
+__device__ __forceinline__ int aem_day(AemState<M> &s, const KParams &p,
+                                       const double *ld, double next,
+                                       uint64_t chan_base,
+                                       unsigned long long &fired)
+{
+    constexpr int NT = M::NT;
+    int err = 0;
+    for (;;) {
+        /* Earliest channel; the first index wins among exact ties. */
+        int tr = 0;
+        double t = s.time[0];
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if (k > 0 && s.time[k] < t) {
+                t = s.time[k];
+                tr = k;
+            }
+        });
+        if (isinf(t) || t >= next)
+            break;
+        fired++;
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if (tr == k) {
+                const int e = aem_fire<M, k>(s, p, ld, t, chan_base);
+                if (e)
+                    err = e;
+            }
+        });
+        if (err)
+            break;
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sampling individuals for an event (SimInf_solver.c:62-324).         */
+/* ------------------------------------------------------------------ */
+
+/* First / last stored row of an E column, as compartment indices. */
+__device__ __forceinline__ int mask_first(uint32_t m) { return __builtin_ctz(m); }
+__device__ __forceinline__ int mask_last(uint32_t m) { return 31 - __builtin_clz(m); }
+__device__ __forceinline__ int mask_prev(uint32_t m, int c)
+{
+    const uint32_t below = m & ((1u << c) - 1u);
+    return below ? 31 - __builtin_clz(below) : -1;
+}
+
+/* sample_select for a node whose state is the register array u.  The E
+ * column is (mask, w).  Fills ind and returns 0 or an error code. */
+template <int NC>
+__device__ int sample_select(const int (&u)[NC], uint32_t mask,
+                             const double (&w)[NC], int n, double proportion,
+                             int (&ind)[NC], siminf_stream *rng)
+{
+    int Nind = 0, Nkinds = 0;
+    const int Nstates = __builtin_popcount(mask);
+
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        ind[c] = 0;
+        if (mask & (1u << c)) {
+            if (u[c] > 0)
+                Nkinds++;
+            Nind += u[c];
+        }
+    });
+
+    if (n == 0) {
+        if (proportion < 0 || proportion > 1)
+            return SIMINF_ERR_INVALID_PROPORTION;
+        n = siminf_binomial(rng, proportion, Nind);
+    }
+
+    if (Nstates <= 0 || n > Nind || n < 0)
+        return SIMINF_ERR_SAMPLE_SELECT;
+
+    if (n == 0)
+        return 0;
+
+    if (Nind == n) {
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c))
+                ind[c] = u[c];
+        });
+        return 0;
+    }
+
+    if (Nstates == 1) {
+        add_at(ind, mask_first(mask), n);
+        return 0;
+    }
+
+    if (Nkinds == 1) {
+        bool done = false;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (!done && (mask & (1u << c)) && u[c] > 0) {
+                ind[c] = n;
+                done = true;
+            }
+        });
+        return 0;
+    }
+
+    /* Are all weights in the column equal? */
+    bool equal = true;
+    {
+        bool have_prev = false;
+        double prev = 0.0;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c)) {
+                if (have_prev && w[c] != prev)
+                    equal = false;
+                prev = w[c];
+                have_prev = true;
+            }
+        });
+    }
+
+    if (equal) {
+        /* Conditional hypergeometric draws over the rows in order; the
+         * row where the loop stops takes the remainder. */
+        const int last = mask_last(mask);
+        bool stopped = false;
+        int stop_row = last;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (!stopped && (mask & (1u << c)) && c != last) {
+                if (n == 0) {
+                    stopped = true;
+                    stop_row = c;
+                } else {
+                    ind[c] = siminf_hypergeometric(rng, u[c], Nind - u[c], n);
+                    Nind -= u[c];
+                    n -= ind[c];
+                }
+            }
+        });
+        add_at(ind, stop_row, n - sel(ind, stop_row));
+        return 0;
+    }
+
+    /* Wallenius urn, one individual per draw. */
+    while (n > 0) {
+        double cum = 0.0;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c))
+                cum += w[c] * (u[c] - ind[c]);
+        });
+        const double rnd = siminf_uniform(rng) * cum;
+
+        int i = -1;
+        bool searching = true, first = true;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c)) {
+                const double wc = w[c] * (u[c] - ind[c]);
+                if (first) {
+                    cum = wc;
+                    first = false;
+                    i = c;
+                    searching = rnd > cum;
+                } else if (searching) {
+                    i = c;
+                    cum += wc;
+                    searching = rnd > cum;
+                }
+            }
+        });
+
+        if (sel(w, i) * (sel(u, i) - sel(ind, i)) == 0.0) {
+            const int first_row = mask_first(mask);
+            while (i > first_row && sel(w, i) * (sel(u, i) - sel(ind, i)) == 0.0)
+                i = mask_prev(mask, i);
+            if (sel(w, i) * (sel(u, i) - sel(ind, i)) == 0.0)
+                return SIMINF_ERR_SAMPLE_SELECT;
+        }
+        add_at(ind, i, 1);
+        n--;
+    }
+    return 0;
+}
+
+/* sample_select_enter (SimInf_solver.c:243-324). */
+template <int NC>
+__device__ int sample_select_enter(const int (&u)[NC], uint32_t mask,
+                                   const double (&w)[NC], int n,
+                                   double proportion, int (&ind)[NC],
+                                   siminf_stream *rng)
+{
+    const int Nstates = __builtin_popcount(mask);
+    static_for<NC>([&](auto C) { ind[decltype(C)::value] = 0; });
+
+    if (n == 0) {
+        if (proportion < 0 || proportion > 1)
+            return SIMINF_ERR_INVALID_PROPORTION;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c))
+                n += u[c];
+        });
+        n = siminf_binomial(rng, proportion, n);
+    }
+
+    if (Nstates <= 0 || n < 0)
+        return SIMINF_ERR_SAMPLE_SELECT;
+    if (n == 0)
+        return 0;
+    if (Nstates == 1) {
+        add_at(ind, mask_first(mask), n);
+        return 0;
+    }
+
+    double w_cum = 0.0;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if (mask & (1u << c))
+            w_cum += w[c];
+    });
+
+    while (n > 0) {
+        const double rnd = siminf_uniform(rng) * w_cum;
+        int i = -1;
+        double cum = 0.0;
+        bool searching = true, first = true;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c)) {
+                if (first) {
+                    cum = w[c];
+                    first = false;
+                    i = c;
+                    searching = rnd > cum;
+                } else if (searching) {
+                    i = c;
+                    cum += w[c];
+                    searching = rnd > cum;
+                }
+            }
+        });
+        if (sel(w, i) == 0.0) {
+            const int first_row = mask_first(mask);
+            while (i > first_row && sel(w, i) == 0.0)
+                i = mask_prev(mask, i);
+            if (sel(w, i) == 0.0)
+                return SIMINF_ERR_SAMPLE_SELECT;
+        }
+        add_at(ind, i, 1);
+        n--;
+    }
+    return 0;
+}
+
+/* Load E column s as (mask, dense weights). */
+template <int NC>
+__device__ __forceinline__ uint32_t load_E(const KParams &p, int s, double (&w)[NC])
+{
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        w[c] = p.Ew[s * NC + c];
+    });
+    return p.Emask[s];
+}
+
+/* Apply an E1 event (exit, enter, internal transfer) to the register
+ * state u of its node (SimInf_solver.c:680-822).  External transfers are
+ * handled by apply_e2. */
+template <int NC>
+__device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rng)
+{
+    const int type = p.ev_type[k];
+    const int select = p.ev_select[k];
+    const int shift = p.ev_shift[k];
+    const int n = p.ev_n[k];
+    const double prop = p.ev_prop[k];
+    double w[NC];
+    int ind[NC];
+    int err;
+    const uint32_t mask = load_E<NC>(p, select, w);
+
+    if (type == SIMINF_EXIT_EVENT) {
+        err = sample_select<NC>(u, mask, w, n, prop, ind, rng);
+        if (err)
+            return err;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (!err && (mask & (1u << c))) {
+                u[c] -= ind[c];
+                if (u[c] < 0)
+                    err = SIMINF_ERR_NEGATIVE_STATE;
+            }
+        });
+        return err;
+    }
+
+    if (type == SIMINF_ENTER_EVENT) {
+        err = sample_select_enter<NC>(u, mask, w, n, prop, ind, rng);
+        if (err)
+            return err;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (!err && (mask & (1u << c))) {
+                if (shift < 0) {
+                    u[c] += ind[c];
+                    if (u[c] < 0)
+                        err = SIMINF_ERR_NEGATIVE_STATE;
+                } else if (!p.Nmat) {
+                    err = SIMINF_ERR_EVENTS_N;
+                } else {
+                    const int ll = p.Nmat[shift * NC + c];
+                    if (c + ll < 0 || c + ll >= NC) {
+                        err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
+                    } else {
+                        add_at(u, c + ll, ind[c]);
+                        if (sel(u, c + ll) < 0)
+                            err = SIMINF_ERR_NEGATIVE_STATE;
+                    }
+                }
+            }
+        });
+        return err;
+    }
+
+    if (type == SIMINF_INTERNAL_TRANSFER_EVENT) {
+        if (!p.Nmat)
+            return SIMINF_ERR_EVENTS_N;
+        if (shift < 0)
+            return SIMINF_ERR_EVENT_SHIFT;
+        err = sample_select<NC>(u, mask, w, n, prop, ind, rng);
+        if (err)
+            return err;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (!err && (mask & (1u << c))) {
+                const int ll = p.Nmat[shift * NC + c];
+                if (c + ll < 0 || c + ll >= NC) {
+                    err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
+                } else {
+                    add_at(u, c + ll, ind[c]);
+                    if (sel(u, c + ll) < 0) {
+                        err = SIMINF_ERR_NEGATIVE_STATE;
+                    } else {
+                        u[c] -= ind[c];
+                        if (u[c] < 0)
+                            err = SIMINF_ERR_NEGATIVE_STATE;
+                    }
+                }
+            }
+        });
+        return err;
+    }
+
+    return SIMINF_UNDEFINED_EVENT;
+}
+
+/* Apply an external transfer event k to the replicate state uw (Nn x NC,
+ * in LDS or global memory) (SimInf_solver.c:824-902).  upd receives the
+ * update flags of source and dest. */
+template <int NC>
+__device__ int apply_e2(const KParams &p, int k, int *uw, int *upd,
+                        siminf_stream *rng)
+{
+    const int node = p.ev_node[k];
+    const int dest = p.ev_dest[k];
+    const int select = p.ev_select[k];
+    const int shift = p.ev_shift[k];
+    double w[NC];
+    int us[NC], ind[NC];
+    int err;
+
+    if (dest < 0 || dest >= p.Nn)
+        return SIMINF_ERR_DEST_OUT_OF_BOUNDS;
+
+    const uint32_t mask = load_E<NC>(p, select, w);
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        us[c] = uw[node * NC + c];
+    });
+    err = sample_select<NC>(us, mask, w, p.ev_n[k], p.ev_prop[k], ind, rng);
+    if (err)
+        return err;
+
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if (!err && (mask & (1u << c))) {
+            int kd = dest * NC + c;
+            if (shift >= 0) {
+                if (!p.Nmat) {
+                    err = SIMINF_ERR_EVENTS_N;
+                    return;
+                }
+                const int ll = p.Nmat[shift * NC + c];
+                if (c + ll < 0 || c + ll >= NC) {
+                    err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
+                    return;
+                }
+                kd += ll;
+            }
+            uw[kd] += ind[c];
+            if (uw[kd] < 0) {
+                err = SIMINF_ERR_NEGATIVE_STATE;
+                return;
+            }
+            uw[node * NC + c] -= ind[c];
+            if (uw[node * NC + c] < 0)
+                err = SIMINF_ERR_NEGATIVE_STATE;
+        }
+    });
+    if (err)
+        return err;
+    upd[dest] = 1;
+    upd[node] = 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Writeback (ssm.c:318-334, mssm.c:323-371, SimInf_solver.c:930-950). */
+/* ------------------------------------------------------------------ */
+
+/* Sparse column writeback for one node: the rows of column col that
+ * belong to node are contiguous; find them by binary search. */
+template <int NROW, class T>
+__device__ __forceinline__ void store_sparse(const int *ir, const int *jc,
+                                             double *pr, long long col,
+                                             int node, const T (&x)[NROW])
+{
+    int lo = jc[col], hi = jc[col + 1];
+    const int first = node * NROW;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ir[mid] < first)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const int end = jc[col + 1];
+    for (int j = lo; j < end && ir[j] < first + NROW; j++)
+        pr[j] = (double)sel(x, ir[j] - first);
+}
+
+template <int NC, int ND>
+__device__ __forceinline__ void store_columns(
+    const KParams &p, int rep, int node, double tt, int &U_it, int &V_it,
+    const int (&u)[NC], const double (&vnew)[ND > 0 ? ND : 1])
+{
+    while (U_it < p.tlen && tt > p.tspan[U_it]) {
+        const long long col = (long long)rep * p.tlen + U_it;
+        if (p.U) {
+            int *dst = p.U + (col * p.Nn + node) * NC;
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                dst[c] = u[c];
+            });
+        } else {
+            store_sparse<NC>(p.irU, p.jcU, p.prU, col, node, u);
+        }
+        U_it++;
+    }
+    while (V_it < p.tlen && tt > p.tspan[V_it]) {
+        if constexpr (ND > 0) {
+            const long long col = (long long)rep * p.tlen + V_it;
+            if (p.V) {
+                double *dst = p.V + (col * p.Nn + node) * ND;
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    dst[c] = vnew[c];
+                });
+            } else {
+                store_sparse<ND>(p.irV, p.jcV, p.prV, col, node, vnew);
+            }
+        }
+        V_it++;
+    }
+}
+
+/* Full refresh of all NT rates after the post-time-step function
+ * (ssm.c:272-301); returns an error code or 0. */
+template <class M>
+__device__ __forceinline__ int refresh_rates(NodeState<M> &s, const KParams &p,
+                                             const double *ld, double tt)
+{
+    int err = 0;
+    double delta = 0.0;
+    static_for<M::NT>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        const double old = s.rate[j];
+        const double r = M::template rate<j>(s.u, s.vnew, ld, p.gdata, tt);
+        s.rate[j] = r;
+        delta += r - old;
+        if (!rate_ok(r))
+            err = SIMINF_ERR_INVALID_RATE;
+    });
+    s.a0 += delta;
+    return err;
+}
+
+__device__ __forceinline__ void add_transitions(const KParams &p, unsigned long long n)
+{
+    /* Wavefront reduction, one atomic per wave. */
+    for (int off = 32; off > 0; off >>= 1)
+        n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63) == 0 && n)
+        atomicAdd(p.transitions, n);
+}
+
+} /* namespace siminf */

@@ -1,0 +1,183 @@
+"""Synthetic workloads with the shapes of the benchmark configurations.
+
+  mparse_sir_readme   SIR via mparse, Nn x (99, 5, 0), beta 0.16,
+                      gamma 0.077, tspan 1:150 (README.md:60-76)
+  sise3_herds         SISe3 on herds of NegBin(mean 130, size 1.5) animals
+                      split 10/15/75 % by age, 1 % of herds seeded, with
+                      scheduled enter / exit / internal / external events
+                      (parameters of man/examples/SISe3.R)
+  seir_million        SEIR on Nn ~ Poisson(500) nodes, 0.5 % seeded with
+                      5 infected, beta 0.16, epsilon 0.25, gamma 0.077
+
+Scheduled events must stay feasible: the reference aborts when an event
+asks for more individuals than a node holds (SimInf_solver.c:102-105).
+The SSA only moves animals between S_k and I_k, so the per-age totals of
+a herd change through events alone; the generator tracks a lower and an
+upper bound of every (herd, age) total through the event list and clamps
+fixed counts to the lower bound.  Proportion draws (used for exits only)
+remove a binomial number of animals; the lower bound then drops by
+p*ub + 7 sd + 5, which the draw exceeds with probability below 1e-11.
+"""
+import numpy as np
+
+from . import models
+
+
+def mparse_sir_readme(Nn=1000, days=150, replicates=1):
+    return models.SIR_mparse(u0={"S": np.full(Nn, 99), "I": np.full(Nn, 5),
+                                 "R": np.zeros(Nn, int)},
+                             tspan=np.arange(1, days + 1, dtype=float),
+                             beta=0.16, gamma=0.077, replicates=replicates)
+
+
+def seir_million(Nn=1_000_000, days=365, seed=7, seeded_frac=0.005):
+    rng = np.random.default_rng(seed)
+    N = rng.poisson(500, Nn)
+    I = np.zeros(Nn, dtype=np.int64)
+    k = max(1, int(round(seeded_frac * Nn)))
+    idx = rng.choice(Nn, k, replace=False)
+    I[idx] = np.minimum(5, N[idx])
+    S = N - I
+    return models.SEIR(u0={"S": S, "E": np.zeros(Nn, int), "I": I,
+                           "R": np.zeros(Nn, int)},
+                       tspan=np.arange(1, days + 1, dtype=float),
+                       beta=0.16, epsilon=0.25, gamma=0.077)
+
+
+SISE3_GDATA = dict(upsilon_1=1.8e-2, upsilon_2=1.8e-2, upsilon_3=1.8e-2,
+                   gamma_1=0.1, gamma_2=0.1, gamma_3=0.1, alpha=1.0,
+                   beta_t1=0.1, beta_t2=0.1, beta_t3=0.125, beta_t4=0.125,
+                   epsilon=0.0)
+
+
+def _power_law_degrees(rng, n, exponent=2.3, kmax=1000):
+    k = np.arange(1, kmax + 1, dtype=float)
+    p = k ** (-exponent)
+    p /= p.sum()
+    return rng.choice(k, size=n, p=p)
+
+
+def sise3_events(u_age, days, rng, enter_rate=0.1, exit_rate=0.08,
+                 internal_every=7, external_rate=0.2, prop_frac=0.1):
+    """Feasible SISe3 events for herds whose per-age totals are u_age
+    (Nn x 3).  Returns the event columns (one-based node/dest/select/shift,
+    as the R layer stores them)."""
+    Nn = u_age.shape[0]
+    lb = u_age.astype(np.int64).copy()
+    ub = u_age.astype(np.int64).copy()
+    w = _power_law_degrees(rng, Nn)
+    w = w / w.sum()
+    out = {c: [] for c in ("event", "time", "node", "dest", "n", "proportion",
+                           "select", "shift")}
+
+    def push(ev, t, node, dest, n, prop, select, shift):
+        out["event"].append(ev)
+        out["time"].append(t)
+        out["node"].append(node + 1)
+        out["dest"].append(dest + 1)
+        out["n"].append(n)
+        out["proportion"].append(prop)
+        out["select"].append(select)
+        out["shift"].append(shift)
+
+    def leave_bound(a, p):
+        sd = np.sqrt(max(a * p * (1 - p), 0.0))
+        return int(min(a, np.ceil(p * a + 7 * sd + 5)))
+
+    for t in range(1, days + 1):
+        # Within a day the solver runs E1 before E2, each in sorted order:
+        # exit (0), enter (1), internal (2), external (3); the bounds are
+        # updated in the same order.
+        day = []
+        n_exit = rng.poisson(exit_rate * Nn)
+        for node in rng.integers(0, Nn, n_exit):
+            age = int(rng.integers(0, 3))
+            day.append((0, node, age))
+        n_enter = rng.poisson(enter_rate * Nn)
+        for node in rng.integers(0, Nn, n_enter):
+            day.append((1, node, 0))
+        if internal_every and t % internal_every == 0:
+            for node in range(Nn):
+                day.append((2, node, 0))
+                day.append((2, node, 1))
+        n_ext = rng.poisson(external_rate * Nn)
+        src = rng.choice(Nn, n_ext, p=w)
+        dst = rng.choice(Nn, n_ext, p=w)
+        for s_, d_ in zip(src, dst):
+            if s_ != d_:
+                day.append((3, int(s_), int(d_)))
+        # Sort like SimInf_events: (event, select) within the day.
+        def sel_of(e):
+            return {0: 4 + e[2], 1: 1, 2: 4 + e[2]}.get(e[0], None)
+        keyed = []
+        for e in day:
+            if e[0] == 3:
+                age = int(rng.integers(0, 3))
+                keyed.append((3, 4 + age, e, age))
+            else:
+                keyed.append((e[0], sel_of(e), e, e[2]))
+        keyed.sort(key=lambda x: (x[0], x[1]))
+        for ev, select, e, age in keyed:
+            node = e[1]
+            if ev == 0:
+                if rng.random() < prop_frac:
+                    p = float(rng.uniform(0.01, 0.05))
+                    push(0, t, node, node, 0, p, select, 0)
+                    lb[node, age] -= min(lb[node, age], leave_bound(ub[node, age], p))
+                else:
+                    n = min(int(rng.integers(1, 11)), int(lb[node, age]))
+                    if n <= 0:
+                        continue
+                    push(0, t, node, node, n, 0.0, select, 0)
+                    lb[node, age] -= n
+                    ub[node, age] -= n
+            elif ev == 1:
+                n = int(rng.integers(1, 11))
+                push(1, t, node, node, n, 0.0, 1, 0)
+                lb[node, 0] += n
+                ub[node, 0] += n
+            elif ev == 2:
+                n = min(int(rng.integers(1, 11)), int(lb[node, age]))
+                if n <= 0:
+                    continue
+                # select 4 with shift 1 moves age 1 -> 2; select 5 with
+                # shift 2 moves age 2 -> 3 (R/SISe3.R:158-160).
+                push(2, t, node, node, n, 0.0, select, age + 1)
+                lb[node, age] -= n
+                ub[node, age] -= n
+                lb[node, age + 1] += n
+                ub[node, age + 1] += n
+            else:
+                dest = e[2]
+                n = min(int(rng.integers(1, 11)), int(lb[node, age]))
+                if n <= 0:
+                    continue
+                push(3, t, node, dest, n, 0.0, select, 0)
+                lb[node, age] -= n
+                ub[node, age] -= n
+                lb[dest, age] += n
+                ub[dest, age] += n
+    return {k: np.asarray(v) for k, v in out.items()}
+
+
+def sise3_herds(Nn=1600, days=1460, seed=11, events=True, replicates=1,
+                **event_rates):
+    rng = np.random.default_rng(seed)
+    size = 1.5
+    mean = 130.0
+    herd = rng.negative_binomial(size, size / (size + mean), Nn)
+    a1 = np.floor(0.10 * herd).astype(int)
+    a2 = np.floor(0.15 * herd).astype(int)
+    a3 = herd - a1 - a2
+    I1 = np.zeros(Nn, dtype=int)
+    seeded = rng.choice(Nn, max(1, Nn // 100), replace=False)
+    I1[seeded] = 10
+    u0 = {"S_1": a1, "I_1": I1, "S_2": a2, "I_2": np.zeros(Nn, int),
+          "S_3": a3, "I_3": np.zeros(Nn, int)}
+    ev = None
+    if events:
+        u_age = np.stack([a1 + I1, a2, a3], axis=1)
+        ev = sise3_events(u_age, days, rng, **event_rates)
+    return models.SISe3(u0=u0, tspan=np.arange(1, days + 1, dtype=float),
+                        events=ev, phi=0.0, end_t1=91, end_t2=182, end_t3=273,
+                        end_t4=365, replicates=replicates, **SISE3_GDATA)

@@ -1,0 +1,182 @@
+"""GPU parity: the HIP solver core (through the C ABI) against the CPU
+checker on identical SimInf_solver_args and seeds.  Integer trajectories
+and transition counts must match bit for bit; V (double) must match bit
+for bit too, since both sides run the same IEEE operation sequence."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import siminf_amd as si
+from siminf_amd import _native, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(model, solver="ssm", seed=0, crn=False):
+    lib = _native.load()
+    dev = lib.SimInf_hip_model_builtin(model.plugin.encode())
+    assert dev
+    b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
+                           crn=int(crn), seed=seed)
+    plan = ctypes.c_void_p()
+    err = lib.SimInf_hip_plan_create(ctypes.byref(plan), ctypes.byref(b.args),
+                                     ctypes.c_void_p(dev))
+    assert err == 0, err
+    try:
+        err = lib.SimInf_hip_plan_run(plan, seed)
+        cp = lambda a, t: _native.ptr(a, t)
+        if err == 0:
+            assert lib.SimInf_hip_plan_download(
+                plan, cp(b.U, ctypes.c_int), cp(b.V, ctypes.c_double),
+                cp(b.prU, ctypes.c_double), cp(b.prV, ctypes.c_double)) == 0
+        tr = ctypes.c_longlong()
+        nd = ctypes.c_longlong()
+        ms = ctypes.c_double()
+        lib.SimInf_hip_plan_stats(plan, ctypes.byref(tr), ctypes.byref(nd),
+                                  ctypes.byref(ms))
+    finally:
+        lib.SimInf_hip_plan_destroy(plan)
+    return err, b, tr.value
+
+
+def _check(model, oracle, solver="ssm", seed=1, crn=False):
+    print("case: %s %s Nn=%d Nrep=%d tlen=%d events=%d seed=%d" % (
+        model.plugin, solver, model.Nn, model.replicates, len(model.tspan),
+        len(model.events["event"]), seed), flush=True)
+    ref = oracle.run(model, solver=solver, seed=seed, crn=crn)
+    err, b, transitions = _gpu(model, solver=solver, seed=seed, crn=crn)
+    assert err == ref.err
+    if err == 0:
+        if b.U is not None:
+            np.testing.assert_array_equal(b.U, ref.U)
+        else:
+            np.testing.assert_array_equal(b.prU, ref.prU)
+        if b.V is not None:
+            np.testing.assert_array_equal(b.V, ref.V)
+        else:
+            np.testing.assert_array_equal(b.prV, ref.prV)
+        assert transitions == ref.transitions
+    return ref, b
+
+
+def test_mparse_sir_readme_full(oracle):
+    """Config 1 at full size: 1000 nodes x 150 days."""
+    m = W.mparse_sir_readme()
+    ref, b = _check(m, oracle, seed=123)
+    assert ref.transitions > 100000
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_seir_ssm(oracle, seed):
+    m = W.seir_million(Nn=20000, days=120, seed=seed, seeded_frac=0.05)
+    _check(m, oracle, solver="ssm", seed=seed)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_seir_aem(oracle, seed):
+    m = W.seir_million(Nn=5000, days=100, seed=seed, seeded_frac=0.05)
+    ref, _ = _check(m, oracle, solver="aem", seed=seed)
+    assert ref.aem_ties == 0
+
+
+def test_sir_builtin_aem_and_ssm(oracle):
+    m = si.SIR(u0={"S": np.arange(50, 250), "I": 3, "R": 0},
+               tspan=np.linspace(0, 60, 31), beta=0.3, gamma=0.1)
+    _check(m, oracle, solver="ssm", seed=9)
+    _check(m, oracle, solver="aem", seed=9)
+
+
+def test_mssm_replicates(oracle):
+    m = W.mparse_sir_readme(Nn=30, days=60, replicates=200)
+    _check(m, oracle, solver="ssm", seed=5)
+    _check(m, oracle, solver="ssm", seed=5, crn=True)
+
+
+def test_sise3_e1_events(oracle):
+    m = W.sise3_herds(Nn=300, days=200, seed=3, external_rate=0.0)
+    assert (m.events["event"] == 3).sum() == 0
+    _check(m, oracle, seed=4)
+
+
+def test_sise3_e2_events(oracle):
+    m = W.sise3_herds(Nn=300, days=200, seed=5)
+    assert (m.events["event"] == 3).sum() > 0
+    _check(m, oracle, seed=6)
+
+
+def test_sise3_e2_events_replicates(oracle):
+    m = W.sise3_herds(Nn=120, days=100, seed=8, replicates=6)
+    _check(m, oracle, seed=7)
+
+
+def test_sise3_config2_full(oracle):
+    """Config 2 at full size: 1600 herds, 1460 days, ~1.2M events."""
+    m = W.sise3_herds()
+    ref, _ = _check(m, oracle, seed=5)
+    assert ref.transitions > 0
+
+
+def test_sparse_output(oracle):
+    m = W.mparse_sir_readme(Nn=50, days=40)
+    rows = np.array([0, 1, 5, 7, 8, 149])
+    cols = np.array([0, 3, 3, 10, 39, 39])
+    m.U_sparse = sp.csc_matrix((np.ones(len(rows)), (rows, cols)),
+                               shape=(m.Nn * m.Nc, len(m.tspan)))
+    _check(m, oracle, seed=2)
+
+
+def test_sise_phi_sparse_v(oracle):
+    m = si.SISe(u0={"S": [100], "I": [0]}, tspan=np.arange(1, 701, 7.0), phi=1.0,
+                upsilon=0.0, gamma=0.0, alpha=0.0, beta_t1=0.19, beta_t2=0.085,
+                beta_t3=0.075, beta_t4=0.185, end_t1=91, end_t2=182, end_t3=273,
+                end_t4=365, epsilon=0.000011)
+    m.V_sparse = sp.csc_matrix(np.ones((1, len(m.tspan))))
+    _check(m, oracle, seed=1)
+
+
+def test_seir_enter_events_known_answer():
+    """tests/SEIR.R:231-295: one individual enters every compartment of
+    every node every day; zero rates."""
+    from tests.golden_data import seir_enter_model, seir_enter_expected
+    m = seir_enter_model()
+    si.run(m, seed=1)
+    np.testing.assert_array_equal(m.U, seir_enter_expected())
+
+
+def test_errors_match(oracle):
+    # Invalid rate at initialisation (tests/SIR.R:483-492).
+    m = si.SIR(u0={"S": [99, 99], "I": 1, "R": 0}, tspan=np.arange(1, 101.0),
+               beta=0.16, gamma=-0.077)
+    for solver in ("ssm", "aem"):
+        ref, _ = _check(m, oracle, solver=solver, seed=1)
+        assert ref.err == -11
+    # Negative state from a corrupted S (tests/SIR.R:554-567).
+    m = si.SIR(u0={"S": [99], "I": 1, "R": 0}, tspan=np.arange(1, 101.0),
+               beta=0.16, gamma=0.0)
+    S = m.S.tocsc()
+    S.data[0] = -1000
+    m.S = S
+    for solver in ("ssm", "aem"):
+        ref, _ = _check(m, oracle, solver=solver, seed=1)
+        assert ref.err == -1
+    # Sampling more individuals than exist (tests/sample_select.R:42-80).
+    m = si.SISe3(u0={"S_1": [0, 1], "I_1": 0, "S_2": 0, "I_2": 0, "S_3": 0, "I_3": 0},
+                 tspan=np.arange(0, 11.0),
+                 events={"event": [3], "time": [1], "node": [2], "dest": [1],
+                         "n": [2], "proportion": [1.0], "select": [4], "shift": [0]},
+                 phi=0.0, upsilon_1=0, upsilon_2=0, upsilon_3=0, gamma_1=1,
+                 gamma_2=1, gamma_3=1, alpha=0, beta_t1=1, beta_t2=1, beta_t3=1,
+                 beta_t4=1, end_t1=91, end_t2=182, end_t3=273, end_t4=365,
+                 epsilon=0)
+    ref, _ = _check(m, oracle, seed=1)
+    assert ref.err == -8
+
+
+def test_run_api_raises_reference_message():
+    m = si.SIR(u0={"S": [99, 99], "I": 1, "R": 0}, tspan=np.arange(1, 101.0),
+               beta=0.16, gamma=-0.077)
+    with pytest.raises(si.SimInfError, match=r"Invalid rate detected \(non-finite or < 0\.0\)\."):
+        si.run(m, solver="ssm", seed=1)
