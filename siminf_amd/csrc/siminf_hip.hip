@@ -150,7 +150,7 @@ __device__ __forceinline__ int e1_day(const KParams &p, int &cur, int end,
     return 0;
 }
 
-template <class M>
+template <class M, bool EVENTS>
 __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
@@ -204,10 +204,12 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                     record_error(p, rep, day, PH_SSA, node, err);
                     break;
                 }
-                err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
-                if (err) {
-                    record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
-                    break;
+                if constexpr (EVENTS) {
+                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                    if (err) {
+                        record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
+                        break;
+                    }
                 }
 
                 /* Post time step and refresh (ssm.c:262-302). */
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
     add_transitions(p, fired);
 }
 
-template <class M>
+template <class M, bool EVENTS>
 __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
@@ -304,10 +306,12 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
                     record_error(p, 0, day, PH_SSA, node, err);
                     break;
                 }
-                err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
-                if (err) {
-                    record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
-                    break;
+                if constexpr (EVENTS) {
+                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                    if (err) {
+                        record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
+                        break;
+                    }
                 }
 
                 /* Post time step; refresh through calcTimes at the node's
@@ -577,24 +581,32 @@ __global__ __launch_bounds__(1024) void k_ssm_reps(KParams p, int u_in_lds)
 struct SimInf_hip_model {
     const char *name;
     int Nc, Nt, Nd, Ngdata;
-    void (*ssm_nodes)(const KParams &, unsigned, unsigned, hipStream_t);
-    void (*aem_nodes)(const KParams &, unsigned, unsigned, hipStream_t);
+    void (*ssm_nodes)(const KParams &, bool, unsigned, unsigned, hipStream_t);
+    void (*aem_nodes)(const KParams &, bool, unsigned, unsigned, hipStream_t);
     hipError_t (*ssm_reps)(const KParams &, int, unsigned, unsigned, size_t,
                            hipStream_t);
 };
 
+/* Models without E1 events get a kernel without the event code, which
+ * needs fewer registers. */
 template <class M>
-static void launch_ssm_nodes(const KParams &p, unsigned grid, unsigned block,
-                             hipStream_t st)
+static void launch_ssm_nodes(const KParams &p, bool events, unsigned grid,
+                             unsigned block, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ssm_nodes<M>, dim3(grid), dim3(block), 0, st, p);
+    if (events)
+        hipLaunchKernelGGL((k_ssm_nodes<M, true>), dim3(grid), dim3(block), 0, st, p);
+    else
+        hipLaunchKernelGGL((k_ssm_nodes<M, false>), dim3(grid), dim3(block), 0, st, p);
 }
 
 template <class M>
-static void launch_aem_nodes(const KParams &p, unsigned grid, unsigned block,
-                             hipStream_t st)
+static void launch_aem_nodes(const KParams &p, bool events, unsigned grid,
+                             unsigned block, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_aem_nodes<M>, dim3(grid), dim3(block), 0, st, p);
+    if (events)
+        hipLaunchKernelGGL((k_aem_nodes<M, true>), dim3(grid), dim3(block), 0, st, p);
+    else
+        hipLaunchKernelGGL((k_aem_nodes<M, false>), dim3(grid), dim3(block), 0, st, p);
 }
 
 template <class M>
@@ -703,6 +715,7 @@ struct SimInf_hip_plan {
     unsigned grid = 0, block = 0;
     size_t lds = 0;
     int u_in_lds = 0;
+    bool has_e1 = false;
 
     /* Host copies needed after runs. */
     int Nn, Nc, Nd, Nrep, tlen;
@@ -854,6 +867,7 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     std::vector<int> e1_ptr(Nn + 1, 0), e1_idx;
     for (int i = 0; i < Nn; i++)
         e1_ptr[i + 1] = e1_ptr[i] + e1_count[i + 1];
+    P->has_e1 = e1_ptr[Nn] > 0;
     e1_idx.resize(std::max(e1_ptr[Nn], 1));
     {
         std::vector<int> fill(e1_ptr.begin(), e1_ptr.end() - 1);
@@ -1058,10 +1072,10 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
     (void)hipEventRecord(P->ev_start, P->stream);
     switch (P->kernel) {
     case K_SSM_NODES:
-        P->model->ssm_nodes(p, P->grid, P->block, P->stream);
+        P->model->ssm_nodes(p, P->has_e1, P->grid, P->block, P->stream);
         break;
     case K_AEM_NODES:
-        P->model->aem_nodes(p, P->grid, P->block, P->stream);
+        P->model->aem_nodes(p, P->has_e1, P->grid, P->block, P->stream);
         break;
     case K_SSM_REPS:
         if (P->model->ssm_reps(p, P->u_in_lds, P->grid, P->block, P->lds,
