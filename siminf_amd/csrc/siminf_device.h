@@ -99,8 +99,13 @@ struct KParams {
     const int *e1_ptr, *e1_idx;
 
     /* E2 schedule: day d's conflict-free levels are e2_day[d] ..
-     * e2_day[d+1]; level l holds e2_idx[e2_lvl[l] .. e2_lvl[l+1]). */
+     * e2_day[d+1]; level l holds e2_idx[e2_lvl[l] .. e2_lvl[l+1]).  The
+     * herds a day's E2 events touch are e2_tn[e2_tn_ptr[d] ..
+     * e2_tn_ptr[d+1]); e2_lsrc[k] / e2_ldst[k] are event k's source and
+     * destination slots in that list (-1 for an invalid destination). */
     const int *e2_day, *e2_lvl, *e2_idx;
+    const int *e2_tn_ptr, *e2_tn, *e2_lsrc, *e2_ldst;
+    int e2_max_touched;
     int ndays;
 
     /* Output: dense U/V or sparse (irU, jcU, prU) / (irV, jcV, prV). */
@@ -111,14 +116,12 @@ struct KParams {
     const int *irV, *jcV;
     double *prV;
 
-    /* Work state for the replicate-per-workgroup kernel. */
-    int *u_work;                     /* Nc x (Nn*Nrep) when u is not in LDS */
-    double *v_work, *vnew_work;      /* Nd x (Nn*Nrep) */
-    double *rate_work;               /* Nt x (Nn*Nrep), SoA by transition */
-    double *a0_work;                 /* Nn*Nrep */
-    unsigned long long *rng_j;       /* Nn*Nrep */
-    double *rng_cache;               /* Nn*Nrep */
-    int *e1_cursor;                  /* Nn*Nrep */
+    /* Coupled kernel: compartment state published for the E2 phase and
+     * the update flags it sets; rep0 is the replicate a grid-wide launch
+     * runs. */
+    int *u_work;                     /* Nc x (Nn*Nrep) */
+    int *upd_work;                   /* Nn*Nrep */
+    int rep0;
 
     /* Results. */
     unsigned long long *err_key;
@@ -152,13 +155,15 @@ __device__ __forceinline__ T sel(const T (&a)[N], int i)
     return r;
 }
 
-template <int N, class T>
-__device__ __forceinline__ void add_at(T (&a)[N], int i, T x)
+/* a[i] += x for a run-time i, written as an unconditional masked update
+ * of every element so that the array stays in registers (a guarded
+ * store per element can be merged back into one indexed store). */
+template <int N>
+__device__ __forceinline__ void add_at(int (&a)[N], int i, int x)
 {
     static_for<N>([&](auto K) {
         constexpr int k = decltype(K)::value;
-        if (i == k)
-            a[k] += x;
+        a[k] += x & -(int)(i == k);
     });
 }
 
@@ -176,30 +181,96 @@ struct NodeState {
     siminf_stream rng;
 };
 
+/* What a lane reads while it fires transitions: its node's ldata and the
+ * gdata copied into registers once (the model reads M::NLD and M::NGD
+ * values; ldata beyond Nld reads as INFINITY, like the reference's
+ * placeholder when Nld == 0), and S and G.  With CANON the run's S and G
+ * equal the model's own tables (M::S[k][c] = S(c,k), M::G[k][j] = G(j,k))
+ * and fold into the code; otherwise their masks and values are loaded
+ * once from the kernel arguments. */
+template <class M, bool CANON>
+struct Ctx {
+    double ld[M::NLD > 0 ? M::NLD : 1];
+    double gd[M::NGD > 0 ? M::NGD : 1];
+    uint32_t smask[CANON ? 1 : M::NT];
+    uint32_t gmask[CANON ? 1 : M::NT];
+    int sval[CANON ? 1 : M::NT * M::NC];
+    unsigned long long seed;
+
+    __device__ __forceinline__ void load(const KParams &p, int node)
+    {
+        const double *l = p.ldata + (long long)node * p.Nld;
+        static_for<(M::NLD > 0 ? M::NLD : 1)>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            ld[i] = i < p.Nld ? l[i] : INFINITY;
+        });
+        static_for<(M::NGD > 0 ? M::NGD : 1)>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            gd[i] = i < M::NGD ? p.gdata[i] : 0.0;
+        });
+        if constexpr (!CANON) {
+            static_for<M::NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                smask[k] = p.Smask[k];
+                gmask[k] = p.Gmask[k];
+                static_for<M::NC>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    sval[k * M::NC + c] = p.Sval[k * M::NC + c];
+                });
+            });
+        }
+        seed = p.seed;
+    }
+
+    template <int K, int C>
+    __device__ __forceinline__ bool s_stored() const
+    {
+        if constexpr (CANON)
+            return M::S[K][C] != 0;
+        else
+            return (smask[K] >> C) & 1u;
+    }
+
+    template <int K, int C>
+    __device__ __forceinline__ int s_val() const
+    {
+        if constexpr (CANON)
+            return M::S[K][C];
+        else
+            return sval[K * M::NC + C];
+    }
+
+    template <int K, int J>
+    __device__ __forceinline__ bool g_stored() const
+    {
+        if constexpr (CANON)
+            return M::G[K][J] != 0;
+        else
+            return (gmask[K] >> J) & 1u;
+    }
+};
+
 /* Fire transition K: apply S(:,K), then recompute the rates listed in
  * G(:,K) in row order and add their changes to a0.  Returns an error
  * code or 0; like the reference the step completes after an error. */
-template <class M, int K>
-__device__ __forceinline__ int fire(NodeState<M> &s, const KParams &p,
-                                    const double *ld, double t)
+template <class M, int K, class CT>
+__device__ __forceinline__ int fire(NodeState<M> &s, const CT &cx, double t)
 {
     int err = 0;
-    const uint32_t smask = p.Smask[K];
     static_for<M::NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
-        if (smask & (1u << c)) {
-            s.u[c] += p.Sval[K * M::NC + c];
+        if (cx.template s_stored<K, c>()) {
+            s.u[c] += cx.template s_val<K, c>();
             if (s.u[c] < 0)
                 err = SIMINF_ERR_NEGATIVE_STATE;
         }
     });
-    const uint32_t gmask = p.Gmask[K];
     double delta = 0.0;
     static_for<M::NT>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        if (gmask & (1u << j)) {
+        if (cx.template g_stored<K, j>()) {
             const double old = s.rate[j];
-            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, t);
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, t);
             s.rate[j] = r;
             delta += r - old;
             if (!rate_ok(r))
@@ -210,9 +281,9 @@ __device__ __forceinline__ int fire(NodeState<M> &s, const KParams &p,
     return err;
 }
 
-template <class M>
-__device__ __forceinline__ int ssa_day(NodeState<M> &s, const KParams &p,
-                                       const double *ld, double tt, double next,
+template <class M, class CT>
+__device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
+                                       double tt, double next,
                                        unsigned long long &fired)
 {
     constexpr int NT = M::NT;
@@ -226,28 +297,34 @@ __device__ __forceinline__ int ssa_day(NodeState<M> &s, const KParams &p,
             break;
         t += tau;
 
+        /* Linear scan of the cumulative rates; the rate of the selected
+         * transition is carried along (no run-time indexing). */
         const double rnd = siminf_uniform(&s.rng) * s.a0;
         int tr = 0;
+        double rt = s.rate[0];
         double cum = s.rate[0];
         bool searching = rnd > cum;
         static_for<NT>([&](auto K) {
             constexpr int k = decltype(K)::value;
             if (k > 0 && searching) {
                 tr = k;
+                rt = s.rate[k];
                 cum += s.rate[k];
                 searching = rnd > cum;
             }
         });
-        /* Floating point fix (ssm.c:170-189). */
-        if (searching)
-            tr = NT - 1;
-        if (sel(s.rate, tr) == 0.0) {
+        /* Floating point fix (ssm.c:170-189): past the end means the last
+         * transition, which the scan already holds; a zero rate walks
+         * back to the nearest non-zero one below it. */
+        if (rt == 0.0) {
             static_for<NT>([&](auto K) {
                 constexpr int k = NT - 1 - decltype(K)::value;
-                if (k > 0 && tr == k && s.rate[k] == 0.0)
+                if (k > 0 && tr == k && s.rate[k] == 0.0) {
                     tr = k - 1;
+                    rt = s.rate[k - 1];
+                }
             });
-            if (sel(s.rate, tr) == 0.0) {
+            if (rt == 0.0) {
                 s.a0 = 0.0;
                 break;
             }
@@ -257,7 +334,7 @@ __device__ __forceinline__ int ssa_day(NodeState<M> &s, const KParams &p,
         static_for<NT>([&](auto K) {
             constexpr int k = decltype(K)::value;
             if (tr == k) {
-                const int e = fire<M, k>(s, p, ld, t);
+                const int e = fire<M, k>(s, cx, t);
                 if (e)
                     err = e;
             }
@@ -287,12 +364,13 @@ struct AemState {
 
 /* calcTimes (aem.c:61-86) for channel J. */
 template <class M, int J>
-__device__ __forceinline__ void calc_times(AemState<M> &s, const KParams &p,
+__device__ __forceinline__ void calc_times(AemState<M> &s,
+                                           unsigned long long seed,
                                            uint64_t chan_base, double tt,
                                            double old_rate, double new_rate)
 {
     const double oldtime = s.time[J];
-    siminf_stream c = siminf_stream_make(p.seed, SIMINF_DOMAIN_AEM,
+    siminf_stream c = siminf_stream_make(seed, SIMINF_DOMAIN_AEM,
                                          chan_base + (uint64_t)J);
     c.j = s.j[J];
     if (isinf(oldtime)) {
@@ -315,48 +393,44 @@ __device__ __forceinline__ void calc_times(AemState<M> &s, const KParams &p,
     }
 }
 
-template <class M, int K>
-__device__ __forceinline__ int aem_fire(AemState<M> &s, const KParams &p,
-                                        const double *ld, double t,
+template <class M, int K, class CT>
+__device__ __forceinline__ int aem_fire(AemState<M> &s, const CT &cx, double t,
                                         uint64_t chan_base)
 {
     int err = 0;
-    const uint32_t smask = p.Smask[K];
     static_for<M::NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
-        if (smask & (1u << c)) {
-            s.u[c] += p.Sval[K * M::NC + c];
+        if (cx.template s_stored<K, c>()) {
+            s.u[c] += cx.template s_val<K, c>();
             if (s.u[c] < 0)
                 err = SIMINF_ERR_NEGATIVE_STATE;
         }
     });
-    const uint32_t gmask = p.Gmask[K];
     static_for<M::NT>([&](auto J) {
         constexpr int j = decltype(J)::value;
-        if (j != K && (gmask & (1u << j))) {
+        if (j != K && cx.template g_stored<K, j>()) {
             const double old = s.rate[j];
-            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, t);
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, t);
             s.rate[j] = r;
             if (!rate_ok(r))
                 err = SIMINF_ERR_INVALID_RATE;
-            calc_times<M, j>(s, p, chan_base, t, old, r);
+            calc_times<M, j>(s, cx.seed, chan_base, t, old, r);
         }
     });
     {
         const double old = s.rate[K];
-        const double r = M::template rate<K>(s.u, s.v, ld, p.gdata, t);
+        const double r = M::template rate<K>(s.u, s.v, cx.ld, cx.gd, t);
         s.rate[K] = r;
         if (!rate_ok(r))
             err = SIMINF_ERR_INVALID_RATE;
-        calc_times<M, K>(s, p, chan_base, t, old, r);
+        calc_times<M, K>(s, cx.seed, chan_base, t, old, r);
     }
     return err;
 }
 
-template <class M>
-__device__ __forceinline__ int aem_day(AemState<M> &s, const KParams &p,
-                                       const double *ld, double next,
-                                       uint64_t chan_base,
+template <class M, class CT>
+__device__ __forceinline__ int aem_day(AemState<M> &s, const CT &cx,
+                                       double next, uint64_t chan_base,
                                        unsigned long long &fired)
 {
     constexpr int NT = M::NT;
@@ -378,7 +452,7 @@ __device__ __forceinline__ int aem_day(AemState<M> &s, const KParams &p,
         static_for<NT>([&](auto K) {
             constexpr int k = decltype(K)::value;
             if (tr == k) {
-                const int e = aem_fire<M, k>(s, p, ld, t, chan_base);
+                const int e = aem_fire<M, k>(s, cx, t, chan_base);
                 if (e)
                     err = e;
             }
@@ -400,6 +474,61 @@ __device__ __forceinline__ int mask_prev(uint32_t m, int c)
 {
     const uint32_t below = m & ((1u << c) - 1u);
     return below ? 31 - __builtin_clz(below) : -1;
+}
+
+/* Inversion over the stored rows of an E column with weights wv: the
+ * first row whose cumulative weight reaches rnd, or the last row when
+ * rounding leaves rnd above the total (SimInf_solver.c:185-193). */
+template <int NC>
+__device__ __forceinline__ int scan_rows(uint32_t mask, const double (&wv)[NC],
+                                         double rnd)
+{
+    int i = -1;
+    double cum = 0.0;
+    bool searching = true, first = true;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if (mask & (1u << c)) {
+            if (first) {
+                cum = wv[c];
+                first = false;
+                i = c;
+                searching = rnd > cum;
+            } else if (searching) {
+                i = c;
+                cum += wv[c];
+                searching = rnd > cum;
+            }
+        }
+    });
+    return i;
+}
+
+/* The largest stored row at or below i whose weight is non-zero, or -1:
+ * the reference's walk back from i (SimInf_solver.c:194-205, :308-316). */
+template <int NC>
+__device__ __forceinline__ int last_nonzero_at_or_below(
+    uint32_t mask, const double (&wv)[NC], int i)
+{
+    int j = -1;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if ((mask & (1u << c)) && c <= i && wv[c] != 0.0)
+            j = c;
+    });
+    return j;
+}
+
+/* Is the run-time-indexed element a[i] negative? */
+template <int NC>
+__device__ __forceinline__ bool negative_at(const int (&a)[NC], int i)
+{
+    bool neg = false;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        neg = neg || (i == c && a[c] < 0);
+    });
+    return neg;
 }
 
 /* sample_select for a node whose state is the register array u.  The E
@@ -437,8 +566,7 @@ __device__ int sample_select(const int (&u)[NC], uint32_t mask,
     if (Nind == n) {
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
-            if (mask & (1u << c))
-                ind[c] = u[c];
+            ind[c] = u[c] & -(int)((mask >> c) & 1u);
         });
         return 0;
     }
@@ -449,14 +577,13 @@ __device__ int sample_select(const int (&u)[NC], uint32_t mask,
     }
 
     if (Nkinds == 1) {
-        bool done = false;
+        int first_kind = -1;
         static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            if (!done && (mask & (1u << c)) && u[c] > 0) {
-                ind[c] = n;
-                done = true;
-            }
+            constexpr int c = NC - 1 - decltype(C)::value;
+            if ((mask & (1u << c)) && u[c] > 0)
+                first_kind = c;
         });
+        add_at(ind, first_kind, n);
         return 0;
     }
 
@@ -495,47 +622,28 @@ __device__ int sample_select(const int (&u)[NC], uint32_t mask,
                 }
             }
         });
-        add_at(ind, stop_row, n - sel(ind, stop_row));
+        add_at(ind, stop_row, n);
         return 0;
     }
 
     /* Wallenius urn, one individual per draw. */
     while (n > 0) {
+        double wv[NC];
         double cum = 0.0;
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
+            wv[c] = w[c] * (u[c] - ind[c]);
             if (mask & (1u << c))
-                cum += w[c] * (u[c] - ind[c]);
+                cum += wv[c];
         });
         const double rnd = siminf_uniform(rng) * cum;
-
-        int i = -1;
-        bool searching = true, first = true;
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            if (mask & (1u << c)) {
-                const double wc = w[c] * (u[c] - ind[c]);
-                if (first) {
-                    cum = wc;
-                    first = false;
-                    i = c;
-                    searching = rnd > cum;
-                } else if (searching) {
-                    i = c;
-                    cum += wc;
-                    searching = rnd > cum;
-                }
-            }
-        });
-
-        if (sel(w, i) * (sel(u, i) - sel(ind, i)) == 0.0) {
-            const int first_row = mask_first(mask);
-            while (i > first_row && sel(w, i) * (sel(u, i) - sel(ind, i)) == 0.0)
-                i = mask_prev(mask, i);
-            if (sel(w, i) * (sel(u, i) - sel(ind, i)) == 0.0)
-                return SIMINF_ERR_SAMPLE_SELECT;
-        }
-        add_at(ind, i, 1);
+        const int i = scan_rows<NC>(mask, wv, rnd);
+        /* Floating point fix (SimInf_solver.c:191-206): the nearest row at
+         * or below i with weight left. */
+        const int j = last_nonzero_at_or_below<NC>(mask, wv, i);
+        if (j < 0)
+            return SIMINF_ERR_SAMPLE_SELECT;
+        add_at(ind, j, 1);
         n--;
     }
     return 0;
@@ -580,32 +688,11 @@ __device__ int sample_select_enter(const int (&u)[NC], uint32_t mask,
 
     while (n > 0) {
         const double rnd = siminf_uniform(rng) * w_cum;
-        int i = -1;
-        double cum = 0.0;
-        bool searching = true, first = true;
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            if (mask & (1u << c)) {
-                if (first) {
-                    cum = w[c];
-                    first = false;
-                    i = c;
-                    searching = rnd > cum;
-                } else if (searching) {
-                    i = c;
-                    cum += w[c];
-                    searching = rnd > cum;
-                }
-            }
-        });
-        if (sel(w, i) == 0.0) {
-            const int first_row = mask_first(mask);
-            while (i > first_row && sel(w, i) == 0.0)
-                i = mask_prev(mask, i);
-            if (sel(w, i) == 0.0)
-                return SIMINF_ERR_SAMPLE_SELECT;
-        }
-        add_at(ind, i, 1);
+        const int i = scan_rows<NC>(mask, w, rnd);
+        const int j = last_nonzero_at_or_below<NC>(mask, w, i);
+        if (j < 0)
+            return SIMINF_ERR_SAMPLE_SELECT;
+        add_at(ind, j, 1);
         n--;
     }
     return 0;
@@ -672,7 +759,7 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
                         err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
                     } else {
                         add_at(u, c + ll, ind[c]);
-                        if (sel(u, c + ll) < 0)
+                        if (negative_at(u, c + ll))
                             err = SIMINF_ERR_NEGATIVE_STATE;
                     }
                 }
@@ -697,7 +784,7 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
                     err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
                 } else {
                     add_at(u, c + ll, ind[c]);
-                    if (sel(u, c + ll) < 0) {
+                    if (negative_at(u, c + ll)) {
                         err = SIMINF_ERR_NEGATIVE_STATE;
                     } else {
                         u[c] -= ind[c];
@@ -713,14 +800,15 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
     return SIMINF_UNDEFINED_EVENT;
 }
 
-/* Apply an external transfer event k to the replicate state uw (Nn x NC,
- * in LDS or global memory) (SimInf_solver.c:824-902).  upd receives the
- * update flags of source and dest. */
+/* Apply an external transfer event k (SimInf_solver.c:824-902) to a
+ * state array uw (NC ints per slot, in LDS or global memory): the source
+ * herd sits in slot src, the destination in slot dst; upd receives the
+ * update flags of both slots.  The destination is checked against the
+ * node count before any slot is touched. */
 template <int NC>
-__device__ int apply_e2(const KParams &p, int k, int *uw, int *upd,
-                        siminf_stream *rng)
+__device__ int apply_e2(const KParams &p, int k, int *uw, int *upd, int src,
+                        int dst, siminf_stream *rng)
 {
-    const int node = p.ev_node[k];
     const int dest = p.ev_dest[k];
     const int select = p.ev_select[k];
     const int shift = p.ev_shift[k];
@@ -734,7 +822,7 @@ __device__ int apply_e2(const KParams &p, int k, int *uw, int *upd,
     const uint32_t mask = load_E<NC>(p, select, w);
     static_for<NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
-        us[c] = uw[node * NC + c];
+        us[c] = uw[src * NC + c];
     });
     err = sample_select<NC>(us, mask, w, p.ev_n[k], p.ev_prop[k], ind, rng);
     if (err)
@@ -743,7 +831,7 @@ __device__ int apply_e2(const KParams &p, int k, int *uw, int *upd,
     static_for<NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
         if (!err && (mask & (1u << c))) {
-            int kd = dest * NC + c;
+            int kd = dst * NC + c;
             if (shift >= 0) {
                 if (!p.Nmat) {
                     err = SIMINF_ERR_EVENTS_N;
@@ -761,15 +849,15 @@ __device__ int apply_e2(const KParams &p, int k, int *uw, int *upd,
                 err = SIMINF_ERR_NEGATIVE_STATE;
                 return;
             }
-            uw[node * NC + c] -= ind[c];
-            if (uw[node * NC + c] < 0)
+            uw[src * NC + c] -= ind[c];
+            if (uw[src * NC + c] < 0)
                 err = SIMINF_ERR_NEGATIVE_STATE;
         }
     });
     if (err)
         return err;
-    upd[dest] = 1;
-    upd[node] = 1;
+    upd[dst] = 1;
+    upd[src] = 1;
     return 0;
 }
 
@@ -794,8 +882,14 @@ __device__ __forceinline__ void store_sparse(const int *ir, const int *jc,
             hi = mid;
     }
     const int end = jc[col + 1];
-    for (int j = lo; j < end && ir[j] < first + NROW; j++)
-        pr[j] = (double)sel(x, ir[j] - first);
+    for (int j = lo; j < end && ir[j] < first + NROW; j++) {
+        const int r = ir[j] - first;
+        static_for<NROW>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (r == c)
+                pr[j] = (double)x[c];
+        });
+    }
 }
 
 template <int NC, int ND>
@@ -835,16 +929,16 @@ __device__ __forceinline__ void store_columns(
 
 /* Full refresh of all NT rates after the post-time-step function
  * (ssm.c:272-301); returns an error code or 0. */
-template <class M>
-__device__ __forceinline__ int refresh_rates(NodeState<M> &s, const KParams &p,
-                                             const double *ld, double tt)
+template <class M, class CT>
+__device__ __forceinline__ int refresh_rates(NodeState<M> &s, const CT &cx,
+                                             double tt)
 {
     int err = 0;
     double delta = 0.0;
     static_for<M::NT>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const double old = s.rate[j];
-        const double r = M::template rate<j>(s.u, s.vnew, ld, p.gdata, tt);
+        const double r = M::template rate<j>(s.u, s.vnew, cx.ld, cx.gd, tt);
         s.rate[j] = r;
         delta += r - old;
         if (!rate_ok(r))

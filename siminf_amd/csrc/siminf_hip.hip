@@ -9,16 +9,16 @@
  *                   SimInf_solver_mssm.c:81-402).
  *   k_aem_nodes<M>  the same for the all-events method
  *                   (SimInf_solver_aem.c:93-459).
- *   k_ssm_reps<M>   one workgroup per replicate, one barrier-separated day
- *                   at a time, for models with E2 events: the replicate's
- *                   compartment state sits in LDS while the day's E2
- *                   events run level by level; the levels are
- *                   conflict-free sets that keep each node's events in
- *                   list order, so no atomics are needed
- *                   (SimInf_solver.c:824-902 run serially in the
+ *   k_coupled<M>    one lane per node, barrier-separated days, for models
+ *                   with E2 events: one replicate over a cooperative grid
+ *                   or one workgroup per replicate; each day's E2 events
+ *                   run level by level on an LDS gather of the herds they
+ *                   touch; the levels are conflict-free sets that keep
+ *                   each node's events in list order, so no atomics are
+ *                   needed (SimInf_solver.c:824-902 run serially in the
  *                   reference, SimInf_solver_ssm.c:244-250).
  */
-#include "siminf_device.h"
+#include "siminf_kernels.h"
 
 #include <hip/hip_runtime.h>
 
@@ -35,13 +35,13 @@ using namespace siminf;
 /* Built-in models as compile-time plugins.                            */
 /* ------------------------------------------------------------------ */
 
-#define SIMINF_RATE_ARGS const int *u, const double *v, const double *ld, \
-    const double *gd, double t
-#define SIMINF_PTS_ARGS double *vn, const int *u, const double *v, \
-    const double *ld, const double *gd, int node, double t
 
 struct ModelSIR {
-    static constexpr int NC = 3, NT = 2, ND = 0;
+    static constexpr int NC = 3, NT = 2, ND = 0, NLD = 2, NGD = 0;
+    static constexpr int S[NT][NC] =
+        {{-1, 1, 0}, {0, -1, 1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1}, {1, 1}};
     template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SIR_S_to_I(u, v, ld, gd, t);
@@ -54,7 +54,11 @@ struct ModelSIR {
 };
 
 struct ModelSEIR {
-    static constexpr int NC = 4, NT = 3, ND = 0;
+    static constexpr int NC = 4, NT = 3, ND = 0, NLD = 3, NGD = 0;
+    static constexpr int S[NT][NC] =
+        {{-1, 1, 0, 0}, {0, -1, 1, 0}, {0, 0, -1, 1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};
     template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SEIR_S_to_E(u, v, ld, gd, t);
@@ -68,7 +72,11 @@ struct ModelSEIR {
 };
 
 struct ModelSIS {
-    static constexpr int NC = 2, NT = 2, ND = 0;
+    static constexpr int NC = 2, NT = 2, ND = 0, NLD = 2, NGD = 0;
+    static constexpr int S[NT][NC] =
+        {{-1, 1}, {1, -1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1}, {1, 1}};
     template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SIS_S_to_I(u, v, ld, gd, t);
@@ -81,7 +89,11 @@ struct ModelSIS {
 };
 
 struct ModelSISe {
-    static constexpr int NC = 2, NT = 2, ND = 1;
+    static constexpr int NC = 2, NT = 2, ND = 1, NLD = 4, NGD = 8;
+    static constexpr int S[NT][NC] =
+        {{-1, 1}, {1, -1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1}, {1, 1}};
     template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SISe_S_to_I(u, v, ld, gd, t);
@@ -94,7 +106,13 @@ struct ModelSISe {
 };
 
 struct ModelSISe3 {
-    static constexpr int NC = 6, NT = 6, ND = 1;
+    static constexpr int NC = 6, NT = 6, ND = 1, NLD = 4, NGD = 12;
+    static constexpr int S[NT][NC] =
+        {{-1, 1, 0, 0, 0, 0}, {1, -1, 0, 0, 0, 0}, {0, 0, -1, 1, 0, 0},
+         {0, 0, 1, -1, 0, 0}, {0, 0, 0, 0, -1, 1}, {0, 0, 0, 0, 1, -1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1, 0, 0, 0, 0}, {1, 1, 0, 0, 0, 0}, {0, 0, 1, 1, 0, 0},
+         {0, 0, 1, 1, 0, 0}, {0, 0, 0, 0, 1, 1}, {0, 0, 0, 0, 1, 1}};
     template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SISe3_S_1_to_I_1(u, v, ld, gd, t);
@@ -111,7 +129,11 @@ struct ModelSISe3 {
 };
 
 struct ModelSIRmparse {
-    static constexpr int NC = 3, NT = 2, ND = 0;
+    static constexpr int NC = 3, NT = 2, ND = 0, NLD = 0, NGD = 2;
+    static constexpr int S[NT][NC] =
+        {{-1, 1, 0}, {0, -1, 1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1}, {1, 1}};
     template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SIR_mparse_trFun1(u, v, ld, gd, t);
@@ -123,514 +145,10 @@ struct ModelSIRmparse {
     }
 };
 
-/* ------------------------------------------------------------------ */
-/* Kernels.                                                            */
-/* ------------------------------------------------------------------ */
-
-/* E1 events of one node due at tt, in list order, drawing from the
- * node's stream.  Returns 0 or an error code; *bad gets the failing
- * event. */
-template <int NC>
-__device__ __forceinline__ int e1_day(const KParams &p, int &cur, int end,
-                                      double tt, int (&u)[NC],
-                                      siminf_stream *rng, bool &upd, int &bad)
-{
-    while (cur < end) {
-        const int k = p.e1_idx[cur];
-        if (p.ev_time[k] > tt)
-            break;
-        const int err = apply_e1<NC>(p, k, u, rng);
-        if (err) {
-            bad = k;
-            return err;
-        }
-        upd = true;
-        cur++;
-    }
-    return 0;
-}
-
-template <class M, bool EVENTS>
-__global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
-{
-    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long fired = 0;
-
-    if (g < p.lanes) {
-        const int rep = (int)(g / p.Nn);
-        const int node = (int)(g - (long long)rep * p.Nn);
-        const double *ld = p.ldata + (long long)node * p.Nld;
-        NodeState<M> s;
-        int err = 0;
-        double tt = p.tt0, next = p.next0;
-
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            s.u[c] = p.u0[g * NC + c];
-        });
-        if constexpr (ND > 0) {
-            static_for<ND>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.v[c] = p.v0[g * ND + c];
-                s.vnew[c] = s.v[c];
-            });
-        }
-        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
-
-        /* Initial rates (ssm.c:96-117). */
-        s.a0 = 0.0;
-        static_for<NT>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, tt);
-            s.rate[j] = r;
-            s.a0 += r;
-            if (!rate_ok(r))
-                err = SIMINF_ERR_INVALID_RATE;
-        });
-
-        if (err) {
-            record_error(p, rep, 0, PH_INIT, node, err);
-        } else {
-            int cur = p.e1_ptr[node];
-            const int end = p.e1_ptr[node + 1];
-            int U_it = 0, V_it = 0, day = 0;
-            for (;;) {
-                bool upd = false;
-                int bad = 0;
-
-                err = ssa_day<M>(s, p, ld, tt, next, fired);
-                if (err) {
-                    record_error(p, rep, day, PH_SSA, node, err);
-                    break;
-                }
-                if constexpr (EVENTS) {
-                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
-                    if (err) {
-                        record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
-                        break;
-                    }
-                }
-
-                /* Post time step and refresh (ssm.c:262-302). */
-                const int rc = M::pts(s.vnew, s.u, s.v, ld, p.gdata, node, tt);
-                if (rc < 0) {
-                    record_error(p, rep, day, PH_PTS, node, rc);
-                    break;
-                }
-                if (rc > 0 || upd) {
-                    err = refresh_rates<M>(s, p, ld, tt);
-                    if (err) {
-                        record_error(p, rep, day, PH_PTS, node, err);
-                        break;
-                    }
-                }
-
-                tt = next;
-                next += 1.0;
-                store_columns<NC, ND>(p, rep, node, tt, U_it, V_it, s.u, s.vnew);
-                if constexpr (ND > 0) {
-                    static_for<ND>([&](auto C) {
-                        constexpr int c = decltype(C)::value;
-                        const double x = s.v[c];
-                        s.v[c] = s.vnew[c];
-                        s.vnew[c] = x;
-                    });
-                }
-                if (U_it >= p.tlen)
-                    break;
-                day++;
-            }
-        }
-    }
-    add_transitions(p, fired);
-}
-
-template <class M, bool EVENTS>
-__global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
-{
-    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long fired = 0;
-
-    if (g < p.lanes) {
-        const int node = (int)g;
-        const double *ld = p.ldata + (long long)node * p.Nld;
-        const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
-        AemState<M> s;
-        int err = 0;
-        double tt = p.tt0, next = p.next0;
-
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            s.u[c] = p.u0[g * NC + c];
-        });
-        if constexpr (ND > 0) {
-            static_for<ND>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.v[c] = p.v0[g * ND + c];
-                s.vnew[c] = s.v[c];
-            });
-        }
-        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
-
-        /* Initial rates and reaction times (aem.c:117-153). */
-        static_for<NT>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, tt);
-            s.rate[j] = r;
-            if (!rate_ok(r))
-                err = SIMINF_ERR_INVALID_RATE;
-            const siminf_stream c = siminf_stream_make(
-                p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
-            double tj = -siminf_log(siminf_uniform_at(&c, 0)) / r + tt;
-            if (tj <= 0.0)
-                tj = INFINITY;
-            s.time[j] = tj;
-            s.inf[j] = 0.0;
-            s.j[j] = 1;
-        });
-
-        if (err) {
-            record_error(p, 0, 0, PH_INIT, node, err);
-        } else {
-            int cur = p.e1_ptr[node];
-            const int end = p.e1_ptr[node + 1];
-            int U_it = 0, V_it = 0, day = 0;
-            for (;;) {
-                bool upd = false;
-                int bad = 0;
-
-                err = aem_day<M>(s, p, ld, next, chan_base, fired);
-                if (err) {
-                    record_error(p, 0, day, PH_SSA, node, err);
-                    break;
-                }
-                if constexpr (EVENTS) {
-                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
-                    if (err) {
-                        record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
-                        break;
-                    }
-                }
-
-                /* Post time step; refresh through calcTimes at the node's
-                 * time, which is the next unit of time (aem.c:329-384). */
-                const int rc = M::pts(s.vnew, s.u, s.v, ld, p.gdata, node, tt);
-                if (rc < 0) {
-                    record_error(p, 0, day, PH_PTS, node, rc);
-                    break;
-                }
-                if (rc > 0 || upd) {
-                    static_for<NT>([&](auto J) {
-                        constexpr int j = decltype(J)::value;
-                        const double old = s.rate[j];
-                        const double r = M::template rate<j>(s.u, s.vnew, ld,
-                                                             p.gdata, tt);
-                        s.rate[j] = r;
-                        if (!rate_ok(r))
-                            err = SIMINF_ERR_INVALID_RATE;
-                        calc_times<M, j>(s, p, chan_base, next, old, r);
-                    });
-                    if (err) {
-                        record_error(p, 0, day, PH_PTS, node, err);
-                        break;
-                    }
-                }
-
-                tt = next;
-                next += 1.0;
-                store_columns<NC, ND>(p, 0, node, tt, U_it, V_it, s.u, s.vnew);
-                if constexpr (ND > 0) {
-                    static_for<ND>([&](auto C) {
-                        constexpr int c = decltype(C)::value;
-                        const double x = s.v[c];
-                        s.v[c] = s.vnew[c];
-                        s.vnew[c] = x;
-                    });
-                }
-                if (U_it >= p.tlen)
-                    break;
-                day++;
-            }
-        }
-    }
-    add_transitions(p, fired);
-}
-
-/* Replicate-per-workgroup ssm/mssm with external transfers.  Per day:
- *   A: each thread runs SSA + E1 for its nodes (state in registers while
- *      a node is processed, kept in global work arrays between days; u in
- *      LDS); barrier;
- *   B: the day's E2 events, one conflict-free level at a time, on the LDS
- *      copy of u; barrier after each level;
- *   C: post time step, refresh, writeback; merged with the next day's A
- *      for the same node, since both only touch that node. */
-template <class M>
-__global__ __launch_bounds__(1024) void k_ssm_reps(KParams p, int u_in_lds)
-{
-    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    extern __shared__ int lds[];
-
-    const int rep = blockIdx.x;
-    const int Nn = p.Nn;
-    const long long base = (long long)rep * Nn;
-    int *uw = u_in_lds ? lds : p.u_work + base * NC;
-    int *upd = u_in_lds ? lds + (long long)Nn * NC : p.e1_cursor + (long long)p.lanes + base;
-    unsigned long long fired = 0;
-    double tt = p.tt0, next = p.next0;
-    int U_it = 0, V_it = 0;
-    /* Errors are kept per thread and combined with __syncthreads_or at
-     * the end of each phase, so every thread leaves the day loop at the
-     * same barrier. */
-    int myerr = 0;
-
-    /* Initialisation. */
-    for (int node = threadIdx.x; node < Nn; node += blockDim.x) {
-        const long long g = base + node;
-        const double *ld = p.ldata + (long long)node * p.Nld;
-        NodeState<M> s;
-        int err = 0;
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            s.u[c] = p.u0[g * NC + c];
-            uw[node * NC + c] = s.u[c];
-        });
-        if constexpr (ND > 0) {
-            static_for<ND>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.v[c] = p.v0[g * ND + c];
-                p.v_work[g * ND + c] = s.v[c];
-                p.vnew_work[g * ND + c] = s.v[c];
-            });
-        }
-        s.a0 = 0.0;
-        static_for<NT>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const double r = M::template rate<j>(s.u, s.v, ld, p.gdata, tt);
-            p.rate_work[(long long)j * p.lanes + g] = r;
-            s.a0 += r;
-            if (!rate_ok(r))
-                err = SIMINF_ERR_INVALID_RATE;
-        });
-        p.a0_work[g] = s.a0;
-        p.rng_j[g] = 0;
-        p.rng_cache[g] = 0.0;
-        p.e1_cursor[g] = p.e1_ptr[node];
-        upd[node] = 0;
-        if (err) {
-            record_error(p, rep, 0, PH_INIT, node, err);
-            myerr = 1;
-        }
-    }
-    if (__syncthreads_or(myerr))
-        return;
-
-    for (int day = 0;; day++) {
-        /* The buffers holding v and v_new swap every day (ssm.c:350-353). */
-        double *vbuf = (day & 1) ? p.vnew_work : p.v_work;
-        double *vnbuf = (day & 1) ? p.v_work : p.vnew_work;
-
-        /* Phase A: SSA and E1 events of this thread's nodes. */
-        for (int node = threadIdx.x; node < Nn; node += blockDim.x) {
-            const long long g = base + node;
-            const double *ld = p.ldata + (long long)node * p.Nld;
-            NodeState<M> s;
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.u[c] = uw[node * NC + c];
-            });
-            if constexpr (ND > 0) {
-                static_for<ND>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    s.v[c] = vbuf[g * ND + c];
-                });
-            }
-            static_for<NT>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                s.rate[j] = p.rate_work[(long long)j * p.lanes + g];
-            });
-            s.a0 = p.a0_work[g];
-            s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
-            s.rng.j = p.rng_j[g];
-            s.rng.cache = p.rng_cache[g];
-
-            int err = ssa_day<M>(s, p, ld, tt, next, fired);
-            if (err) {
-                record_error(p, rep, day, PH_SSA, node, err);
-                myerr = 1;
-            } else {
-                int cur = p.e1_cursor[g], bad = 0;
-                bool e1 = false;
-                err = e1_day<NC>(p, cur, p.e1_ptr[node + 1], tt, s.u, &s.rng,
-                                 e1, bad);
-                if (err) {
-                    record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
-                    myerr = 1;
-                }
-                p.e1_cursor[g] = cur;
-                if (e1)
-                    upd[node] = 1;
-            }
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                uw[node * NC + c] = s.u[c];
-            });
-            static_for<NT>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                p.rate_work[(long long)j * p.lanes + g] = s.rate[j];
-            });
-            p.a0_work[g] = s.a0;
-            p.rng_j[g] = s.rng.j;
-            p.rng_cache[g] = s.rng.cache;
-        }
-        if (__syncthreads_or(myerr))
-            break;
-
-        /* Phase B: external transfers, level by level. */
-        if (day < p.ndays) {
-            for (int l = p.e2_day[day]; l < p.e2_day[day + 1]; l++) {
-                for (int i = p.e2_lvl[l] + threadIdx.x; i < p.e2_lvl[l + 1];
-                     i += blockDim.x) {
-                    const int k = p.e2_idx[i];
-                    siminf_stream rng = siminf_stream_make(
-                        p.seed, SIMINF_DOMAIN_E2,
-                        (uint64_t)rep * (uint64_t)p.len + (uint64_t)k);
-                    const int err = apply_e2<NC>(p, k, uw, upd, &rng);
-                    if (err) {
-                        record_error(p, rep, day, PH_E2, p.ev_rank[k], err);
-                        myerr = 1;
-                    }
-                }
-                /* Later levels still run after a failure: an event that
-                 * fails there can only be later in list order than the
-                 * first failure of its own chain, so the smallest key is
-                 * still the serial run's first failure. */
-                __syncthreads();
-            }
-        }
-        if (__syncthreads_or(myerr))
-            break;
-
-        /* Phase C: post time step, refresh, writeback. */
-        const double tt_next = next;
-        for (int node = threadIdx.x; node < Nn; node += blockDim.x) {
-            const long long g = base + node;
-            const double *ld = p.ldata + (long long)node * p.Nld;
-            NodeState<M> s;
-            int err = 0;
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.u[c] = uw[node * NC + c];
-            });
-            if constexpr (ND > 0) {
-                static_for<ND>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    s.v[c] = vbuf[g * ND + c];
-                    s.vnew[c] = vnbuf[g * ND + c];
-                });
-            }
-            const int rc = M::pts(s.vnew, s.u, s.v, ld, p.gdata, node, tt);
-            if (rc < 0) {
-                record_error(p, rep, day, PH_PTS, node, rc);
-                myerr = 1;
-            } else if (rc > 0 || upd[node]) {
-                static_for<NT>([&](auto J) {
-                    constexpr int j = decltype(J)::value;
-                    s.rate[j] = p.rate_work[(long long)j * p.lanes + g];
-                });
-                s.a0 = p.a0_work[g];
-                err = refresh_rates<M>(s, p, ld, tt);
-                if (err) {
-                    record_error(p, rep, day, PH_PTS, node, err);
-                    myerr = 1;
-                }
-                static_for<NT>([&](auto J) {
-                    constexpr int j = decltype(J)::value;
-                    p.rate_work[(long long)j * p.lanes + g] = s.rate[j];
-                });
-                p.a0_work[g] = s.a0;
-                upd[node] = 0;
-            }
-            if constexpr (ND > 0) {
-                static_for<ND>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    vnbuf[g * ND + c] = s.vnew[c];
-                });
-            }
-            int Ui = U_it, Vi = V_it;
-            store_columns<NC, ND>(p, rep, node, tt_next, Ui, Vi, s.u, s.vnew);
-        }
-
-        tt = next;
-        next += 1.0;
-        while (U_it < p.tlen && tt > p.tspan[U_it])
-            U_it++;
-        while (V_it < p.tlen && tt > p.tspan[V_it])
-            V_it++;
-        if (__syncthreads_or(myerr) || U_it >= p.tlen)
-            break;
-    }
-    add_transitions(p, fired);
-}
-
-/* ------------------------------------------------------------------ */
-/* Model registry.                                                     */
-/* ------------------------------------------------------------------ */
-
-struct SimInf_hip_model {
-    const char *name;
-    int Nc, Nt, Nd, Ngdata;
-    void (*ssm_nodes)(const KParams &, bool, unsigned, unsigned, hipStream_t);
-    void (*aem_nodes)(const KParams &, bool, unsigned, unsigned, hipStream_t);
-    hipError_t (*ssm_reps)(const KParams &, int, unsigned, unsigned, size_t,
-                           hipStream_t);
-};
-
-/* Models without E1 events get a kernel without the event code, which
- * needs fewer registers. */
-template <class M>
-static void launch_ssm_nodes(const KParams &p, bool events, unsigned grid,
-                             unsigned block, hipStream_t st)
-{
-    if (events)
-        hipLaunchKernelGGL((k_ssm_nodes<M, true>), dim3(grid), dim3(block), 0, st, p);
-    else
-        hipLaunchKernelGGL((k_ssm_nodes<M, false>), dim3(grid), dim3(block), 0, st, p);
-}
-
-template <class M>
-static void launch_aem_nodes(const KParams &p, bool events, unsigned grid,
-                             unsigned block, hipStream_t st)
-{
-    if (events)
-        hipLaunchKernelGGL((k_aem_nodes<M, true>), dim3(grid), dim3(block), 0, st, p);
-    else
-        hipLaunchKernelGGL((k_aem_nodes<M, false>), dim3(grid), dim3(block), 0, st, p);
-}
-
-template <class M>
-static hipError_t launch_ssm_reps(const KParams &p, int u_in_lds, unsigned grid,
-                                  unsigned block, size_t lds, hipStream_t st)
-{
-    if (lds > 0) {
-        const hipError_t e = hipFuncSetAttribute(
-            (const void *)k_ssm_reps<M>,
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess)
-            return e;
-    }
-    hipLaunchKernelGGL(k_ssm_reps<M>, dim3(grid), dim3(block), lds, st, p,
-                       u_in_lds);
-    return hipGetLastError();
-}
-
-#define SIMINF_MODEL_ENTRY(NAME, M, NGDATA)                               \
-    {NAME, M::NC, M::NT, M::ND, NGDATA, launch_ssm_nodes<M>,              \
-     launch_aem_nodes<M>, launch_ssm_reps<M>}
-
 /* gdata lengths: SISe.R:177-178 (8 values), SISe3.R:188-190 (12), the
- * mparse SIR (beta, gamma); SIR, SEIR and SIS read ldata only. */
+ * mparse SIR (beta, gamma); SIR, SEIR and SIS read ldata only.  S and G
+ * are the constructors' matrices (SIR.R:127-133, SEIR.R:136-143,
+ * SIS.R:122-128, SISe.R:146-151, SISe3.R:162-178). */
 static const SimInf_hip_model builtin_models[] = {
     SIMINF_MODEL_ENTRY("SIR", ModelSIR, 0),
     SIMINF_MODEL_ENTRY("SEIR", ModelSEIR, 0),
@@ -667,7 +185,7 @@ extern "C" int SimInf_hip_model_dims(const SimInf_hip_model *model, int *Nc,
 
 namespace {
 
-enum Kernel { K_SSM_NODES, K_AEM_NODES, K_SSM_REPS };
+enum Kernel { K_SSM_NODES, K_AEM_NODES, K_COUPLED_GRID, K_COUPLED_REPS };
 
 struct DeviceBuffers {
     std::vector<void *> ptrs;
@@ -714,8 +232,8 @@ struct SimInf_hip_plan {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     unsigned grid = 0, block = 0;
     size_t lds = 0;
-    int u_in_lds = 0;
     bool has_e1 = false;
+    bool canon = false;
 
     /* Host copies needed after runs. */
     int Nn, Nc, Nd, Nrep, tlen;
@@ -765,6 +283,8 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
         return SIMINF_ERR_INVALID_MODEL;
     *out = nullptr;
 
+    if (model->abi != SIMINF_HIP_PLUGIN_ABI)
+        return SIMINF_ERR_INVALID_MODEL;
     const int Nn = a->Nn, Nc = a->Nc, Nt = a->Nt, Nd = a->Nd;
     const int Nrep = a->Nrep < 1 ? 1 : a->Nrep;
     if (Nn <= 0 || Nc != model->Nc || Nt != model->Nt || Nd != model->Nd ||
@@ -882,7 +402,9 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     /* E2 levels: an event's level is one more than the last level that
      * touched its source or destination on the same day. */
     std::vector<int> e2_day(ndays + 1, 0), e2_lvl(1, 0), e2_idx;
-    int n_e2 = 0;
+    std::vector<int> e2_tn_ptr(ndays + 1, 0), e2_tn, e2_lsrc(std::max(len, 1), -1),
+        e2_ldst(std::max(len, 1), -1);
+    int n_e2 = 0, max_touched = 0;
     {
         std::vector<int> last_level(Nn, -1), last_day(Nn, -1);
         std::vector<std::vector<int>> levels;
@@ -914,6 +436,26 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
                 levels[lvl].push_back(k);
                 n_e2++;
             }
+            /* Herds touched today, and each event's slots among them. */
+            const int t0 = (int)e2_tn.size();
+            for (auto &lv : levels) {
+                for (int ev : lv) {
+                    for (int side = 0; side < 2; side++) {
+                        const int nd = side ? ev_dest[ev] : ev_node[ev];
+                        int &slot = side ? e2_ldst[ev] : e2_lsrc[ev];
+                        if (nd < 0 || nd >= Nn)
+                            continue;
+                        if (last_day[nd] != d + ndays + 1) {
+                            last_day[nd] = d + ndays + 1;
+                            last_level[nd] = (int)e2_tn.size() - t0;
+                            e2_tn.push_back(nd);
+                        }
+                        slot = last_level[nd];
+                    }
+                }
+            }
+            e2_tn_ptr[d + 1] = (int)e2_tn.size();
+            max_touched = std::max(max_touched, (int)e2_tn.size() - t0);
             for (auto &lv : levels) {
                 e2_idx.insert(e2_idx.end(), lv.begin(), lv.end());
                 e2_lvl.push_back((int)e2_idx.size());
@@ -922,6 +464,27 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
         }
         if (e2_idx.empty())
             e2_idx.push_back(0);
+        if (e2_tn.empty())
+            e2_tn.push_back(0);
+    }
+
+    /* The fast kernels fold the model's own S and G into the code; any
+     * other S or G (a caller may pass modified matrices) runs the
+     * variant that reads them from the arguments. */
+    {
+        bool same = true;
+        for (int k = 0; k < Nt && same; k++) {
+            for (int c = 0; c < Nc; c++) {
+                const bool stored = (Smask[k] >> c) & 1u;
+                const int v = stored ? Sval[(size_t)k * Nc + c] : 0;
+                if (stored != (model->S[k * Nc + c] != 0) || v != model->S[k * Nc + c])
+                    same = false;
+            }
+            for (int j = 0; j < Nt; j++)
+                if ((((Gmask[k] >> j) & 1u) != 0) != (model->G[k * Nt + j] != 0))
+                    same = false;
+        }
+        P->canon = same;
     }
 
     /* Kernel choice. */
@@ -932,8 +495,11 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
             return SIMINF_ERR_UNKNOWN_SOLVER;
         }
         P->kernel = K_AEM_NODES;
+    } else if (n_e2 == 0) {
+        P->kernel = K_SSM_NODES;
     } else {
-        P->kernel = n_e2 > 0 ? K_SSM_REPS : K_SSM_NODES;
+        P->kernel = (Nrep == 1 || Nn > SIMINF_REP_BLOCK) ? K_COUPLED_GRID
+                                                         : K_COUPLED_REPS;
     }
 
     /* Device buffers. */
@@ -991,6 +557,10 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     p.e2_day = B.upload(e2_day.data(), e2_day.size());
     p.e2_lvl = B.upload(e2_lvl.data(), e2_lvl.size());
     p.e2_idx = B.upload(e2_idx.data(), e2_idx.size());
+    p.e2_tn_ptr = B.upload(e2_tn_ptr.data(), e2_tn_ptr.size());
+    p.e2_tn = B.upload(e2_tn.data(), e2_tn.size());
+    p.e2_lsrc = B.upload(e2_lsrc.data(), e2_lsrc.size());
+    p.e2_ldst = B.upload(e2_ldst.data(), e2_ldst.size());
 
     const size_t ncols = (size_t)Nrep * a->tlen;
     if (a->U) {
@@ -1012,30 +582,29 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     p.err_key = B.alloc<unsigned long long>(1);
     p.transitions = B.alloc<unsigned long long>(1);
 
-    if (P->kernel == K_SSM_REPS) {
-        p.v_work = B.alloc<double>((size_t)lanes * Nd);
-        p.vnew_work = B.alloc<double>((size_t)lanes * Nd);
-        p.rate_work = B.alloc<double>((size_t)lanes * Nt);
-        p.a0_work = B.alloc<double>((size_t)lanes);
-        p.rng_j = B.alloc<unsigned long long>((size_t)lanes);
-        p.rng_cache = B.alloc<double>((size_t)lanes);
-        /* e1 cursors, then the update flags when u is not in LDS. */
-        p.e1_cursor = B.alloc<int>((size_t)lanes * 2);
-        int dev = 0, max_lds = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&max_lds,
-                                    hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-        const size_t need = (size_t)Nn * (Nc + 1) * sizeof(int);
-        if (need + 1024 <= (size_t)max_lds) {
-            P->u_in_lds = 1;
-            P->lds = need;
+    if (P->kernel == K_COUPLED_GRID || P->kernel == K_COUPLED_REPS) {
+        p.u_work = B.alloc<int>((size_t)lanes * Nc);
+        p.upd_work = B.alloc<int>((size_t)lanes);
+        /* LDS holds the herds one day's E2 events touch (u and a flag);
+         * days that touch more run on the global state. */
+        const int cap = std::min(max_touched, 48 * 1024 / (int)((Nc + 1) * sizeof(int)));
+        p.e2_max_touched = cap;
+        P->lds = (size_t)std::max(cap, 1) * (Nc + 1) * sizeof(int);
+        if (P->kernel == K_COUPLED_GRID) {
+            P->block = SIMINF_GRID_BLOCK;
+            P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK);
+            int max_blocks = 0;
+            if (model->coupled(p, P->canon, true, P->grid, P->block, P->lds,
+                               nullptr, &max_blocks) != hipSuccess ||
+                (int)P->grid > max_blocks) {
+                /* Too many nodes for one co-resident grid. */
+                delete P;
+                return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+            }
         } else {
-            P->u_in_lds = 0;
-            P->lds = 0;
-            p.u_work = B.alloc<int>((size_t)lanes * Nc);
+            P->block = (unsigned)(((Nn + 63) / 64) * 64);
+            P->grid = (unsigned)Nrep;
         }
-        P->block = (unsigned)std::min(1024, ((Nn + 63) / 64) * 64);
-        P->grid = (unsigned)Nrep;
     } else {
         P->block = 256;
         P->grid = (unsigned)((lanes + 255) / 256);
@@ -1072,14 +641,23 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
     (void)hipEventRecord(P->ev_start, P->stream);
     switch (P->kernel) {
     case K_SSM_NODES:
-        P->model->ssm_nodes(p, P->has_e1, P->grid, P->block, P->stream);
+        P->model->ssm_nodes(p, P->has_e1, P->canon, P->grid, P->block, P->stream);
         break;
     case K_AEM_NODES:
-        P->model->aem_nodes(p, P->has_e1, P->grid, P->block, P->stream);
+        P->model->aem_nodes(p, P->has_e1, P->canon, P->grid, P->block, P->stream);
         break;
-    case K_SSM_REPS:
-        if (P->model->ssm_reps(p, P->u_in_lds, P->grid, P->block, P->lds,
-                               P->stream) != hipSuccess)
+    case K_COUPLED_GRID:
+        /* One cooperative launch per replicate. */
+        for (int r = 0; r < P->Nrep; r++) {
+            p.rep0 = r;
+            if (P->model->coupled(p, P->canon, true, P->grid, P->block, P->lds,
+                                  P->stream, nullptr) != hipSuccess)
+                return SIMINF_ERR_DEVICE;
+        }
+        break;
+    case K_COUPLED_REPS:
+        if (P->model->coupled(p, P->canon, false, P->grid, P->block, P->lds,
+                              P->stream, nullptr) != hipSuccess)
             return SIMINF_ERR_DEVICE;
         break;
     }

@@ -1,0 +1,636 @@
+/*
+ * siminf_kernels.h -- the solver kernels as templates over a model
+ * plugin, and the plugin record the library dispatches through.  Built-in
+ * models instantiate them in siminf_hip.hip; a model generated by the
+ * parser (siminf_amd/mparse.py) instantiates them in its own shared object
+ * and hands the library a SimInf_hip_model.
+ *
+ * A model plugin M provides NC, NT, ND, NLD, NGD (ldata and gdata values
+ * its functions read), its canonical tables S[NT][NC] (S[k][c] = S(c,k))
+ * and G[NT][NT] (G[k][j] = G(j,k)), rate<J>(u, v, ldata, gdata, t) and
+ * pts(v_new, u, v, ldata, gdata, node, t) as __device__ functions.
+ */
+#pragma once
+
+#include "siminf_device.h"
+
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+namespace siminf {
+
+#define SIMINF_RATE_ARGS const int *u, const double *v, const double *ld, \
+    const double *gd, double t
+#define SIMINF_PTS_ARGS double *vn, const int *u, const double *v, \
+    const double *ld, const double *gd, int node, double t
+
+/* ------------------------------------------------------------------ */
+/* Kernels.                                                            */
+/* ------------------------------------------------------------------ */
+
+/* E1 events of one node due at tt, in list order, drawing from the
+ * node's stream.  Returns 0 or an error code; *bad gets the failing
+ * event. */
+template <int NC>
+__device__ __forceinline__ int e1_day(const KParams &p, int &cur, int end,
+                                      double tt, int (&u)[NC],
+                                      siminf_stream *rng, bool &upd, int &bad)
+{
+    while (cur < end) {
+        const int k = p.e1_idx[cur];
+        if (p.ev_time[k] > tt)
+            break;
+        const int err = apply_e1<NC>(p, k, u, rng);
+        if (err) {
+            bad = k;
+            return err;
+        }
+        upd = true;
+        cur++;
+    }
+    return 0;
+}
+
+template <class M, bool EVENTS, bool CANON>
+__global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long fired = 0;
+
+    if (g < p.lanes) {
+        const int rep = (int)(g / p.Nn);
+        const int node = (int)(g - (long long)rep * p.Nn);
+        Ctx<M, CANON> cx;
+        cx.load(p, node);
+        NodeState<M> s;
+        int err = 0;
+        double tt = p.tt0, next = p.next0;
+
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            s.u[c] = p.u0[g * NC + c];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.v[c] = p.v0[g * ND + c];
+                s.vnew[c] = s.v[c];
+            });
+        }
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+
+        /* Initial rates (ssm.c:96-117). */
+        s.a0 = 0.0;
+        static_for<NT>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+            s.rate[j] = r;
+            s.a0 += r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+        });
+
+        if (err) {
+            record_error(p, rep, 0, PH_INIT, node, err);
+        } else {
+            int cur = p.e1_ptr[node];
+            const int end = p.e1_ptr[node + 1];
+            int U_it = 0, V_it = 0, day = 0;
+            for (;;) {
+                bool upd = false;
+                int bad = 0;
+
+                err = ssa_day<M>(s, cx, tt, next, fired);
+                if (err) {
+                    record_error(p, rep, day, PH_SSA, node, err);
+                    break;
+                }
+                if constexpr (EVENTS) {
+                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                    if (err) {
+                        record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
+                        break;
+                    }
+                }
+
+                /* Post time step and refresh (ssm.c:262-302). */
+                const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+                if (rc < 0) {
+                    record_error(p, rep, day, PH_PTS, node, rc);
+                    break;
+                }
+                if (rc > 0 || upd) {
+                    err = refresh_rates<M>(s, cx, tt);
+                    if (err) {
+                        record_error(p, rep, day, PH_PTS, node, err);
+                        break;
+                    }
+                }
+
+                tt = next;
+                next += 1.0;
+                store_columns<NC, ND>(p, rep, node, tt, U_it, V_it, s.u, s.vnew);
+                if constexpr (ND > 0) {
+                    static_for<ND>([&](auto C) {
+                        constexpr int c = decltype(C)::value;
+                        const double x = s.v[c];
+                        s.v[c] = s.vnew[c];
+                        s.vnew[c] = x;
+                    });
+                }
+                if (U_it >= p.tlen)
+                    break;
+                day++;
+            }
+        }
+    }
+    add_transitions(p, fired);
+}
+
+template <class M, bool EVENTS, bool CANON>
+__global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long fired = 0;
+
+    if (g < p.lanes) {
+        const int node = (int)g;
+        Ctx<M, CANON> cx;
+        cx.load(p, node);
+        const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
+        AemState<M> s;
+        int err = 0;
+        double tt = p.tt0, next = p.next0;
+
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            s.u[c] = p.u0[g * NC + c];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.v[c] = p.v0[g * ND + c];
+                s.vnew[c] = s.v[c];
+            });
+        }
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+
+        /* Initial rates and reaction times (aem.c:117-153). */
+        static_for<NT>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+            s.rate[j] = r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+            const siminf_stream c = siminf_stream_make(
+                p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
+            double tj = -siminf_log(siminf_uniform_at(&c, 0)) / r + tt;
+            if (tj <= 0.0)
+                tj = INFINITY;
+            s.time[j] = tj;
+            s.inf[j] = 0.0;
+            s.j[j] = 1;
+        });
+
+        if (err) {
+            record_error(p, 0, 0, PH_INIT, node, err);
+        } else {
+            int cur = p.e1_ptr[node];
+            const int end = p.e1_ptr[node + 1];
+            int U_it = 0, V_it = 0, day = 0;
+            for (;;) {
+                bool upd = false;
+                int bad = 0;
+
+                err = aem_day<M>(s, cx, next, chan_base, fired);
+                if (err) {
+                    record_error(p, 0, day, PH_SSA, node, err);
+                    break;
+                }
+                if constexpr (EVENTS) {
+                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                    if (err) {
+                        record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
+                        break;
+                    }
+                }
+
+                /* Post time step; refresh through calcTimes at the node's
+                 * time, which is the next unit of time (aem.c:329-384). */
+                const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+                if (rc < 0) {
+                    record_error(p, 0, day, PH_PTS, node, rc);
+                    break;
+                }
+                if (rc > 0 || upd) {
+                    static_for<NT>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        const double old = s.rate[j];
+                        const double r = M::template rate<j>(s.u, s.vnew, cx.ld,
+                                                             cx.gd, tt);
+                        s.rate[j] = r;
+                        if (!rate_ok(r))
+                            err = SIMINF_ERR_INVALID_RATE;
+                        calc_times<M, j>(s, cx.seed, chan_base, next, old, r);
+                    });
+                    if (err) {
+                        record_error(p, 0, day, PH_PTS, node, err);
+                        break;
+                    }
+                }
+
+                tt = next;
+                next += 1.0;
+                store_columns<NC, ND>(p, 0, node, tt, U_it, V_it, s.u, s.vnew);
+                if constexpr (ND > 0) {
+                    static_for<ND>([&](auto C) {
+                        constexpr int c = decltype(C)::value;
+                        const double x = s.v[c];
+                        s.v[c] = s.vnew[c];
+                        s.vnew[c] = x;
+                    });
+                }
+                if (U_it >= p.tlen)
+                    break;
+                day++;
+            }
+        }
+    }
+    add_transitions(p, fired);
+}
+
+/* ------------------------------------------------------------------ */
+/* Coupled models: external transfers (E2) link the nodes every day.    */
+/* ------------------------------------------------------------------ */
+
+/* Has an error been recorded at or before (rep, day, phase)?  Read after
+ * a barrier, this gives every block the same answer: errors of later
+ * phases, which other blocks may be writing meanwhile, are ignored. */
+__device__ __forceinline__ bool error_by(const KParams &p, int rep, int day,
+                                         int phase)
+{
+    const unsigned long long k = __hip_atomic_load(
+        p.err_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (k >> 27) <= (error_key(rep, day, phase, 0, -1) >> 27);
+}
+
+/* One day's external transfers of replicate rep, run by one workgroup.
+ * The herds they touch are gathered into LDS, the conflict-free levels
+ * run there with a workgroup barrier between levels, and the herds are
+ * scattered back with their update flags.  Events in one level touch
+ * disjoint herds, so no atomics are needed; within a herd the events
+ * keep list order.  When the touched set does not fit the LDS buffer the
+ * levels run on the global state directly.  Returns 1 if this thread
+ * recorded an error. */
+template <int NC>
+__device__ int e2_day(const KParams &p, int rep, int day, int *uw, int *updw,
+                      int *lu, int *lupd, int lds_cap)
+{
+    int myerr = 0;
+    if (day >= p.ndays || p.e2_day[day] == p.e2_day[day + 1])
+        return 0;
+    const int t0 = p.e2_tn_ptr[day], nt = p.e2_tn_ptr[day + 1] - t0;
+    const bool in_lds = nt <= lds_cap;
+    if (in_lds) {
+        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+            const int node = p.e2_tn[t0 + i];
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                lu[i * NC + c] = uw[node * NC + c];
+            });
+            lupd[i] = 0;
+        }
+        __syncthreads();
+    }
+    for (int l = p.e2_day[day]; l < p.e2_day[day + 1]; l++) {
+        for (int i = p.e2_lvl[l] + threadIdx.x; i < p.e2_lvl[l + 1];
+             i += blockDim.x) {
+            const int k = p.e2_idx[i];
+            siminf_stream rng = siminf_stream_make(
+                p.seed, SIMINF_DOMAIN_E2,
+                (uint64_t)rep * (uint64_t)p.len + (uint64_t)k);
+            /* One code path for both places: generic pointers address
+             * LDS and global memory alike. */
+            int *su = in_lds ? lu : uw;
+            int *sf = in_lds ? lupd : updw;
+            const int src = in_lds ? p.e2_lsrc[k] : p.ev_node[k];
+            const int dst = in_lds ? p.e2_ldst[k] : p.ev_dest[k];
+            const int err = apply_e2<NC>(p, k, su, sf, src, dst, &rng);
+            if (err) {
+                record_error(p, rep, day, PH_E2, p.ev_rank[k], err);
+                myerr = 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (in_lds) {
+        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+            const int node = p.e2_tn[t0 + i];
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                uw[node * NC + c] = lu[i * NC + c];
+            });
+            if (lupd[i])
+                updw[node] = 1;
+        }
+    }
+    return myerr;
+}
+
+/* Day loop with one lane per node and the node's state resident in VGPRs
+ * across days.  GRID: one replicate spread over all workgroups of a
+ * cooperative launch, joined by grid barriers; E2 runs in workgroup 0.
+ * Otherwise: one workgroup per replicate, joined by workgroup barriers.
+ * Per day:  A  SSA + E1 of the lane's node, its u published to global
+ *              memory;                                   barrier
+ *           B  the day's E2 events (e2_day);              barrier
+ *           C  reload u, post time step, refresh, writeback.
+ * (SimInf_solver_ssm.c:134-362, SimInf_solver_mssm.c:158-383.) */
+template <class M, bool CANON, bool GRID, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    extern __shared__ int lds[];
+    const int rep = GRID ? p.rep0 : (int)blockIdx.x;
+    const int node = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
+                          : (int)threadIdx.x;
+    const bool active = node < p.Nn;
+    const long long base = (long long)rep * p.Nn;
+    const long long g = base + node;
+    int *uw = p.u_work + base * NC;
+    int *updw = p.upd_work + base;
+    int *lu = lds;
+    int *lupd = lds + p.e2_max_touched * NC;
+    unsigned long long fired = 0;
+    double tt = p.tt0, next = p.next0;
+    int U_it = 0, V_it = 0;
+    int myerr = 0;
+
+    NodeState<M> s;
+    Ctx<M, CANON> cx;
+    int cur = 0, end = 0;
+
+    auto barrier = [&]() {
+        if constexpr (GRID) {
+            /* Publish this lane's writes device-wide before the grid
+             * barrier and observe the others' after it. */
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            cooperative_groups::this_grid().sync();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        } else {
+            __syncthreads();
+        }
+    };
+    /* Uniform exit decision after a barrier. */
+    auto failed = [&](int day, int phase) -> bool {
+        if constexpr (GRID)
+            return error_by(p, rep, day, phase);
+        else
+            return __syncthreads_or(myerr) != 0;
+    };
+
+    if (active) {
+        cx.load(p, node);
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            s.u[c] = p.u0[g * NC + c];
+            uw[node * NC + c] = s.u[c];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.v[c] = p.v0[g * ND + c];
+                s.vnew[c] = s.v[c];
+            });
+        }
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+        s.a0 = 0.0;
+        int err = 0;
+        static_for<NT>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+            s.rate[j] = r;
+            s.a0 += r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+        });
+        updw[node] = 0;
+        cur = p.e1_ptr[node];
+        end = p.e1_ptr[node + 1];
+        if (err) {
+            record_error(p, rep, 0, PH_INIT, node, err);
+            myerr = 1;
+        }
+    }
+    barrier();
+    if (failed(0, PH_INIT))
+        return;
+
+    for (int day = 0;; day++) {
+        /* A: SSA and E1 events. */
+        if (active) {
+            bool upd = false;
+            int bad = 0;
+            int err = ssa_day<M>(s, cx, tt, next, fired);
+            if (err) {
+                record_error(p, rep, day, PH_SSA, node, err);
+                myerr = 1;
+            } else {
+                err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                if (err) {
+                    record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
+                    myerr = 1;
+                }
+            }
+            if (upd)
+                updw[node] = 1;
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                uw[node * NC + c] = s.u[c];
+            });
+        }
+        barrier();
+        if (failed(day, PH_E1))
+            break;
+
+        /* B: external transfers in one workgroup. */
+        if (!GRID || blockIdx.x == 0)
+            myerr |= e2_day<NC>(p, rep, day, uw, updw, lu, lupd,
+                                p.e2_max_touched);
+        barrier();
+        if (failed(day, PH_E2))
+            break;
+
+        /* C: post time step, refresh, writeback. */
+        if (active) {
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.u[c] = uw[node * NC + c];
+            });
+            const bool upd = updw[node] != 0;
+            updw[node] = 0;
+            const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+            if (rc < 0) {
+                record_error(p, rep, day, PH_PTS, node, rc);
+                myerr = 1;
+            } else if (rc > 0 || upd) {
+                const int err = refresh_rates<M>(s, cx, tt);
+                if (err) {
+                    record_error(p, rep, day, PH_PTS, node, err);
+                    myerr = 1;
+                }
+            }
+        }
+        tt = next;
+        next += 1.0;
+        if (active) {
+            store_columns<NC, ND>(p, rep, node, tt, U_it, V_it, s.u, s.vnew);
+            if constexpr (ND > 0) {
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    const double x = s.v[c];
+                    s.v[c] = s.vnew[c];
+                    s.vnew[c] = x;
+                });
+            }
+        } else {
+            while (U_it < p.tlen && tt > p.tspan[U_it])
+                U_it++;
+        }
+        if (U_it >= p.tlen)
+            break;
+    }
+    add_transitions(p, fired);
+}
+
+/* Layout version of SimInf_hip_model, checked when a plugin is used. */
+#define SIMINF_HIP_PLUGIN_ABI 1
+
+} /* namespace siminf */
+
+struct SimInf_hip_model {
+    int abi;
+    const char *name;
+    int Nc, Nt, Nd, Ngdata;
+    const int *S;    /* canonical S: S[k*Nc + c] = S(c, k) */
+    const int *G;    /* canonical G: G[k*Nt + j] = G(j, k) */
+    void (*ssm_nodes)(const siminf::KParams &, bool, bool, unsigned, unsigned,
+                      hipStream_t);
+    void (*aem_nodes)(const siminf::KParams &, bool, bool, unsigned, unsigned,
+                      hipStream_t);
+    hipError_t (*coupled)(const siminf::KParams &, bool, bool, unsigned,
+                          unsigned, size_t, hipStream_t, int *);
+};
+
+namespace siminf {
+
+/* Models without E1 events get a kernel without the event code, which
+ * needs fewer registers. */
+template <class M>
+static void launch_ssm_nodes(const KParams &p, bool events, bool canon,
+                             unsigned grid, unsigned block, hipStream_t st)
+{
+    if (events && canon)
+        hipLaunchKernelGGL((k_ssm_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
+    else if (events)
+        hipLaunchKernelGGL((k_ssm_nodes<M, true, false>), dim3(grid), dim3(block), 0, st, p);
+    else if (canon)
+        hipLaunchKernelGGL((k_ssm_nodes<M, false, true>), dim3(grid), dim3(block), 0, st, p);
+    else
+        hipLaunchKernelGGL((k_ssm_nodes<M, false, false>), dim3(grid), dim3(block), 0, st, p);
+}
+
+template <class M>
+static void launch_aem_nodes(const KParams &p, bool events, bool canon,
+                             unsigned grid, unsigned block, hipStream_t st)
+{
+    if (events && canon)
+        hipLaunchKernelGGL((k_aem_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
+    else if (events)
+        hipLaunchKernelGGL((k_aem_nodes<M, true, false>), dim3(grid), dim3(block), 0, st, p);
+    else if (canon)
+        hipLaunchKernelGGL((k_aem_nodes<M, false, true>), dim3(grid), dim3(block), 0, st, p);
+    else
+        hipLaunchKernelGGL((k_aem_nodes<M, false, false>), dim3(grid), dim3(block), 0, st, p);
+}
+
+/* Block sizes of the coupled kernel: 64 lanes per workgroup when one
+ * replicate spans the grid, up to 512 lanes (one per node) when a
+ * workgroup holds a replicate. */
+#define SIMINF_GRID_BLOCK 64
+#define SIMINF_REP_BLOCK 512
+
+template <class M, bool CANON, bool GRID>
+static const void *coupled_fn()
+{
+    return GRID ? (const void *)k_coupled<M, CANON, true, SIMINF_GRID_BLOCK>
+                : (const void *)k_coupled<M, CANON, false, SIMINF_REP_BLOCK>;
+}
+
+/* Launch the coupled kernel.  With grid, a cooperative launch of `grid`
+ * workgroups that must all be resident; max_blocks (out, optional)
+ * reports how many can be, for the caller's planning. */
+template <class M>
+static hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
+                                 unsigned grid, unsigned block, size_t lds,
+                                 hipStream_t st, int *max_blocks)
+{
+    const void *fn = grid_mode ? (canon ? coupled_fn<M, true, true>()
+                                        : coupled_fn<M, false, true>())
+                               : (canon ? coupled_fn<M, true, false>()
+                                        : coupled_fn<M, false, false>());
+    hipError_t e;
+    if (lds > 0) {
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e != hipSuccess)
+            return e;
+    }
+    if (max_blocks) {
+        int per_cu = 0, dev = 0, cus = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds);
+        if (e != hipSuccess)
+            return e;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        *max_blocks = per_cu * cus;
+        return hipSuccess;
+    }
+    if (grid_mode) {
+        KParams q = p;
+        void *args[] = {&q};
+        return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args,
+                                          (unsigned)lds, st);
+    }
+    hipLaunchKernelGGL(((void (*)(KParams))fn), dim3(grid), dim3(block), lds, st, p);
+    return hipGetLastError();
+}
+
+/* Canonical S and G of the model as flat tables for the host's check. */
+template <class M>
+struct CanonTables {
+    int S[M::NT * M::NC];
+    int G[M::NT * M::NT];
+    constexpr CanonTables() : S(), G()
+    {
+        for (int k = 0; k < M::NT; k++) {
+            for (int c = 0; c < M::NC; c++)
+                S[k * M::NC + c] = M::S[k][c];
+            for (int j = 0; j < M::NT; j++)
+                G[k * M::NT + j] = M::G[k][j];
+        }
+    }
+};
+
+template <class M>
+inline constexpr CanonTables<M> canon_tables{};
+
+#define SIMINF_MODEL_ENTRY(NAME, M, NGDATA)                               \
+    {SIMINF_HIP_PLUGIN_ABI, NAME, M::NC, M::NT, M::ND, NGDATA,            \
+     siminf::canon_tables<M>.S, siminf::canon_tables<M>.G,               \
+     siminf::launch_ssm_nodes<M>, siminf::launch_aem_nodes<M>,            \
+     siminf::launch_coupled<M>}
+
+} /* namespace siminf */

@@ -37,8 +37,10 @@
 
 #if defined(__HIPCC__)
 #  define SIMINF_FN __host__ __device__ static inline
+#  define SIMINF_COLD_FN __host__ __device__ static __attribute__((noinline))
 #else
 #  define SIMINF_FN static inline
+#  define SIMINF_COLD_FN static __attribute__((noinline))
 #endif
 
 /* Stream domains. */
@@ -242,7 +244,7 @@ SIMINF_FN double siminf_powi(double x, int n)
 
 /* log(k!) - [(k + 1/2) log(k + 1) - (k + 1) + log(2 pi) / 2], the
  * Stirling-series remainder used by BTRD. */
-SIMINF_FN double siminf_stirling_fc(int k)
+SIMINF_COLD_FN double siminf_stirling_fc(int k)
 {
     if (k < 10) {
         const double HALF_LOG_2PI = 0.91893853320467274178;
@@ -281,8 +283,9 @@ SIMINF_FN int siminf_binomial_inv(siminf_stream *s, double p, int n)
     }
 }
 
-/* Binomial(n, p) with 0 < p <= 0.5 and n*p >= 30: BTRD. */
-SIMINF_FN int siminf_binomial_btrd(siminf_stream *s, double p, int n)
+/* Binomial(n, p) with 0 < p <= 0.5 and n*p >= 30: BTRD.  Rarely taken,
+ * so kept out of line to spare the callers' registers. */
+SIMINF_COLD_FN int siminf_binomial_btrd(siminf_stream *s, double p, int n)
 {
     const double q = 1.0 - p;
     const double dn = (double)n;

@@ -31,6 +31,29 @@ def _vector_after(src, start, name):
     return _numbers(src[m.end():i - 1]), i
 
 
+def _r_strings(src, start):
+    """The string literals of the c(...) that opens at src[start:]."""
+    i = src.index("c(", start) + 2
+    out, depth = [], 1
+    while depth:
+        ch = src[i]
+        if ch == '"':
+            j, buf = i + 1, []
+            while src[j] != '"':
+                if src[j] == "\\":
+                    buf.append({"n": "\n", "t": "\t"}.get(src[j + 1], src[j + 1]))
+                    j += 2
+                else:
+                    buf.append(src[j])
+                    j += 1
+            out.append("".join(buf))
+            i = j + 1
+            continue
+        depth += {"(": 1, ")": -1}.get(ch, 0)
+        i += 1
+    return out
+
+
 def main():
     seir = open(os.path.join(REF, "SEIR.R")).read()
     start = seir.index("U_expected <- data.frame(")
@@ -48,6 +71,13 @@ def main():
         vals, _ = _vector_after(seir, i, "structure")
         zero[key] = [int(v) for v in vals]
     json.dump(zero, open(os.path.join(HERE, "seir_zero_rates.json"), "w"))
+
+    mp = open(os.path.join(REF, "mparse.R")).read()
+    blocks = [m.start() for m in re.finditer(r"\nC_code <- c\(", mp)]
+    json.dump({"source": "tests/mparse.R C_code blocks at lines %s" % [
+                   mp.count("\n", 0, b) + 2 for b in blocks],
+               "blocks": [_r_strings(mp, b) for b in blocks]},
+              open(os.path.join(HERE, "mparse_c_code.json"), "w"), indent=0)
 
     phi = open(os.path.join(REF, "phi.R")).read()
     vals, _ = _vector_after(phi, 0, "phi_exp")

@@ -180,3 +180,28 @@ def test_run_api_raises_reference_message():
                beta=0.16, gamma=-0.077)
     with pytest.raises(si.SimInfError, match=r"Invalid rate detected \(non-finite or < 0\.0\)\."):
         si.run(m, solver="ssm", seed=1)
+
+
+def test_noncanonical_tables(oracle):
+    """S with an explicitly stored zero and G with extra (harmless)
+    dependencies select the kernels that read S and G from the arguments;
+    the trajectory must equal both the checker's and the canonical run's."""
+    m = W.sise3_herds(Nn=200, days=120, seed=9)
+    ref_canon, b_canon = _check(m, oracle, seed=3)
+    S = m.S.toarray()
+    G = m.G.toarray()
+    G[0, 2] = 1.0          # rate 1 recomputed after transition 3 fires
+    G[5, 0] = 1.0
+    S_sp = sp.csc_matrix(S)
+    # store an explicit zero in column 0, row 2
+    S_sp = sp.csc_matrix((np.append(S_sp.data, 0.0),
+                          (np.append(S_sp.indices, 2),
+                           np.append(np.repeat(np.arange(6), np.diff(S_sp.indptr)), 0))),
+                         shape=S.shape)
+    S_sp.sum_duplicates()
+    S_sp.sort_indices()
+    m.S = S_sp
+    m.G = sp.csc_matrix(G)
+    ref, b = _check(m, oracle, seed=3)
+    np.testing.assert_array_equal(b.U, b_canon.U)
+    np.testing.assert_array_equal(b.V, b_canon.V)
