@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libsiminf_oracle.so")
 
 _lib = None
+_host_plugins = {}
 
 
 def load():
@@ -68,8 +69,18 @@ def run(model, solver="ssm", seed=0, crn=False):
     lib = load()
     b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
                            crn=1 if crn else 0, seed=int(seed))
-    if lib.oracle_set_model(ctypes.byref(b.args), model.plugin.encode()):
-        raise ValueError("no host plugin for %s" % model.plugin)
+    if isinstance(model.plugin, str):
+        if lib.oracle_set_model(ctypes.byref(b.args), model.plugin.encode()):
+            raise ValueError("no host plugin for %s" % model.plugin)
+    else:
+        # A parsed model: the same functions compiled for the host.
+        from siminf_amd import plugins
+        so = plugins.compile_host(model.plugin.parsed)
+        if so not in _host_plugins:
+            _host_plugins[so] = ctypes.CDLL(so)
+        h = _host_plugins[so]
+        b.args.tr_fun = ctypes.cast(h.siminf_tr_fun, ctypes.c_void_p)
+        b.args.pts_fun = ctypes.c_void_p.in_dll(h, "siminf_pts_fun")
     err = lib.oracle_run(ctypes.byref(b.args))
     return OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
 

@@ -4,7 +4,8 @@ writeback as hand-written HIP for gfx950, behind the reference's
 SimInf_solver_args C boundary (include/siminf_hip.h)."""
 from .model import SimInfModel, SimInfError, make_events, run, trajectory
 from .models import SIR, SEIR, SIS, SISe, SISe3, SIR_mparse
+from .modelparser import mparse
 from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
-           "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse"]
+           "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse"]

@@ -150,9 +150,7 @@ def run(model, solver="ssm", seed=None, crn=False, lib=None):
     if isinstance(seed, (bool, np.bool_)) or not np.isfinite(float(seed)):
         _raise(lib, -22)
     seed = int(seed) & 0xFFFFFFFFFFFFFFFF
-    dev = lib.SimInf_hip_model_builtin(model.plugin.encode())
-    if not dev:
-        raise SimInfError(-9, "Unknown device plugin '%s'." % model.plugin)
+    dev = device_plugin(model, lib)
     b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
                            crn=1 if crn else 0, seed=seed)
     import ctypes
@@ -161,6 +159,18 @@ def run(model, solver="ssm", seed=None, crn=False, lib=None):
         _raise(lib, err)
     _attach(model, b)
     return model
+
+
+def device_plugin(model, lib=None):
+    """SimInf_hip_model* for the model: a built-in plugin by name, or the
+    compiled plugin of a parsed model."""
+    if isinstance(model.plugin, str):
+        lib = lib or _native.load()
+        dev = lib.SimInf_hip_model_builtin(model.plugin.encode())
+        if not dev:
+            raise SimInfError(-9, "Unknown device plugin '%s'." % model.plugin)
+        return dev
+    return model.plugin.device_model()
 
 
 def _attach(model, b):

@@ -73,7 +73,7 @@ def variable_names(x, is_vector_ok, what):
     if isinstance(x, dict):
         lbl = [str(k) for k in x.keys()]
     elif isinstance(x, (list, tuple)) and len(x) == 2 and isinstance(x[0], (list, tuple)):
-        lbl = [str(k) if k is not None else "" for k in x[0]]
+        lbl = ["" if k is None else str(k) for k in x[0]]
     else:
         raise MparseError("'%s' must either be a 'data.frame' or a 'matrix'." % what)
     value = list(range(len(lbl)))
@@ -541,4 +541,65 @@ class ParsedModel:
         self.pts_fun = pts_fun
         self.pre_code = pre_code
         self.use_enum = use_enum
+        self.compartments_names = comps
         self.ldata_names, self.gdata_names, self.v0_names = ldn, gdn, v0n
+        self.Nld = ldn.n_values if ldn is not None else 0
+        self.Ngdata = gdn.n_values if gdn is not None else 0
+        self.Nd = v0n.n_values if v0n is not None else 0
+
+
+def _named_rows(x, n_nodes):
+    """(names, Nrows x Nn matrix) from a dict of per-node columns (the
+    data.frame form), a dict of scalars (one node), or None."""
+    if x is None:
+        return None, np.zeros((0, n_nodes))
+    names = list(x.keys())
+    rows = [np.broadcast_to(np.atleast_1d(np.asarray(x[k], dtype=float)), (n_nodes,))
+            for k in names]
+    return names, np.vstack(rows) if rows else np.zeros((0, n_nodes))
+
+
+def mparse(transitions, compartments, ldata=None, gdata=None, u0=None, v0=None,
+           tspan=None, events=None, E=None, N=None, pts_fun=None,
+           use_enum=False, pre_code=None, replicates=1):
+    """Parse a model description into a SimInfModel whose plugin is
+    generated from the transitions (R/mparse.R:647-724).
+
+    u0, ldata and v0 map names to per-node values (data.frame columns);
+    gdata maps names to values (a named vector; None keys are unnamed).
+    """
+    from .model import SimInfModel
+    from .plugins import GeneratedPlugin
+    if u0 is None:
+        raise MparseError("Missing columns in initial state.")
+    missing = [c for c in compartments if c not in u0]
+    if missing:
+        raise MparseError("Missing columns in initial state.")
+    cols = [np.atleast_1d(np.asarray(u0[c])) for c in compartments]
+    n_nodes = max(len(c) for c in cols)
+    u = np.vstack([np.broadcast_to(c, (n_nodes,)) for c in cols]).astype(np.int64)
+    ld_names, ld = _named_rows(ldata, n_nodes)
+    v_names, v = _named_rows(v0, n_nodes)
+    if gdata is None:
+        g_pairs = None
+        gvec = []
+    else:
+        keys = list(gdata.keys())
+        g_pairs = ([k if k is not None else "" for k in keys], None)
+        gvec = [float(gdata[k]) for k in keys]
+    parsed = ParsedModel(transitions, compartments,
+                         ldata={k: None for k in ld_names} if ld_names else None,
+                         gdata=(g_pairs[0], gvec) if g_pairs else None,
+                         v0={k: None for k in v_names} if v_names else None,
+                         pts_fun=pts_fun, use_enum=use_enum, pre_code=pre_code,
+                         single_node=n_nodes == 1)
+    if replicates > 1:
+        u = np.tile(u, (1, replicates))
+        v = np.tile(v, (1, replicates))
+    model = SimInfModel(GeneratedPlugin(parsed), parsed.G, parsed.S, tspan, u,
+                        ldata=ld, gdata=gvec, v0=v if v.shape[0] else None,
+                        E=E if E is not None else np.zeros((len(compartments), 0)),
+                        N=N, events=events, replicates=replicates)
+    model.C_code = parsed.C_code
+    model.G_rownames = parsed.G_rownames
+    return model

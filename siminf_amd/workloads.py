@@ -1,7 +1,12 @@
 """Synthetic workloads with the shapes of the benchmark configurations.
 
   mparse_sir_readme   SIR via mparse, Nn x (99, 5, 0), beta 0.16,
-                      gamma 0.077, tspan 1:150 (README.md:60-76)
+                      gamma 0.077, tspan 1:150 (README.md:60-76), on the
+                      hand-written plugin of that model
+  mparse_sir_generated  the same model through the parser, on the plugin
+                      generated from its transitions
+  mparse_sis_env      a parsed SIS model with a continuous state updated by
+                      a post-time-step function, ldata and enum constants
   sise3_herds         SISe3 on herds of NegBin(mean 130, size 1.5) animals
                       split 10/15/75 % by age, 1 % of herds seeded, with
                       scheduled enter / exit / internal / external events
@@ -28,6 +33,43 @@ def mparse_sir_readme(Nn=1000, days=150, replicates=1):
                                  "R": np.zeros(Nn, int)},
                              tspan=np.arange(1, days + 1, dtype=float),
                              beta=0.16, gamma=0.077, replicates=replicates)
+
+
+SIR_TRANSITIONS = ["S -> beta*S*I/N -> I", "I -> gamma*I -> R", "N <- S+I+R"]
+
+
+def mparse_sir_generated(Nn=1000, days=150, replicates=1):
+    """The README model built by the parser: its plugin is generated from
+    the transitions and compiled for the device."""
+    from .modelparser import mparse
+    return mparse(SIR_TRANSITIONS, ["S", "I", "R"], gdata={"beta": 0.16, "gamma": 0.077},
+                  u0={"S": np.full(Nn, 99), "I": np.full(Nn, 5), "R": np.zeros(Nn, int)},
+                  tspan=np.arange(1, days + 1, dtype=float), replicates=replicates)
+
+
+PTS_DECAY = ["v_new[PHI] = v[PHI] * (1.0 - gdata[DECAY]) +",
+             "    gdata[SHED] * u[I] / (u[S] + u[I] + 1.0);",
+             "if (!R_FINITE(v_new[PHI]))",
+             "    return SIMINF_ERR_V_IS_NOT_FINITE;",
+             "if (v_new[PHI] < 0.0)",
+             "    return SIMINF_ERR_V_IS_NEGATIVE;",
+             "return 1;"]
+
+
+def mparse_sis_env(Nn=200, days=120, seed=3, replicates=1):
+    """A parsed SIS model with an environmental infectious pressure phi
+    (a continuous state v0) that the post time step updates, per-node
+    recovery rates (ldata) and named constants (use_enum)."""
+    from .modelparser import mparse
+    rng = np.random.default_rng(seed)
+    S = rng.integers(20, 400, Nn)
+    I = (rng.random(Nn) < 0.1) * rng.integers(1, 10, Nn)
+    return mparse(["S -> upsilon*phi*S -> I", "I -> gamma*I -> S"], ["S", "I"],
+                  ldata={"gamma": rng.uniform(0.05, 0.2, Nn)},
+                  gdata={"upsilon": 0.05, "DECAY": 0.1, "SHED": 0.3},
+                  v0={"phi": np.zeros(Nn)}, u0={"S": S, "I": I},
+                  tspan=np.arange(1, days + 1, dtype=float), pts_fun=PTS_DECAY,
+                  use_enum=True, replicates=replicates)
 
 
 def seir_million(Nn=1_000_000, days=365, seed=7, seeded_frac=0.005):

@@ -11,13 +11,14 @@ import scipy.sparse as sp
 
 import siminf_amd as si
 from siminf_amd import _native, workloads as W
+from siminf_amd.model import device_plugin
 
 pytestmark = pytest.mark.gpu
 
 
 def _gpu(model, solver="ssm", seed=0, crn=False):
     lib = _native.load()
-    dev = lib.SimInf_hip_model_builtin(model.plugin.encode())
+    dev = device_plugin(model, lib)
     assert dev
     b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
                            crn=int(crn), seed=seed)
@@ -44,7 +45,7 @@ def _gpu(model, solver="ssm", seed=0, crn=False):
 
 def _check(model, oracle, solver="ssm", seed=1, crn=False):
     print("case: %s %s Nn=%d Nrep=%d tlen=%d events=%d seed=%d" % (
-        model.plugin, solver, model.Nn, model.replicates, len(model.tspan),
+        getattr(model.plugin, "name", model.plugin), solver, model.Nn, model.replicates, len(model.tspan),
         len(model.events["event"]), seed), flush=True)
     ref = oracle.run(model, solver=solver, seed=seed, crn=crn)
     err, b, transitions = _gpu(model, solver=solver, seed=seed, crn=crn)
@@ -205,3 +206,23 @@ def test_noncanonical_tables(oracle):
     ref, b = _check(m, oracle, seed=3)
     np.testing.assert_array_equal(b.U, b_canon.U)
     np.testing.assert_array_equal(b.V, b_canon.V)
+
+
+def test_generated_sir_plugin(oracle):
+    """The README model through the parser: the device plugin compiled
+    from its generated C gives the checker's trajectories (which run the
+    same C on the host) and those of the hand-written plugin."""
+    gen = W.mparse_sir_generated(Nn=300, days=80, replicates=2)
+    _, b_gen = _check(gen, oracle, seed=21)
+    hand = W.mparse_sir_readme(Nn=300, days=80, replicates=2)
+    _, b_hand = _check(hand, oracle, seed=21)
+    np.testing.assert_array_equal(b_gen.U, b_hand.U)
+
+
+@pytest.mark.parametrize("solver", ["ssm", "aem"])
+def test_generated_pts_plugin(oracle, solver):
+    """Generated model with v0, ldata, enum constants and a post time step
+    that writes v_new: U and V bit-exact against the checker."""
+    m = W.mparse_sis_env(Nn=400, days=150)
+    ref, _ = _check(m, oracle, solver=solver, seed=8)
+    assert ref.transitions > 0 and ref.V.max() > 0
