@@ -89,24 +89,40 @@ struct KParams {
     const double *Ew;
     const int *Nmat;
 
-    /* Events (zero-based node/dest/select/shift), sorted by time. */
-    const int *ev_type, *ev_time, *ev_node, *ev_dest, *ev_n, *ev_select,
-        *ev_shift, *ev_rank, *ev_day;
-    const double *ev_prop;
-
-    /* E1 events grouped by node in list order: e1_idx[e1_ptr[i] ..
-     * e1_ptr[i+1]) are node i's events. */
-    const int *e1_ptr, *e1_idx;
-
-    /* E2 schedule: day d's conflict-free levels are e2_day[d] ..
-     * e2_day[d+1]; level l holds e2_idx[e2_lvl[l] .. e2_lvl[l+1]).  The
-     * herds a day's E2 events touch are e2_tn[e2_tn_ptr[d] ..
-     * e2_tn_ptr[d+1]); e2_lsrc[k] / e2_ldst[k] are event k's source and
-     * destination slots in that list (-1 for an invalid destination). */
-    const int *e2_day, *e2_lvl, *e2_idx;
-    const int *e2_tn_ptr, *e2_tn, *e2_lsrc, *e2_ldst;
-    int e2_max_touched;
+    /* Events.  ev_rank[k] is event k's position in the caller's list
+     * (error reports).  The kernels read packed records, 32 bytes each
+     * plus the proportion:
+     *   E1 (exit, enter, internal transfer), grouped by node in list
+     *   order: node i's are e1_rec[2j], e1_rec[2j+1], e1_prop[j] for j in
+     *   e1_ptr[i] .. e1_ptr[i+1], with
+     *     e1_rec[2j]   = (time, type, n, select)
+     *     e1_rec[2j+1] = (shift, k, 0, 0);
+     *   E2 (external transfer), in level order: day d's conflict-free
+     *   levels are e2_day[d] .. e2_day[d+1], level l holds positions
+     *   e2_lvl[l] .. e2_lvl[l+1], and position i is
+     *     e2_rec[2i]   = (k, n, select, shift)
+     *     e2_rec[2i+1] = (src slot, dst slot, src node, dest node)
+     *   where the slots index the herds of the day's touched list
+     *   e2_tn[e2_tn_ptr[d] .. e2_tn_ptr[d+1]) (dst slot -1 for an invalid
+     *   destination).  select and shift are zero-based, shift -1 for
+     *   none. */
+    const int *ev_rank;
+    const int *e1_ptr;
+    const int4 *e1_rec;
+    const double *e1_prop;
+    const int *e2_day, *e2_lvl;
+    const int4 *e2_rec;
+    const double *e2_prop;
+    const int *e2_tn_ptr, *e2_tn;
     int ndays;
+    int Nselect;
+
+    /* LDS layout of the coupled kernel (byte offsets into the dynamic
+     * LDS block) for one day's E2 work, used when the day has at most
+     * e2_cap_t touched herds, e2_cap_ev events and e2_cap_lv levels
+     * (other days run from global memory). */
+    int e2_cap_t, e2_cap_ev, e2_cap_lv;
+    int lds_u, lds_upd, lds_rec, lds_prop, lds_lvl, lds_pre;
 
     /* Output: dense U/V or sparse (irU, jcU, prU) / (irV, jcV, prV). */
     int *U;
@@ -519,16 +535,18 @@ __device__ __forceinline__ int last_nonzero_at_or_below(
     return j;
 }
 
-/* Is the run-time-indexed element a[i] negative? */
+/* Is the run-time-indexed element a[i] negative?  Written as a masked
+ * sum, like add_at: a chain of compares would be folded back into a
+ * run-time index, which moves the array to scratch memory. */
 template <int NC>
 __device__ __forceinline__ bool negative_at(const int (&a)[NC], int i)
 {
-    bool neg = false;
+    int x = 0;
     static_for<NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
-        neg = neg || (i == c && a[c] < 0);
+        x |= a[c] & -(int)(i == c);
     });
-    return neg;
+    return x < 0;
 }
 
 /* sample_select for a node whose state is the register array u.  The E
@@ -698,32 +716,43 @@ __device__ int sample_select_enter(const int (&u)[NC], uint32_t mask,
     return 0;
 }
 
+/* E and N as the event code reads them: Emask[s] has bit c set when
+ * E[c,s] is stored and Ew[s*NC+c] holds its weight; N[shift*NC+c] is
+ * N(c, shift), or N is null when the model has no shift matrix.  Small
+ * and read by every event, they stay in the caches. */
+struct EvTab {
+    const uint32_t *Emask;
+    const double *Ew;
+    const int *N;
+};
+
+__device__ __forceinline__ EvTab global_tables(const KParams &p)
+{
+    return EvTab{p.Emask, p.Ew, p.Nmat};
+}
+
 /* Load E column s as (mask, dense weights). */
 template <int NC>
-__device__ __forceinline__ uint32_t load_E(const KParams &p, int s, double (&w)[NC])
+__device__ __forceinline__ uint32_t load_E(const EvTab &tab, int s, double (&w)[NC])
 {
     static_for<NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
-        w[c] = p.Ew[s * NC + c];
+        w[c] = tab.Ew[s * NC + c];
     });
-    return p.Emask[s];
+    return tab.Emask[s];
 }
 
 /* Apply an E1 event (exit, enter, internal transfer) to the register
  * state u of its node (SimInf_solver.c:680-822).  External transfers are
  * handled by apply_e2. */
 template <int NC>
-__device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rng)
+__device__ int apply_e1(const EvTab &tab, int type, int n, int select, int shift,
+                        double prop, int (&u)[NC], siminf_stream *rng)
 {
-    const int type = p.ev_type[k];
-    const int select = p.ev_select[k];
-    const int shift = p.ev_shift[k];
-    const int n = p.ev_n[k];
-    const double prop = p.ev_prop[k];
     double w[NC];
     int ind[NC];
     int err;
-    const uint32_t mask = load_E<NC>(p, select, w);
+    const uint32_t mask = load_E<NC>(tab, select, w);
 
     if (type == SIMINF_EXIT_EVENT) {
         err = sample_select<NC>(u, mask, w, n, prop, ind, rng);
@@ -751,10 +780,10 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
                     u[c] += ind[c];
                     if (u[c] < 0)
                         err = SIMINF_ERR_NEGATIVE_STATE;
-                } else if (!p.Nmat) {
+                } else if (!tab.N) {
                     err = SIMINF_ERR_EVENTS_N;
                 } else {
-                    const int ll = p.Nmat[shift * NC + c];
+                    const int ll = tab.N[shift * NC + c];
                     if (c + ll < 0 || c + ll >= NC) {
                         err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
                     } else {
@@ -769,7 +798,7 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
     }
 
     if (type == SIMINF_INTERNAL_TRANSFER_EVENT) {
-        if (!p.Nmat)
+        if (!tab.N)
             return SIMINF_ERR_EVENTS_N;
         if (shift < 0)
             return SIMINF_ERR_EVENT_SHIFT;
@@ -779,7 +808,7 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
             if (!err && (mask & (1u << c))) {
-                const int ll = p.Nmat[shift * NC + c];
+                const int ll = tab.N[shift * NC + c];
                 if (c + ll < 0 || c + ll >= NC) {
                     err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
                 } else {
@@ -800,31 +829,29 @@ __device__ int apply_e1(const KParams &p, int k, int (&u)[NC], siminf_stream *rn
     return SIMINF_UNDEFINED_EVENT;
 }
 
-/* Apply an external transfer event k (SimInf_solver.c:824-902) to a
- * state array uw (NC ints per slot, in LDS or global memory): the source
- * herd sits in slot src, the destination in slot dst; upd receives the
- * update flags of both slots.  The destination is checked against the
- * node count before any slot is touched. */
+/* Apply an external transfer (SimInf_solver.c:824-902) to a state array
+ * uw (NC ints per slot, in LDS or global memory): the source herd sits in
+ * slot src, the destination in slot dst; upd receives the update flags of
+ * both slots.  dest is the destination node, checked against the node
+ * count before any slot is touched. */
 template <int NC>
-__device__ int apply_e2(const KParams &p, int k, int *uw, int *upd, int src,
-                        int dst, siminf_stream *rng)
+__device__ int apply_e2(const EvTab &tab, int n, int select, int shift,
+                        double prop, int dest, int Nn, int *uw, int *upd,
+                        int src, int dst, siminf_stream *rng)
 {
-    const int dest = p.ev_dest[k];
-    const int select = p.ev_select[k];
-    const int shift = p.ev_shift[k];
     double w[NC];
     int us[NC], ind[NC];
     int err;
 
-    if (dest < 0 || dest >= p.Nn)
+    if (dest < 0 || dest >= Nn)
         return SIMINF_ERR_DEST_OUT_OF_BOUNDS;
 
-    const uint32_t mask = load_E<NC>(p, select, w);
+    const uint32_t mask = load_E<NC>(tab, select, w);
     static_for<NC>([&](auto C) {
         constexpr int c = decltype(C)::value;
         us[c] = uw[src * NC + c];
     });
-    err = sample_select<NC>(us, mask, w, p.ev_n[k], p.ev_prop[k], ind, rng);
+    err = sample_select<NC>(us, mask, w, n, prop, ind, rng);
     if (err)
         return err;
 
@@ -833,11 +860,11 @@ __device__ int apply_e2(const KParams &p, int k, int *uw, int *upd, int src,
         if (!err && (mask & (1u << c))) {
             int kd = dst * NC + c;
             if (shift >= 0) {
-                if (!p.Nmat) {
+                if (!tab.N) {
                     err = SIMINF_ERR_EVENTS_N;
                     return;
                 }
-                const int ll = p.Nmat[shift * NC + c];
+                const int ll = tab.N[shift * NC + c];
                 if (c + ll < 0 || c + ll >= NC) {
                     err = SIMINF_ERR_SHIFT_OUT_OF_BOUNDS;
                     return;

@@ -42,12 +42,12 @@ struct ModelSIR {
         {{-1, 1, 0}, {0, -1, 1}};
     static constexpr int G[NT][NT] =
         {{1, 1}, {1, 1}};
-    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SIR_S_to_I(u, v, ld, gd, t);
         else return SIR_I_to_R(u, v, ld, gd, t);
     }
-    __device__ static int pts(SIMINF_PTS_ARGS)
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return siminf_pts_noop(vn, u, v, ld, gd, node, t);
     }
@@ -59,13 +59,13 @@ struct ModelSEIR {
         {{-1, 1, 0, 0}, {0, -1, 1, 0}, {0, 0, -1, 1}};
     static constexpr int G[NT][NT] =
         {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};
-    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SEIR_S_to_E(u, v, ld, gd, t);
         else if constexpr (J == 1) return SEIR_E_to_I(u, v, ld, gd, t);
         else return SEIR_I_to_R(u, v, ld, gd, t);
     }
-    __device__ static int pts(SIMINF_PTS_ARGS)
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return siminf_pts_noop(vn, u, v, ld, gd, node, t);
     }
@@ -77,12 +77,12 @@ struct ModelSIS {
         {{-1, 1}, {1, -1}};
     static constexpr int G[NT][NT] =
         {{1, 1}, {1, 1}};
-    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SIS_S_to_I(u, v, ld, gd, t);
         else return SIS_I_to_S(u, v, ld, gd, t);
     }
-    __device__ static int pts(SIMINF_PTS_ARGS)
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return siminf_pts_noop(vn, u, v, ld, gd, node, t);
     }
@@ -94,12 +94,12 @@ struct ModelSISe {
         {{-1, 1}, {1, -1}};
     static constexpr int G[NT][NT] =
         {{1, 1}, {1, 1}};
-    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SISe_S_to_I(u, v, ld, gd, t);
         else return SISe_I_to_S(u, v, ld, gd, t);
     }
-    __device__ static int pts(SIMINF_PTS_ARGS)
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return SISe_pts(vn, u, v, ld, gd, node, t);
     }
@@ -113,7 +113,7 @@ struct ModelSISe3 {
     static constexpr int G[NT][NT] =
         {{1, 1, 0, 0, 0, 0}, {1, 1, 0, 0, 0, 0}, {0, 0, 1, 1, 0, 0},
          {0, 0, 1, 1, 0, 0}, {0, 0, 0, 0, 1, 1}, {0, 0, 0, 0, 1, 1}};
-    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SISe3_S_1_to_I_1(u, v, ld, gd, t);
         else if constexpr (J == 1) return SISe3_I_1_to_S_1(u, v, ld, gd, t);
@@ -122,7 +122,7 @@ struct ModelSISe3 {
         else if constexpr (J == 4) return SISe3_S_3_to_I_3(u, v, ld, gd, t);
         else return SISe3_I_3_to_S_3(u, v, ld, gd, t);
     }
-    __device__ static int pts(SIMINF_PTS_ARGS)
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return SISe3_pts(vn, u, v, ld, gd, node, t);
     }
@@ -134,12 +134,12 @@ struct ModelSIRmparse {
         {{-1, 1, 0}, {0, -1, 1}};
     static constexpr int G[NT][NT] =
         {{1, 1}, {1, 1}};
-    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
     {
         if constexpr (J == 0) return SIR_mparse_trFun1(u, v, ld, gd, t);
         else return SIR_mparse_trFun2(u, v, ld, gd, t);
     }
-    __device__ static int pts(SIMINF_PTS_ARGS)
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return siminf_pts_noop(vn, u, v, ld, gd, node, t);
     }
@@ -275,6 +275,50 @@ static int event_day(int tm, double tt0)
     return (int)((double)tm - std::floor(tt0));
 }
 
+/* LDS of the coupled kernel, 64 KiB at most: buffers for one day's E2
+ * work sized for the largest day or, when that does not fit, the budget
+ * split 45 / 50 / 5 % over herds, event records with their precomputed
+ * uniforms, and level bounds (larger days run from global memory).  Sets the offsets in p and returns the
+ * byte size. */
+static size_t coupled_lds_layout(KParams &p, int Nc, int max_t, int max_ev,
+                                 int max_lv)
+{
+    const size_t budget = 64 * 1024;
+    auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    const size_t per_t = (size_t)(Nc + 1) * sizeof(int);
+    const size_t per_ev = 2 * sizeof(int4) + (1 + SIMINF_E2_PRE) * sizeof(double);
+    auto need = [&](size_t t, size_t ev, size_t lv) {
+        return align16(t * Nc * sizeof(int)) + align16(t * sizeof(int)) +
+               ev * 2 * sizeof(int4) + align16(ev * sizeof(double)) +
+               align16((lv + 1) * sizeof(int)) +
+               ev * SIMINF_E2_PRE * sizeof(double);
+    };
+    size_t cap_t = (size_t)max_t, cap_ev = (size_t)max_ev, cap_lv = (size_t)max_lv;
+    if (need(cap_t, cap_ev, cap_lv) > budget) {
+        const size_t rest = budget - 128;
+        cap_t = std::min(cap_t, rest * 45 / 100 / per_t);
+        cap_ev = std::min(cap_ev, rest * 50 / 100 / per_ev);
+        cap_lv = std::min(cap_lv, rest * 5 / 100 / sizeof(int) - 1);
+    }
+    p.e2_cap_t = (int)cap_t;
+    p.e2_cap_ev = (int)cap_ev;
+    p.e2_cap_lv = (int)cap_lv;
+    size_t off = 0;
+    p.lds_u = (int)off;
+    off = align16(off + cap_t * Nc * sizeof(int));
+    p.lds_upd = (int)off;
+    off = align16(off + cap_t * sizeof(int));
+    p.lds_rec = (int)off;
+    off = align16(off + cap_ev * 2 * sizeof(int4));
+    p.lds_prop = (int)off;
+    off = align16(off + cap_ev * sizeof(double));
+    p.lds_lvl = (int)off;
+    off = align16(off + (cap_lv + 1) * sizeof(int));
+    p.lds_pre = (int)off;
+    off = align16(off + cap_ev * SIMINF_E2_PRE * sizeof(double));
+    return std::max<size_t>(off, 16);
+}
+
 extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
                                       const SimInf_solver_args *a,
                                       const SimInf_hip_model *model)
@@ -398,13 +442,22 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
             e1_idx[fill[ev_node[k]]++] = k;
         }
     }
+    const int n_e1 = e1_ptr[Nn];
+    std::vector<int4> e1_rec((size_t)std::max(n_e1, 1) * 2, int4{0, 0, 0, 0});
+    std::vector<double> e1_prop(std::max(n_e1, 1), 0.0);
+    for (int j = 0; j < n_e1; j++) {
+        const int k = e1_idx[j];
+        e1_rec[2 * (size_t)j] = int4{a->time[k], a->event[k], a->n[k], ev_select[k]};
+        e1_rec[2 * (size_t)j + 1] = int4{ev_shift[k], k, 0, 0};
+        e1_prop[j] = a->proportion[k];
+    }
 
     /* E2 levels: an event's level is one more than the last level that
      * touched its source or destination on the same day. */
     std::vector<int> e2_day(ndays + 1, 0), e2_lvl(1, 0), e2_idx;
     std::vector<int> e2_tn_ptr(ndays + 1, 0), e2_tn, e2_lsrc(std::max(len, 1), -1),
         e2_ldst(std::max(len, 1), -1);
-    int n_e2 = 0, max_touched = 0;
+    int n_e2 = 0, max_touched = 0, max_day_events = 0, max_day_levels = 0;
     {
         std::vector<int> last_level(Nn, -1), last_day(Nn, -1);
         std::vector<std::vector<int>> levels;
@@ -456,16 +509,25 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
             }
             e2_tn_ptr[d + 1] = (int)e2_tn.size();
             max_touched = std::max(max_touched, (int)e2_tn.size() - t0);
+            const int i_start = (int)e2_idx.size();
             for (auto &lv : levels) {
                 e2_idx.insert(e2_idx.end(), lv.begin(), lv.end());
                 e2_lvl.push_back((int)e2_idx.size());
             }
             e2_day[d + 1] = (int)e2_lvl.size() - 1;
+            max_day_events = std::max(max_day_events, (int)e2_idx.size() - i_start);
+            max_day_levels = std::max(max_day_levels, (int)levels.size());
         }
-        if (e2_idx.empty())
-            e2_idx.push_back(0);
         if (e2_tn.empty())
             e2_tn.push_back(0);
+    }
+    std::vector<int4> e2_rec(std::max<size_t>(e2_idx.size(), 1) * 2, int4{0, 0, 0, 0});
+    std::vector<double> e2_prop(std::max<size_t>(e2_idx.size(), 1), 0.0);
+    for (size_t i = 0; i < e2_idx.size(); i++) {
+        const int k = e2_idx[i];
+        e2_rec[2 * i] = int4{k, a->n[k], ev_select[k], ev_shift[k]};
+        e2_rec[2 * i + 1] = int4{e2_lsrc[k], e2_ldst[k], ev_node[k], ev_dest[k]};
+        e2_prop[i] = a->proportion[k];
     }
 
     /* The fast kernels fold the model's own S and G into the code; any
@@ -542,25 +604,17 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     p.Ew = B.upload(Ew.data(), Ew.size());
     p.Nmat = a->N ? B.upload(a->N, (size_t)Nc * std::max(Nshift, 1)) : nullptr;
 
-    p.ev_type = B.upload(a->event, (size_t)len);
-    p.ev_time = B.upload(a->time, (size_t)len);
-    p.ev_node = B.upload(ev_node.data(), (size_t)len);
-    p.ev_dest = B.upload(ev_dest.data(), (size_t)len);
-    p.ev_n = B.upload(a->n, (size_t)len);
-    p.ev_select = B.upload(ev_select.data(), (size_t)len);
-    p.ev_shift = B.upload(ev_shift.data(), (size_t)len);
     p.ev_rank = B.upload(ev_rank.data(), (size_t)len);
-    p.ev_day = B.upload(ev_day.data(), (size_t)len);
-    p.ev_prop = B.upload(a->proportion, (size_t)len);
     p.e1_ptr = B.upload(e1_ptr.data(), e1_ptr.size());
-    p.e1_idx = B.upload(e1_idx.data(), e1_idx.size());
+    p.e1_rec = B.upload(e1_rec.data(), e1_rec.size());
+    p.e1_prop = B.upload(e1_prop.data(), e1_prop.size());
     p.e2_day = B.upload(e2_day.data(), e2_day.size());
     p.e2_lvl = B.upload(e2_lvl.data(), e2_lvl.size());
-    p.e2_idx = B.upload(e2_idx.data(), e2_idx.size());
+    p.e2_rec = B.upload(e2_rec.data(), e2_rec.size());
+    p.e2_prop = B.upload(e2_prop.data(), e2_prop.size());
     p.e2_tn_ptr = B.upload(e2_tn_ptr.data(), e2_tn_ptr.size());
     p.e2_tn = B.upload(e2_tn.data(), e2_tn.size());
-    p.e2_lsrc = B.upload(e2_lsrc.data(), e2_lsrc.size());
-    p.e2_ldst = B.upload(e2_ldst.data(), e2_ldst.size());
+    p.Nselect = Nselect;
 
     const size_t ncols = (size_t)Nrep * a->tlen;
     if (a->U) {
@@ -585,11 +639,8 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     if (P->kernel == K_COUPLED_GRID || P->kernel == K_COUPLED_REPS) {
         p.u_work = B.alloc<int>((size_t)lanes * Nc);
         p.upd_work = B.alloc<int>((size_t)lanes);
-        /* LDS holds the herds one day's E2 events touch (u and a flag);
-         * days that touch more run on the global state. */
-        const int cap = std::min(max_touched, 48 * 1024 / (int)((Nc + 1) * sizeof(int)));
-        p.e2_max_touched = cap;
-        P->lds = (size_t)std::max(cap, 1) * (Nc + 1) * sizeof(int);
+        P->lds = coupled_lds_layout(p, Nc, max_touched, max_day_events,
+                                    max_day_levels);
         if (P->kernel == K_COUPLED_GRID) {
             P->block = SIMINF_GRID_BLOCK;
             P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK);

@@ -19,6 +19,11 @@
 
 namespace siminf {
 
+/* Model functions are always inlined: an out-of-line call would take
+ * the addresses of the node's state arrays and move them from registers
+ * to scratch memory. */
+#define SIMINF_MODEL_FN __device__ __forceinline__ static
+
 #define SIMINF_RATE_ARGS const int *u, const double *v, const double *ld, \
     const double *gd, double t
 #define SIMINF_PTS_ARGS double *vn, const int *u, const double *v, \
@@ -28,25 +33,50 @@ namespace siminf {
 /* Kernels.                                                            */
 /* ------------------------------------------------------------------ */
 
+/* A node's next E1 event, held in registers.  The record is loaded when
+ * the previous one has been applied, so the load overlaps the following
+ * SSA work instead of sitting on the day's critical path. */
+struct E1Cursor {
+    int cur, end;
+    int4 a, b;         /* (time, type, n, select), (shift, k, 0, 0) */
+    double prop;
+
+    __device__ __forceinline__ void start(const KParams &p, int node)
+    {
+        cur = p.e1_ptr[node];
+        end = p.e1_ptr[node + 1];
+        fetch(p);
+    }
+    __device__ __forceinline__ void fetch(const KParams &p)
+    {
+        if (cur < end) {
+            a = p.e1_rec[2 * cur];
+            b = p.e1_rec[2 * cur + 1];
+            prop = p.e1_prop[cur];
+        }
+    }
+};
+
 /* E1 events of one node due at tt, in list order, drawing from the
  * node's stream.  Returns 0 or an error code; *bad gets the failing
  * event. */
 template <int NC>
-__device__ __forceinline__ int e1_day(const KParams &p, int &cur, int end,
-                                      double tt, int (&u)[NC],
+__device__ __forceinline__ int e1_day(const KParams &p, const EvTab &tab,
+                                      E1Cursor &ec, double tt, int (&u)[NC],
                                       siminf_stream *rng, bool &upd, int &bad)
 {
-    while (cur < end) {
-        const int k = p.e1_idx[cur];
-        if (p.ev_time[k] > tt)
+    while (ec.cur < ec.end) {
+        if ((double)ec.a.x > tt)
             break;
-        const int err = apply_e1<NC>(p, k, u, rng);
+        const int err = apply_e1<NC>(tab, ec.a.y, ec.a.z, ec.a.w, ec.b.x,
+                                     ec.prop, u, rng);
         if (err) {
-            bad = k;
+            bad = ec.b.y;
             return err;
         }
         upd = true;
-        cur++;
+        ec.cur++;
+        ec.fetch(p);
     }
     return 0;
 }
@@ -94,8 +124,10 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
         if (err) {
             record_error(p, rep, 0, PH_INIT, node, err);
         } else {
-            int cur = p.e1_ptr[node];
-            const int end = p.e1_ptr[node + 1];
+            const EvTab tab = global_tables(p);
+            E1Cursor ec;
+            if constexpr (EVENTS)
+                ec.start(p, node);
             int U_it = 0, V_it = 0, day = 0;
             for (;;) {
                 bool upd = false;
@@ -107,7 +139,7 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                     break;
                 }
                 if constexpr (EVENTS) {
-                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                    err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
                     if (err) {
                         record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
                         break;
@@ -197,8 +229,10 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
         if (err) {
             record_error(p, 0, 0, PH_INIT, node, err);
         } else {
-            int cur = p.e1_ptr[node];
-            const int end = p.e1_ptr[node + 1];
+            const EvTab tab = global_tables(p);
+            E1Cursor ec;
+            if constexpr (EVENTS)
+                ec.start(p, node);
             int U_it = 0, V_it = 0, day = 0;
             for (;;) {
                 bool upd = false;
@@ -210,7 +244,7 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
                     break;
                 }
                 if constexpr (EVENTS) {
-                    err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                    err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
                     if (err) {
                         record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
                         break;
@@ -276,65 +310,119 @@ __device__ __forceinline__ bool error_by(const KParams &p, int rep, int day,
     return (k >> 27) <= (error_key(rep, day, phase, 0, -1) >> 27);
 }
 
-/* One day's external transfers of replicate rep, run by one workgroup.
- * The herds they touch are gathered into LDS, the conflict-free levels
- * run there with a workgroup barrier between levels, and the herds are
- * scattered back with their update flags.  Events in one level touch
- * disjoint herds, so no atomics are needed; within a herd the events
- * keep list order.  When the touched set does not fit the LDS buffer the
- * levels run on the global state directly.  Returns 1 if this thread
- * recorded an error. */
-template <int NC>
-__device__ int e2_day(const KParams &p, int rep, int day, int *uw, int *updw,
-                      int *lu, int *lupd, int lds_cap)
+/* Uniforms computed ahead per external transfer event (even). */
+#define SIMINF_E2_PRE 8
+
+/* The levels of one day's external transfers over a state array su (NC
+ * ints per slot) with update flags sf, event records rec/prop and level
+ * bounds lvl (absolute positions; the day starts at i0).  Instantiated
+ * once for LDS operands and once for global ones, so that each gets its
+ * own memory instructions instead of generic ones. */
+template <int NC, bool LDS>
+__device__ __forceinline__ int e2_levels(const KParams &p, const EvTab &tab,
+                                         int rep, int day, int *su, int *sf,
+                                         const int4 *rec, const double *prop,
+                                         const int *lvl, int nlv, int i0,
+                                         const double *pre)
 {
     int myerr = 0;
-    if (day >= p.ndays || p.e2_day[day] == p.e2_day[day + 1])
-        return 0;
-    const int t0 = p.e2_tn_ptr[day], nt = p.e2_tn_ptr[day + 1] - t0;
-    const bool in_lds = nt <= lds_cap;
-    if (in_lds) {
-        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-            const int node = p.e2_tn[t0 + i];
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                lu[i * NC + c] = uw[node * NC + c];
-            });
-            lupd[i] = 0;
-        }
-        __syncthreads();
-    }
-    for (int l = p.e2_day[day]; l < p.e2_day[day + 1]; l++) {
-        for (int i = p.e2_lvl[l] + threadIdx.x; i < p.e2_lvl[l + 1];
-             i += blockDim.x) {
-            const int k = p.e2_idx[i];
+    const int tid = (int)threadIdx.x, nth = (int)blockDim.x;
+    for (int l = 0; l < nlv; l++) {
+        const int a = lvl[l] - i0, b = lvl[l + 1] - i0;
+        for (int i = a + tid; i < b; i += nth) {
+            const int4 r0 = rec[2 * i], r1 = rec[2 * i + 1];
             siminf_stream rng = siminf_stream_make(
                 p.seed, SIMINF_DOMAIN_E2,
-                (uint64_t)rep * (uint64_t)p.len + (uint64_t)k);
-            /* One code path for both places: generic pointers address
-             * LDS and global memory alike. */
-            int *su = in_lds ? lu : uw;
-            int *sf = in_lds ? lupd : updw;
-            const int src = in_lds ? p.e2_lsrc[k] : p.ev_node[k];
-            const int dst = in_lds ? p.e2_ldst[k] : p.ev_dest[k];
-            const int err = apply_e2<NC>(p, k, su, sf, src, dst, &rng);
+                (uint64_t)rep * (uint64_t)p.len + (uint64_t)r0.x);
+            if constexpr (LDS) {
+                rng.pre = pre + (long long)i * SIMINF_E2_PRE;
+                rng.npre = SIMINF_E2_PRE;
+            }
+            const int err = apply_e2<NC>(tab, r0.y, r0.z, r0.w, prop[i], r1.w,
+                                         p.Nn, su, sf, LDS ? r1.x : r1.z,
+                                         LDS ? r1.y : r1.w, &rng);
             if (err) {
-                record_error(p, rep, day, PH_E2, p.ev_rank[k], err);
+                record_error(p, rep, day, PH_E2, p.ev_rank[r0.x], err);
                 myerr = 1;
             }
         }
         __syncthreads();
     }
-    if (in_lds) {
-        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-            const int node = p.e2_tn[t0 + i];
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                uw[node * NC + c] = lu[i * NC + c];
-            });
-            if (lupd[i])
-                updw[node] = 1;
-        }
+    return myerr;
+}
+
+/* One day's external transfers of replicate rep, run by one workgroup.
+ * The day's event records, level bounds and the herds they touch are
+ * staged in LDS with coalesced loads, the conflict-free levels run there
+ * with a workgroup barrier between levels, and the herds are scattered
+ * back with their update flags.  Events in one level touch disjoint
+ * herds, so no atomics are needed; within a herd the events keep list
+ * order.  A day larger than the LDS buffers runs the same levels on the
+ * global state and records.  Returns 1 if this thread recorded an
+ * error. */
+template <int NC>
+__device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int rep, int day,
+                      int *uw, int *updw, unsigned char *lds)
+{
+    if (day >= p.ndays)
+        return 0;
+    const int l0 = p.e2_day[day], nlv = p.e2_day[day + 1] - l0;
+    if (nlv == 0)
+        return 0;
+    const int i0 = p.e2_lvl[l0], nev = p.e2_lvl[l0 + nlv] - i0;
+    const int t0 = p.e2_tn_ptr[day], nt = p.e2_tn_ptr[day + 1] - t0;
+    const int4 *rec = p.e2_rec + 2 * (long long)i0;
+    const double *prop = p.e2_prop + i0;
+    const int *lvl = p.e2_lvl + l0;
+    if (!(nt <= p.e2_cap_t && nev <= p.e2_cap_ev && nlv <= p.e2_cap_lv))
+        return e2_levels<NC, false>(p, tab, rep, day, uw, updw, rec, prop, lvl,
+                                    nlv, i0, nullptr);
+
+    const int tid = (int)threadIdx.x, nth = (int)blockDim.x;
+    int *lu = (int *)(lds + p.lds_u);
+    int *lupd = (int *)(lds + p.lds_upd);
+    int4 *lrec = (int4 *)(lds + p.lds_rec);
+    double *lprop = (double *)(lds + p.lds_prop);
+    int *llvl = (int *)(lds + p.lds_lvl);
+    double *lpre = (double *)(lds + p.lds_pre);
+    for (int i = tid; i < nt; i += nth) {
+        const int node = p.e2_tn[t0 + i];
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            lu[i * NC + c] = uw[node * NC + c];
+        });
+        lupd[i] = 0;
+    }
+    for (int i = tid; i < 2 * nev; i += nth)
+        lrec[i] = rec[i];
+    for (int i = tid; i < nev; i += nth)
+        lprop[i] = prop[i];
+    for (int i = tid; i <= nlv; i += nth)
+        llvl[i] = lvl[i];
+    /* The first uniforms of every event's stream, computed in parallel
+     * here so that the levels, which run one after another, find them in
+     * LDS instead of each running Philox on the critical path. */
+    for (int q = tid; q < nev * (SIMINF_E2_PRE / 2); q += nth) {
+        const int i = q / (SIMINF_E2_PRE / 2), b = q % (SIMINF_E2_PRE / 2);
+        const siminf_stream st = siminf_stream_make(
+            p.seed, SIMINF_DOMAIN_E2,
+            (uint64_t)rep * (uint64_t)p.len + (uint64_t)rec[2 * i].x);
+        double u0, u1;
+        siminf_uniform_pair(&st, (uint64_t)b, &u0, &u1);
+        lpre[i * SIMINF_E2_PRE + 2 * b] = u0;
+        lpre[i * SIMINF_E2_PRE + 2 * b + 1] = u1;
+    }
+    __syncthreads();
+    const int myerr = e2_levels<NC, true>(p, tab, rep, day, lu, lupd, lrec,
+                                          lprop, llvl, nlv, i0, lpre);
+    for (int i = tid; i < nt; i += nth) {
+        const int node = p.e2_tn[t0 + i];
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            uw[node * NC + c] = lu[i * NC + c];
+        });
+        if (lupd[i])
+            updw[node] = 1;
     }
     return myerr;
 }
@@ -352,7 +440,8 @@ template <class M, bool CANON, bool GRID, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    extern __shared__ int lds[];
+    extern __shared__ int4 lds4[];
+    unsigned char *lds = (unsigned char *)lds4;
     const int rep = GRID ? p.rep0 : (int)blockIdx.x;
     const int node = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
                           : (int)threadIdx.x;
@@ -361,8 +450,6 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     const long long g = base + node;
     int *uw = p.u_work + base * NC;
     int *updw = p.upd_work + base;
-    int *lu = lds;
-    int *lupd = lds + p.e2_max_touched * NC;
     unsigned long long fired = 0;
     double tt = p.tt0, next = p.next0;
     int U_it = 0, V_it = 0;
@@ -370,7 +457,9 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 
     NodeState<M> s;
     Ctx<M, CANON> cx;
-    int cur = 0, end = 0;
+    E1Cursor ec;
+
+    const EvTab tab = global_tables(p);
 
     auto barrier = [&]() {
         if constexpr (GRID) {
@@ -417,8 +506,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 err = SIMINF_ERR_INVALID_RATE;
         });
         updw[node] = 0;
-        cur = p.e1_ptr[node];
-        end = p.e1_ptr[node + 1];
+        ec.start(p, node);
         if (err) {
             record_error(p, rep, 0, PH_INIT, node, err);
             myerr = 1;
@@ -438,7 +526,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 record_error(p, rep, day, PH_SSA, node, err);
                 myerr = 1;
             } else {
-                err = e1_day<NC>(p, cur, end, tt, s.u, &s.rng, upd, bad);
+                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
                 if (err) {
                     record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
                     myerr = 1;
@@ -457,8 +545,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 
         /* B: external transfers in one workgroup. */
         if (!GRID || blockIdx.x == 0)
-            myerr |= e2_day<NC>(p, rep, day, uw, updw, lu, lupd,
-                                p.e2_max_touched);
+            myerr |= e2_day<NC>(p, tab, rep, day, uw, updw, lds);
         barrier();
         if (failed(day, PH_E2))
             break;

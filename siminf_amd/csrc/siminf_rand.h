@@ -36,7 +36,7 @@
 #include <stdint.h>
 
 #if defined(__HIPCC__)
-#  define SIMINF_FN __host__ __device__ static inline
+#  define SIMINF_FN __host__ __device__ static inline __attribute__((always_inline))
 #  define SIMINF_COLD_FN __host__ __device__ static __attribute__((noinline))
 #else
 #  define SIMINF_FN static inline
@@ -53,6 +53,8 @@ typedef struct siminf_stream {
     uint32_t id0, id1;     /* stream identity (domain in id1's top bits) */
     uint64_t j;            /* index of the next uniform */
     double cache;          /* uniform 2b+1 when j is odd */
+    const double *pre;     /* uniforms 0 .. npre-1, computed beforehand */
+    uint64_t npre;         /* even; 0 when nothing was computed ahead */
 } siminf_stream;
 
 SIMINF_FN uint32_t siminf_mulhi32(uint32_t a, uint32_t b)
@@ -69,14 +71,16 @@ SIMINF_FN void siminf_philox4x32_10(
     const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
     for (int round = 0; round < 10; round++) {
-        const uint32_t hi0 = siminf_mulhi32(M0, c0), lo0 = M0 * c0;
-        const uint32_t hi1 = siminf_mulhi32(M1, c2), lo1 = M1 * c2;
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        /* One 32x32->64 product per multiplier gives both halves (a
+         * single v_mad_u64_u32 on the device). */
+        const uint64_t p0 = (uint64_t)M0 * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)M1 * (uint64_t)c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
         c0 = n0;
-        c1 = lo1;
+        c1 = (uint32_t)p1;
         c2 = n2;
-        c3 = lo0;
+        c3 = (uint32_t)p0;
         k0 += W0;
         k1 += W1;
     }
@@ -102,13 +106,19 @@ SIMINF_FN siminf_stream siminf_stream_make(
     s.id1 = (domain << 28) | ((uint32_t)(id >> 32) & 0x0FFFFFFFu);
     s.j = 0;
     s.cache = 0.0;
+    s.pre = 0;
+    s.npre = 0;
     return s;
 }
 
-/* Next uniform of the stream, in (0, 1). */
+/* Next uniform of the stream, in (0, 1).  The first npre uniforms may
+ * have been computed ahead (by other lanes) into pre; they are the same
+ * values this function would compute. */
 SIMINF_FN double siminf_uniform(siminf_stream *s)
 {
     uint32_t w[4];
+    if (s->j < s->npre)
+        return s->pre[s->j++];
     if (s->j & 1u) {
         s->j++;
         return s->cache;
@@ -118,6 +128,17 @@ SIMINF_FN double siminf_uniform(siminf_stream *s)
     s->cache = siminf_u01(w[2], w[3]);
     s->j++;
     return siminf_u01(w[0], w[1]);
+}
+
+/* Uniforms 2b and 2b+1 of a stream (one Philox block). */
+SIMINF_FN void siminf_uniform_pair(const siminf_stream *s, uint64_t b,
+                                   double *u0, double *u1)
+{
+    uint32_t w[4];
+    siminf_philox4x32_10((uint32_t)b, (uint32_t)(b >> 32), s->id0, s->id1,
+                         s->key0, s->key1, w);
+    *u0 = siminf_u01(w[0], w[1]);
+    *u1 = siminf_u01(w[2], w[3]);
 }
 
 /* The j-th uniform of a stream without touching any stream state. */
@@ -283,9 +304,8 @@ SIMINF_FN int siminf_binomial_inv(siminf_stream *s, double p, int n)
     }
 }
 
-/* Binomial(n, p) with 0 < p <= 0.5 and n*p >= 30: BTRD.  Rarely taken,
- * so kept out of line to spare the callers' registers. */
-SIMINF_COLD_FN int siminf_binomial_btrd(siminf_stream *s, double p, int n)
+/* Binomial(n, p) with 0 < p <= 0.5 and n*p >= 30: BTRD. */
+SIMINF_FN int siminf_binomial_btrd(siminf_stream *s, double p, int n)
 {
     const double q = 1.0 - p;
     const double dn = (double)n;
@@ -367,6 +387,24 @@ SIMINF_COLD_FN int siminf_binomial_btrd(siminf_stream *s, double p, int n)
     }
 }
 
+/* BTRD is rarely taken, so it is kept out of line to spare the callers'
+ * registers.  The stream goes in and out by value: a pointer argument
+ * would force the caller's stream, which the SSA loop uses on every
+ * firing, out of registers into memory. */
+typedef struct siminf_btrd_out {
+    siminf_stream s;
+    int x;
+} siminf_btrd_out;
+
+SIMINF_COLD_FN siminf_btrd_out siminf_binomial_btrd_cold(siminf_stream s,
+                                                         double p, int n)
+{
+    siminf_btrd_out r;
+    r.x = siminf_binomial_btrd(&s, p, n);
+    r.s = s;
+    return r;
+}
+
 /* Binomial(n, p) variate for 0 <= p <= 1, n >= 0. */
 SIMINF_FN int siminf_binomial(siminf_stream *s, double p, int n)
 {
@@ -379,10 +417,13 @@ SIMINF_FN int siminf_binomial(siminf_stream *s, double p, int n)
         p = 1.0 - p;
         flip = 1;
     }
-    if ((double)n * p < 30.0)
+    if ((double)n * p < 30.0) {
         x = siminf_binomial_inv(s, p, n);
-    else
-        x = siminf_binomial_btrd(s, p, n);
+    } else {
+        const siminf_btrd_out r = siminf_binomial_btrd_cold(*s, p, n);
+        *s = r.s;
+        x = r.x;
+    }
     return flip ? n - x : x;
 }
 

@@ -51,15 +51,17 @@ def device_source(parsed, name):
              '#include "siminf_kernels.h"',
              '#include "siminf_plugin_api.h"',
              "",
-             "#pragma clang force_cuda_host_device begin"]
+             "#pragma clang force_cuda_host_device begin",
+             "#pragma clang attribute push (__attribute__((always_inline)), apply_to = function)"]
     lines += _fun_lines(parsed)
-    lines += ["#pragma clang force_cuda_host_device end", ""]
+    lines += ["#pragma clang attribute pop",
+              "#pragma clang force_cuda_host_device end", ""]
     lines += ["struct GeneratedModel {",
               "    static constexpr int NC = %d, NT = %d, ND = %d, NLD = %d, NGD = %d;"
               % (NC, NT, ND, NLD, NGD),
               "    static constexpr int S[NT][NC] = %s;" % fmt(S),
               "    static constexpr int G[NT][NT] = %s;" % fmt(G),
-              "    template <int J> __device__ static double rate(SIMINF_RATE_ARGS)",
+              "    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)",
               "    {"]
     for k in range(NT):
         kw = "if constexpr" if k == 0 else "else if constexpr"
@@ -71,7 +73,7 @@ def device_source(parsed, name):
             lines.append("        %s (J == %d) return trFun%d(u, v, ld, gd, t);"
                          % (kw, k, k + 1))
     lines += ["    }",
-              "    __device__ static int pts(SIMINF_PTS_ARGS)",
+              "    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)",
               "    {",
               "        return ptsFun(vn, u, v, ld, gd, node, t);",
               "    }",
@@ -151,6 +153,14 @@ def compile_host(parsed):
             raise RuntimeError("gcc failed for the generated model:\n" + r.stderr[-4000:])
         os.replace(tmp, so)
     return so
+
+
+def clear_cache():
+    """Remove every cached plugin source and object."""
+    if os.path.isdir(CACHE):
+        for f in os.listdir(CACHE):
+            if f.startswith(("dev_", "host_")):
+                os.remove(os.path.join(CACHE, f))
 
 
 def load_device(so):
