@@ -45,6 +45,9 @@ def load():
         lib.oracle_hypergeometric.argtypes = [ctypes.c_ulonglong, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, ip]
+        lib.oracle_prefetch_mismatches.restype = ctypes.c_int
+        lib.oracle_prefetch_mismatches.argtypes = [ctypes.c_ulonglong, ctypes.c_int,
+                                                   ctypes.c_int]
         lib.oracle_sample.restype = ctypes.c_int
         lib.oracle_sample.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ip,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -128,6 +131,10 @@ def hypergeometric(seed, n1, n2, t, count):
     load().oracle_hypergeometric(seed, n1, n2, t, count,
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     return out
+
+
+def prefetch_mismatches(seed, npre, count):
+    return load().oracle_prefetch_mismatches(seed, npre, count)
 
 
 def sample(E, u_node, select, n, proportion, seed, count, enter=False):

@@ -1026,6 +1026,48 @@ void oracle_hypergeometric(unsigned long long seed, int n1, int n2, int t,
         out[i] = siminf_hypergeometric(&s, n1, n2, t);
 }
 
+/* Streams whose first npre uniforms were computed ahead must give the
+ * same variates and leave the stream at the same position as streams
+ * that compute every uniform themselves.  Runs count trials over a
+ * spread of urns (small and large, complement branch, urns that run out
+ * of good items) followed by a binomial and three plain uniforms, and
+ * returns the number of trials that differ. */
+int oracle_prefetch_mismatches(unsigned long long seed, int npre, int count)
+{
+    double pre[64];
+    int bad = 0;
+    uint32_t x = 12345u;
+    if (npre > 64 || npre % 2)
+        return -1;
+    for (int i = 0; i < count; i++) {
+        siminf_stream a = siminf_stream_make(seed, SIMINF_DOMAIN_E2, (uint64_t)i);
+        siminf_stream b = a;
+        int n1, n2, t, ka, kb, ba, bb;
+        x = x * 1664525u + 1013904223u;
+        n1 = (int)(x >> 8) % 40;
+        x = x * 1664525u + 1013904223u;
+        n2 = (int)(x >> 8) % 40;
+        x = x * 1664525u + 1013904223u;
+        t = (int)(x >> 8) % (n1 + n2 + 2);
+        for (int q = 0; q < npre / 2; q++)
+            siminf_uniform_pair(&b, (uint64_t)q, &pre[2 * q], &pre[2 * q + 1]);
+        b.pre = pre;
+        b.npre = (uint64_t)npre;
+        ka = siminf_hypergeometric(&a, n1, n2, t);
+        kb = siminf_hypergeometric(&b, n1, n2, t);
+        ba = siminf_binomial(&a, 0.3, n1 + 3);
+        bb = siminf_binomial(&b, 0.3, n1 + 3);
+        if (ka != kb || ba != bb || a.j != b.j) {
+            bad++;
+            continue;
+        }
+        for (int r = 0; r < 3; r++)
+            if (siminf_uniform(&a) != siminf_uniform(&b))
+                bad++;
+    }
+    return bad;
+}
+
 /* sample_select / sample_select_enter on one node's state, repeated
  * count times from one stream; out is count x Nc. */
 int oracle_sample(int enter, const int *irE, const int *jcE, const double *prE,

@@ -535,6 +535,18 @@ __device__ __forceinline__ int last_nonzero_at_or_below(
     return j;
 }
 
+/* The run-time-indexed element a[i], as a masked sum (see add_at). */
+template <int NC>
+__device__ __forceinline__ int value_at(const int (&a)[NC], int i)
+{
+    int x = 0;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        x |= a[c] & -(int)(i == c);
+    });
+    return x;
+}
+
 /* Is the run-time-indexed element a[i] negative?  Written as a masked
  * sum, like add_at: a chain of compares would be folded back into a
  * run-time index, which moves the array to scratch memory. */
@@ -623,24 +635,22 @@ __device__ int sample_select(const int (&u)[NC], uint32_t mask,
 
     if (equal) {
         /* Conditional hypergeometric draws over the rows in order; the
-         * row where the loop stops takes the remainder. */
+         * last row takes the remainder (once n reaches zero the rest get
+         * nothing).  A run-time loop over the rows, so that every lane
+         * runs the same single copy of the urn whichever rows its column
+         * holds. */
         const int last = mask_last(mask);
-        bool stopped = false;
-        int stop_row = last;
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            if (!stopped && (mask & (1u << c)) && c != last) {
-                if (n == 0) {
-                    stopped = true;
-                    stop_row = c;
-                } else {
-                    ind[c] = siminf_hypergeometric(rng, u[c], Nind - u[c], n);
-                    Nind -= u[c];
-                    n -= ind[c];
-                }
-            }
-        });
-        add_at(ind, stop_row, n);
+        uint32_t rows = mask & ~(1u << last);
+        while (rows && n > 0) {
+            const int c = mask_first(rows);
+            rows &= rows - 1u;
+            const int uc = value_at(u, c);
+            const int x = siminf_hypergeometric(rng, uc, Nind - uc, n);
+            add_at(ind, c, x);
+            Nind -= uc;
+            n -= x;
+        }
+        add_at(ind, last, n);
         return 0;
     }
 
@@ -854,6 +864,33 @@ __device__ int apply_e2(const EvTab &tab, int n, int select, int shift,
     err = sample_select<NC>(us, mask, w, n, prop, ind, rng);
     if (err)
         return err;
+
+    if (shift < 0 && src != dst) {
+        /* No shift and two distinct herds: the destination's counts are
+         * loaded together and both herds written back in one pass. */
+        int ud[NC];
+        bool neg = false;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c))
+                ud[c] = uw[dst * NC + c];
+        });
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (mask & (1u << c)) {
+                ud[c] += ind[c];
+                us[c] -= ind[c];
+                neg = neg | (ud[c] < 0) | (us[c] < 0);
+                uw[dst * NC + c] = ud[c];
+                uw[src * NC + c] = us[c];
+            }
+        });
+        if (neg)
+            return SIMINF_ERR_NEGATIVE_STATE;
+        upd[dst] = 1;
+        upd[src] = 1;
+        return 0;
+    }
 
     static_for<NC>([&](auto C) {
         constexpr int c = decltype(C)::value;

@@ -275,15 +275,14 @@ static int event_day(int tm, double tt0)
     return (int)((double)tm - std::floor(tt0));
 }
 
-/* LDS of the coupled kernel, 64 KiB at most: buffers for one day's E2
- * work sized for the largest day or, when that does not fit, the budget
- * split 45 / 50 / 5 % over herds, event records with their precomputed
- * uniforms, and level bounds (larger days run from global memory).  Sets the offsets in p and returns the
- * byte size. */
+/* LDS of the coupled kernel, about `budget` bytes at most: buffers for
+ * one day's E2 work sized for the largest day or, when that does not fit,
+ * the budget split 45 / 50 / 5 % over herds, event records with their
+ * precomputed uniforms, and level bounds (larger days run from global
+ * memory).  Sets the offsets in p and returns the byte size. */
 static size_t coupled_lds_layout(KParams &p, int Nc, int max_t, int max_ev,
-                                 int max_lv)
+                                 int max_lv, size_t budget)
 {
-    const size_t budget = 64 * 1024;
     auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
     const size_t per_t = (size_t)(Nc + 1) * sizeof(int);
     const size_t per_ev = 2 * sizeof(int4) + (1 + SIMINF_E2_PRE) * sizeof(double);
@@ -295,10 +294,11 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int max_t, int max_ev,
     };
     size_t cap_t = (size_t)max_t, cap_ev = (size_t)max_ev, cap_lv = (size_t)max_lv;
     if (need(cap_t, cap_ev, cap_lv) > budget) {
-        const size_t rest = budget - 128;
+        const size_t rest = budget > 1024 ? budget - 256 : 768;
+        const size_t lv_slots = rest * 5 / 100 / sizeof(int);
         cap_t = std::min(cap_t, rest * 45 / 100 / per_t);
         cap_ev = std::min(cap_ev, rest * 50 / 100 / per_ev);
-        cap_lv = std::min(cap_lv, rest * 5 / 100 / sizeof(int) - 1);
+        cap_lv = std::min(cap_lv, lv_slots > 1 ? lv_slots - 1 : 0);
     }
     p.e2_cap_t = (int)cap_t;
     p.e2_cap_ev = (int)cap_ev;
@@ -639,8 +639,29 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     if (P->kernel == K_COUPLED_GRID || P->kernel == K_COUPLED_REPS) {
         p.u_work = B.alloc<int>((size_t)lanes * Nc);
         p.upd_work = B.alloc<int>((size_t)lanes);
+        /* LDS budget: at most 128 KiB, and small enough that the
+         * workgroups a grid-wide launch needs fit on the CUs together
+         * (every workgroup gets the same allocation). */
+        int dev = 0, lds_max = 64 * 1024, lds_cu = 64 * 1024, cus = 1;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                  dev) != hipSuccess)
+            lds_max = 64 * 1024;
+        if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  dev) != hipSuccess)
+            lds_cu = lds_max;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess || cus < 1)
+            cus = 1;
+        size_t budget = std::min<size_t>((size_t)lds_max, 128 * 1024);
+        if (P->kernel == K_COUPLED_GRID) {
+            const int blocks = (Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK;
+            const int per_cu = (blocks + cus - 1) / cus;
+            const size_t share = (size_t)lds_cu / (size_t)per_cu;
+            budget = std::min<size_t>(budget, share > 2048 ? share - 1024 : 1024);
+        }
         P->lds = coupled_lds_layout(p, Nc, max_touched, max_day_events,
-                                    max_day_levels);
+                                    max_day_levels, budget);
         if (P->kernel == K_COUPLED_GRID) {
             P->block = SIMINF_GRID_BLOCK;
             P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK);

@@ -311,7 +311,7 @@ __device__ __forceinline__ bool error_by(const KParams &p, int rep, int day,
 }
 
 /* Uniforms computed ahead per external transfer event (even). */
-#define SIMINF_E2_PRE 8
+#define SIMINF_E2_PRE 12
 
 /* The levels of one day's external transfers over a state array su (NC
  * ints per slot) with update flags sf, event records rec/prop and level

@@ -420,8 +420,12 @@ SIMINF_FN int siminf_binomial(siminf_stream *s, double p, int n)
     if ((double)n * p < 30.0) {
         x = siminf_binomial_inv(s, p, n);
     } else {
+        /* Only the position and the cached uniform come back: the other
+         * fields stay what the caller knows them to be (for a node's
+         * stream, no uniforms computed ahead). */
         const siminf_btrd_out r = siminf_binomial_btrd_cold(*s, p, n);
-        *s = r.s;
+        s->j = r.s.j;
+        s->cache = r.s.cache;
         x = r.x;
     }
     return flip ? n - x : x;
@@ -438,6 +442,25 @@ SIMINF_FN int siminf_hypergeometric(siminf_stream *s, int n1, int n2, int t)
     if (t > total)
         t = total;
     draws = (2 * t <= total) ? t : total - t;
+    if (s->j + (uint64_t)draws <= s->npre) {
+        /* Every uniform the urn can need was computed ahead: the same
+         * draws as below, without the stream's branches, and with the
+         * counts kept as doubles (exact: they are integers < 2^31). */
+        const double *pre = s->pre + s->j;
+        double dgood = (double)good, dleft = (double)left;
+        int i = 0;
+        while (i < draws) {
+            const int take = pre[i] * dleft < dgood;
+            i++;
+            k += take;
+            dgood -= (double)take;
+            if (take && dgood == 0.0)
+                break;
+            dleft -= 1.0;
+        }
+        s->j += (uint64_t)i;
+        return (2 * t <= total) ? k : n1 - k;
+    }
     for (int i = 0; i < draws; i++) {
         const double u = siminf_uniform(s);
         if (u * (double)left < (double)good) {

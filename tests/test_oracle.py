@@ -167,6 +167,15 @@ def test_sample_select_deterministic_branches(oracle):
     assert oracle.sample(E, u, 1, 0, -0.1, 1, 1)[0] == -18
 
 
+@pytest.mark.parametrize("npre", [0, 2, 8, 12, 32])
+def test_precomputed_uniforms_change_nothing(oracle, npre):
+    """The device computes an external transfer's first uniforms ahead in
+    parallel and the urn then runs over them without the stream's
+    branches; variates and stream positions must be those of a stream
+    that computes every uniform itself."""
+    assert oracle.prefetch_mismatches(77, npre, 20000) == 0
+
+
 def test_sample_select_equal_weights_is_hypergeometric(oracle):
     E = _E([{0: 1, 1: 1, 2: 1}], 3)
     u = np.array([5, 8, 7])
