@@ -123,6 +123,9 @@ struct KParams {
      * (other days run from global memory). */
     int e2_cap_t, e2_cap_ev, e2_cap_lv;
     int lds_u, lds_upd, lds_rec, lds_prop, lds_lvl, lds_pre;
+    /* E and N for the E2 work, staged once per kernel (zero caps when
+     * they do not fit). */
+    int lds_Ew, lds_Emask, lds_N;
 
     /* Output: dense U/V or sparse (irU, jcU, prU) / (irV, jcV, prV). */
     int *U;

@@ -171,7 +171,8 @@ extern "C" const SimInf_hip_model *SimInf_hip_model_builtin(const char *name)
 extern "C" int SimInf_hip_model_dims(const SimInf_hip_model *model, int *Nc,
                                      int *Nt, int *Nd)
 {
-    if (!model)
+    if (!model || model->abi != SIMINF_HIP_PLUGIN_ABI ||
+        model->kparams_sig != SIMINF_KPARAMS_SIG)
         return SIMINF_ERR_INVALID_MODEL;
     if (Nc) *Nc = model->Nc;
     if (Nt) *Nt = model->Nt;
@@ -280,10 +281,27 @@ static int event_day(int tm, double tt0)
  * the budget split 45 / 50 / 5 % over herds, event records with their
  * precomputed uniforms, and level bounds (larger days run from global
  * memory).  Sets the offsets in p and returns the byte size. */
-static size_t coupled_lds_layout(KParams &p, int Nc, int max_t, int max_ev,
-                                 int max_lv, size_t budget)
+static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
+                                 bool has_N, int max_t, int max_ev, int max_lv,
+                                 size_t budget)
 {
     auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    /* E and N first; when they take more than a quarter of the budget
+     * every day runs from global memory. */
+    size_t tab = 0;
+    p.lds_Ew = (int)tab;
+    tab = align16(tab + (size_t)std::max(Nselect, 1) * Nc * sizeof(double));
+    p.lds_Emask = (int)tab;
+    tab = align16(tab + (size_t)std::max(Nselect, 1) * sizeof(uint32_t));
+    p.lds_N = (int)tab;
+    if (has_N)
+        tab = align16(tab + (size_t)std::max(Nshift, 1) * Nc * sizeof(int));
+    if (tab > budget / 4) {
+        p.e2_cap_t = p.e2_cap_ev = p.e2_cap_lv = 0;
+        p.lds_u = p.lds_upd = p.lds_rec = p.lds_prop = p.lds_lvl = p.lds_pre = 0;
+        return 16;
+    }
+    budget -= tab;
     const size_t per_t = (size_t)(Nc + 1) * sizeof(int);
     const size_t per_ev = 2 * sizeof(int4) + (1 + SIMINF_E2_PRE) * sizeof(double);
     auto need = [&](size_t t, size_t ev, size_t lv) {
@@ -303,7 +321,7 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int max_t, int max_ev,
     p.e2_cap_t = (int)cap_t;
     p.e2_cap_ev = (int)cap_ev;
     p.e2_cap_lv = (int)cap_lv;
-    size_t off = 0;
+    size_t off = tab;
     p.lds_u = (int)off;
     off = align16(off + cap_t * Nc * sizeof(int));
     p.lds_upd = (int)off;
@@ -327,7 +345,7 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
         return SIMINF_ERR_INVALID_MODEL;
     *out = nullptr;
 
-    if (model->abi != SIMINF_HIP_PLUGIN_ABI)
+    if (model->abi != SIMINF_HIP_PLUGIN_ABI || model->kparams_sig != SIMINF_KPARAMS_SIG)
         return SIMINF_ERR_INVALID_MODEL;
     const int Nn = a->Nn, Nc = a->Nc, Nt = a->Nt, Nd = a->Nd;
     const int Nrep = a->Nrep < 1 ? 1 : a->Nrep;
@@ -660,8 +678,9 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
             const size_t share = (size_t)lds_cu / (size_t)per_cu;
             budget = std::min<size_t>(budget, share > 2048 ? share - 1024 : 1024);
         }
-        P->lds = coupled_lds_layout(p, Nc, max_touched, max_day_events,
-                                    max_day_levels, budget);
+        P->lds = coupled_lds_layout(p, Nc, Nselect, Nshift, a->N != nullptr,
+                                    max_touched, max_day_events, max_day_levels,
+                                    budget);
         if (P->kernel == K_COUPLED_GRID) {
             P->block = SIMINF_GRID_BLOCK;
             P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK);

@@ -311,7 +311,7 @@ __device__ __forceinline__ bool error_by(const KParams &p, int rep, int day,
 }
 
 /* Uniforms computed ahead per external transfer event (even). */
-#define SIMINF_E2_PRE 12
+#define SIMINF_E2_PRE SIMINF_PRE_MAX
 
 /* The levels of one day's external transfers over a state array su (NC
  * ints per slot) with update flags sf, event records rec/prop and level
@@ -379,6 +379,9 @@ __device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int re
                                     nlv, i0, nullptr);
 
     const int tid = (int)threadIdx.x, nth = (int)blockDim.x;
+    const EvTab ltab{(const uint32_t *)(lds + p.lds_Emask),
+                     (const double *)(lds + p.lds_Ew),
+                     p.Nmat ? (const int *)(lds + p.lds_N) : nullptr};
     int *lu = (int *)(lds + p.lds_u);
     int *lupd = (int *)(lds + p.lds_upd);
     int4 *lrec = (int4 *)(lds + p.lds_rec);
@@ -413,7 +416,7 @@ __device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int re
         lpre[i * SIMINF_E2_PRE + 2 * b + 1] = u1;
     }
     __syncthreads();
-    const int myerr = e2_levels<NC, true>(p, tab, rep, day, lu, lupd, lrec,
+    const int myerr = e2_levels<NC, true>(p, ltab, rep, day, lu, lupd, lrec,
                                           lprop, llvl, nlv, i0, lpre);
     for (int i = tid; i < nt; i += nth) {
         const int node = p.e2_tn[t0 + i];
@@ -460,6 +463,24 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     E1Cursor ec;
 
     const EvTab tab = global_tables(p);
+
+    /* E and N into LDS for the workgroup that runs the E2 levels. */
+    if (!GRID || blockIdx.x == 0) {
+        if (p.e2_cap_ev > 0) {
+            uint32_t *lEmask = (uint32_t *)(lds + p.lds_Emask);
+            double *lEw = (double *)(lds + p.lds_Ew);
+            for (int i = threadIdx.x; i < p.Nselect; i += blockDim.x)
+                lEmask[i] = p.Emask[i];
+            for (int i = threadIdx.x; i < p.Nselect * NC; i += blockDim.x)
+                lEw[i] = p.Ew[i];
+            if (p.Nmat) {
+                int *lN = (int *)(lds + p.lds_N);
+                for (int i = threadIdx.x; i < p.Nshift * NC; i += blockDim.x)
+                    lN[i] = p.Nmat[i];
+            }
+        }
+        __syncthreads();
+    }
 
     auto barrier = [&]() {
         if constexpr (GRID) {
@@ -593,12 +614,24 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 }
 
 /* Layout version of SimInf_hip_model, checked when a plugin is used. */
-#define SIMINF_HIP_PLUGIN_ABI 1
+#define SIMINF_HIP_PLUGIN_ABI 2
+
+/* Signature of the KParams layout a plugin was compiled against: the
+ * library hands its KParams to the plugin's kernels, so a plugin built
+ * from other headers than the library is refused instead of run. */
+#define SIMINF_KPARAMS_SIG                                                 \
+    ((long long)sizeof(siminf::KParams) * 1000003LL +                      \
+     (long long)offsetof(siminf::KParams, e1_rec) * 10007LL +              \
+     (long long)offsetof(siminf::KParams, e2_rec) * 1009LL +               \
+     (long long)offsetof(siminf::KParams, lds_pre) * 101LL +               \
+     (long long)offsetof(siminf::KParams, U) * 11LL +                      \
+     (long long)offsetof(siminf::KParams, err_key))
 
 } /* namespace siminf */
 
 struct SimInf_hip_model {
     int abi;
+    long long kparams_sig;
     const char *name;
     int Nc, Nt, Nd, Ngdata;
     const int *S;    /* canonical S: S[k*Nc + c] = S(c, k) */
@@ -715,7 +748,8 @@ template <class M>
 inline constexpr CanonTables<M> canon_tables{};
 
 #define SIMINF_MODEL_ENTRY(NAME, M, NGDATA)                               \
-    {SIMINF_HIP_PLUGIN_ABI, NAME, M::NC, M::NT, M::ND, NGDATA,            \
+    {SIMINF_HIP_PLUGIN_ABI, SIMINF_KPARAMS_SIG, NAME, M::NC, M::NT, M::ND, \
+     NGDATA,                                                              \
      siminf::canon_tables<M>.S, siminf::canon_tables<M>.G,               \
      siminf::launch_ssm_nodes<M>, siminf::launch_aem_nodes<M>,            \
      siminf::launch_coupled<M>}

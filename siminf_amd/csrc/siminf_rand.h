@@ -43,6 +43,16 @@
 #  define SIMINF_COLD_FN static __attribute__((noinline))
 #endif
 
+#if defined(__clang__)
+#  define SIMINF_UNROLL _Pragma("unroll")
+#else
+#  define SIMINF_UNROLL
+#endif
+
+/* Longest run of uniforms a sampler takes from values computed ahead
+ * (callers compute at most this many per stream). */
+#define SIMINF_PRE_MAX 12
+
 /* Stream domains. */
 #define SIMINF_DOMAIN_NODE  0u  /* (replicate, node): SSA and E1 events */
 #define SIMINF_DOMAIN_E2    1u  /* (replicate, event): one per E2 event */
@@ -442,23 +452,32 @@ SIMINF_FN int siminf_hypergeometric(siminf_stream *s, int n1, int n2, int t)
     if (t > total)
         t = total;
     draws = (2 * t <= total) ? t : total - t;
-    if (s->j + (uint64_t)draws <= s->npre) {
+    if (draws <= SIMINF_PRE_MAX && s->j + (uint64_t)draws <= s->npre) {
         /* Every uniform the urn can need was computed ahead: the same
-         * draws as below, without the stream's branches, and with the
-         * counts kept as doubles (exact: they are integers < 2^31). */
+         * draws as below, from values loaded together up front, in a
+         * fully unrolled loop without the stream's branches; the counts
+         * are kept as doubles (exact: integers < 2^31). */
         const double *pre = s->pre + s->j;
+        double v[SIMINF_PRE_MAX];
         double dgood = (double)good, dleft = (double)left;
-        int i = 0;
-        while (i < draws) {
-            const int take = pre[i] * dleft < dgood;
-            i++;
-            k += take;
-            dgood -= (double)take;
-            if (take && dgood == 0.0)
-                break;
-            dleft -= 1.0;
+        int used = 0, done = 0;
+        SIMINF_UNROLL
+        for (int i = 0; i < SIMINF_PRE_MAX; i++)
+            v[i] = i < draws ? pre[i] : 0.0;
+        SIMINF_UNROLL
+        for (int i = 0; i < SIMINF_PRE_MAX; i++) {
+            if (i < draws && !done) {
+                const int take = v[i] * dleft < dgood;
+                used = i + 1;
+                k += take;
+                dgood -= (double)take;
+                if (take && dgood == 0.0)
+                    done = 1;
+                else
+                    dleft -= 1.0;
+            }
         }
-        s->j += (uint64_t)i;
+        s->j += (uint64_t)used;
         return (2 * t <= total) ? k : n1 - k;
     }
     for (int i = 0; i < draws; i++) {

@@ -198,3 +198,20 @@ def test_device_plugin_builds_and_exports():
     assert _native.load().SimInf_hip_model_dims(
         ctypes.c_void_p(dev), ctypes.byref(nc), ctypes.byref(nt), ctypes.byref(nd)) == 0
     assert (nc.value, nt.value, nd.value) == (2, 2, 1)
+
+
+def test_plugin_built_against_other_headers_is_refused():
+    """A device plugin records the ABI version and the layout signature of
+    the kernel parameter block it was compiled against; the library
+    refuses one that differs (it would otherwise hand the plugin's kernels
+    a parameter block of another layout)."""
+    import ctypes
+    from siminf_amd import _native
+    lib = _native.load()
+    good = lib.SimInf_hip_model_builtin(b"SIR")
+    assert lib.SimInf_hip_model_dims(ctypes.c_void_p(good), None, None, None) == 0
+    raw = (ctypes.c_char * 16).from_address(good).raw
+    fake = ctypes.create_string_buffer(raw + b"\0" * 112)   # abi, signature, ...
+    ctypes.memmove(ctypes.addressof(fake) + 8, ctypes.byref(ctypes.c_longlong(12345)), 8)
+    assert lib.SimInf_hip_model_dims(ctypes.cast(fake, ctypes.c_void_p), None, None,
+                                     None) == -9     # SIMINF_ERR_INVALID_MODEL
