@@ -111,16 +111,23 @@ def main():
     if err:
         raise SystemExit("plan_create failed: %d" % err)
 
+    split = []
+
     def step(seed):
         e = lib.SimInf_hip_plan_run(plan, seed)
         if e:
             raise SystemExit("run failed: %d %s" % (e, lib.SimInf_hip_error_message(e)))
         tr, nd, ms = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
         lib.SimInf_hip_plan_stats(plan, ctypes.byref(tr), ctypes.byref(nd), ctypes.byref(ms))
+        sm, fm, fc = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        lib.SimInf_hip_plan_kernel_times(plan, ctypes.byref(sm), ctypes.byref(fm),
+                                         ctypes.byref(fc))
+        split.append((sm.value, fm.value, fc.value))
         return tr.value, nd.value, ms.value
 
     for i in range(args.warmup):
         step(100 + i)
+    del split[:]
     lat, kms, trans = [], [], 0
     node_days = 0
     t0 = time.perf_counter()
@@ -134,8 +141,21 @@ def main():
     wall = time.perf_counter() - t0
     lib.SimInf_hip_plan_destroy(plan)
 
-    bytes_per_launch = algorithmic_bytes(model)
-    kernel_s = statistics.mean(kms) / 1e3
+    # Roofline of the dominant kernel.  The solver kernel's algorithmic
+    # bytes are the whole run's less the trajectory columns the fill kernel
+    # writes; the fill kernel's are those columns (Nc ints and Nd doubles
+    # per node-column) plus its read of each frozen node's state.
+    total_bytes = algorithmic_bytes(model)
+    solver_ms = statistics.mean(x[0] for x in split)
+    fill_ms = statistics.mean(x[1] for x in split)
+    fill_cols = statistics.mean(x[2] for x in split)
+    per_col = 4 * model.Nc + 8 * model.Nd
+    fill_bytes = fill_cols * per_col
+    if fill_ms > solver_ms:
+        dominant, kernel_s, bytes_per_launch = "k_fill_frozen", fill_ms / 1e3, fill_bytes
+    else:
+        dominant, kernel_s = "solver", solver_ms / 1e3
+        bytes_per_launch = total_bytes - fill_bytes
     achieved = bytes_per_launch / kernel_s / 1e9
     out = {
         "metric": "node-days/sec (SSA transitions/sec and p50/p99 simulation latency alongside)",
@@ -156,9 +176,11 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "kernel_ms_mean": statistics.mean(kms),
+        "kernel_ms_split": {"solver": solver_ms, "fill": fill_ms,
+                            "fill_node_columns": fill_cols},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": None, "kernel": dominant,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if not args.no_cpu_baseline:

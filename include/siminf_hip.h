@@ -202,6 +202,13 @@ int SimInf_hip_plan_stats(const SimInf_hip_plan *plan,
                           long long *transitions, long long *node_days,
                           double *kernel_ms);
 
+/* Device times of the last run split by kernel (no reference
+ * counterpart): the solver kernel(s), and the kernel that writes the
+ * trajectory columns of nodes whose state stopped changing (0 when the
+ * plan has none), with the number of node-columns it wrote. */
+int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *plan, double *solver_ms,
+                                 double *fill_ms, long long *fill_columns);
+
 void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
 
 #ifdef __cplusplus

@@ -485,21 +485,24 @@ static int oracle_direct_replicate(oracle_ctx *c, int rep, int is_mssm)
         /* (1) SSA per node (ssm.c:144-235). */
         for (int node = 0; node < Nn && !error; node++) {
             for (;;) {
-                double cum, rnd, tau, delta = 0.0;
+                double cum, rnd, tau, u_tau, u_tr, delta = 0.0;
                 int tr;
 
                 if (sum_t_rate[node] <= 0.0) {
                     t_time[node] = next_unit_of_time;
                     break;
                 }
-                tau = -siminf_log(siminf_uniform(&rng[node])) / sum_t_rate[node];
+                /* One aligned pair of uniforms per attempt: the waiting
+                 * time from the first, the transition from the second. */
+                siminf_uniform2(&rng[node], &u_tau, &u_tr);
+                tau = -siminf_log(u_tau) / sum_t_rate[node];
                 if ((tau + t_time[node]) >= next_unit_of_time) {
                     t_time[node] = next_unit_of_time;
                     break;
                 }
                 t_time[node] += tau;
 
-                rnd = siminf_uniform(&rng[node]) * sum_t_rate[node];
+                rnd = u_tr * sum_t_rate[node];
                 tr = 0;
                 cum = t_rate[node * Nt];
                 while (tr < Nt && rnd > cum) {

@@ -81,6 +81,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_run",
     "SimInf_hip_plan_download",
     "SimInf_hip_plan_stats",
+    "SimInf_hip_plan_kernel_times",
     "SimInf_hip_plan_destroy",
 )
 
@@ -123,6 +124,10 @@ def load(path=None):
     lib.SimInf_hip_plan_stats.argtypes = [
         vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
         ctypes.POINTER(ctypes.c_double)]
+    lib.SimInf_hip_plan_kernel_times.restype = ctypes.c_int
+    lib.SimInf_hip_plan_kernel_times.argtypes = [
+        vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+        ctypes.POINTER(ctypes.c_longlong)]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
     if path == LIB_PATH:

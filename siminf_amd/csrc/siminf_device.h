@@ -23,6 +23,8 @@
 #include <utility>
 
 #include "siminf_rand.h"
+
+#include <type_traits>
 #include "siminf_models.h"
 #include "../../include/siminf_hip.h"
 
@@ -142,6 +144,17 @@ struct KParams {
     int *upd_work;                   /* Nn*Nrep */
     int rep0;
 
+    /* Node kernels with a dense trajectory: a node whose state can no
+     * longer change records the first column it has not written
+     * (frozen_from, tlen when it never froze) and its state; a fill
+     * kernel then writes those columns in trajectory order.
+     * frozen_cols counts the node-columns left to it.  Null when
+     * unused. */
+    int *frozen_from;
+    int *frozen_u;
+    double *frozen_v;
+    unsigned long long *frozen_cols;
+
     /* Results. */
     unsigned long long *err_key;
     unsigned long long *transitions;
@@ -209,6 +222,7 @@ struct NodeState {
  * once from the kernel arguments. */
 template <class M, bool CANON>
 struct Ctx {
+    static constexpr bool canon = CANON;
     double ld[M::NLD > 0 ? M::NLD : 1];
     double gd[M::NGD > 0 ? M::NGD : 1];
     uint32_t smask[CANON ? 1 : M::NT];
@@ -300,6 +314,60 @@ __device__ __forceinline__ int fire(NodeState<M> &s, const CT &cx, double t)
     return err;
 }
 
+/* Bits k of G's row j: the transitions whose firing makes rate j
+ * stale. */
+template <class M, int J>
+__device__ __forceinline__ constexpr unsigned g_row_mask()
+{
+    unsigned m = 0;
+    for (int k = 0; k < M::NT; k++)
+        if (M::G[k][J] != 0)
+            m |= 1u << k;
+    return m;
+}
+
+/* fire<M, K> for a run-time transition tr in one code path, for models
+ * whose S and G are folded into the code: the state moves by S's column
+ * tr through selects, and every rate G marks is recomputed in G's row
+ * order (the others are evaluated and discarded, which changes nothing:
+ * rates are pure functions).  Lanes of a wavefront that picked different
+ * transitions then run the same instructions instead of one variant
+ * after another. */
+template <class M, class CT>
+__device__ __forceinline__ int fire_any(NodeState<M> &s, const CT &cx, int tr,
+                                        double t)
+{
+    int err = 0;
+    static_for<M::NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        int d = 0;
+        static_for<M::NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if constexpr (M::S[k][c] != 0)
+                d += tr == k ? M::S[k][c] : 0;
+        });
+        s.u[c] += d;
+        if (s.u[c] < 0)
+            err = SIMINF_ERR_NEGATIVE_STATE;
+    });
+    double delta = 0.0;
+    static_for<M::NT>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr unsigned gm = g_row_mask<M, j>();
+        if constexpr (gm != 0) {
+            const bool in = (gm >> tr) & 1u;
+            const double old = s.rate[j];
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, t);
+            s.rate[j] = in ? r : old;
+            delta += in ? r - old : 0.0;
+            if (in && !rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+        }
+    });
+    s.a0 += delta;
+    return err;
+}
+
 template <class M, class CT>
 __device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
                                        double tt, double next,
@@ -311,14 +379,16 @@ __device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
     for (;;) {
         if (s.a0 <= 0.0)
             break;
-        const double tau = -siminf_log(siminf_uniform(&s.rng)) / s.a0;
+        double u_tau, u_tr;
+        siminf_uniform2(&s.rng, &u_tau, &u_tr);
+        const double tau = -siminf_log(u_tau) / s.a0;
         if ((tau + t) >= next)
             break;
         t += tau;
 
         /* Linear scan of the cumulative rates; the rate of the selected
          * transition is carried along (no run-time indexing). */
-        const double rnd = siminf_uniform(&s.rng) * s.a0;
+        const double rnd = u_tr * s.a0;
         int tr = 0;
         double rt = s.rate[0];
         double cum = s.rate[0];
@@ -350,14 +420,18 @@ __device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
         }
 
         fired++;
-        static_for<NT>([&](auto K) {
-            constexpr int k = decltype(K)::value;
-            if (tr == k) {
-                const int e = fire<M, k>(s, cx, t);
-                if (e)
-                    err = e;
-            }
-        });
+        if constexpr (CT::canon && NT <= 8) {
+            err = fire_any<M>(s, cx, tr, t);
+        } else {
+            static_for<NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                if (tr == k) {
+                    const int e = fire<M, k>(s, cx, t);
+                    if (e)
+                        err = e;
+                }
+            });
+        }
         /* The node stops at its first failing step. */
         if (err)
             break;
@@ -992,6 +1066,46 @@ __device__ __forceinline__ void store_columns(
         }
         V_it++;
     }
+}
+
+/* Does the model declare that its post-time-step function never changes
+ * anything (PTS_NOOP)?  Models that do not say are assumed to. */
+template <class M, class = void>
+struct pts_noop : std::false_type {};
+template <class M>
+struct pts_noop<M, std::void_t<decltype(M::PTS_NOOP)>>
+    : std::bool_constant<M::PTS_NOOP> {};
+
+/* A node whose state can no longer change (all rates zero, no events
+ * left, a post-time-step function that does nothing) holds the same
+ * state in every remaining column: record it for the fill kernel
+ * instead of stepping the remaining days. */
+template <int NC, int ND>
+__device__ __forceinline__ void record_frozen(const KParams &p, long long g,
+                                              int U_it, const int (&u)[NC],
+                                              const double (&vnew)[ND > 0 ? ND : 1])
+{
+    p.frozen_from[g] = U_it;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        p.frozen_u[g * NC + c] = u[c];
+    });
+    if constexpr (ND > 0) {
+        static_for<ND>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            p.frozen_v[g * ND + c] = vnew[c];
+        });
+    }
+}
+
+/* Sum of a per-lane count into *dst, one atomic per wavefront. */
+__device__ __forceinline__ void add_count(unsigned long long *dst,
+                                          unsigned long long n)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63) == 0 && n)
+        atomicAdd(dst, n);
 }
 
 /* Full refresh of all NT rates after the post-time-step function

@@ -38,6 +38,7 @@ using namespace siminf;
 
 struct ModelSIR {
     static constexpr int NC = 3, NT = 2, ND = 0, NLD = 2, NGD = 0;
+    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
     static constexpr int S[NT][NC] =
         {{-1, 1, 0}, {0, -1, 1}};
     static constexpr int G[NT][NT] =
@@ -55,6 +56,7 @@ struct ModelSIR {
 
 struct ModelSEIR {
     static constexpr int NC = 4, NT = 3, ND = 0, NLD = 3, NGD = 0;
+    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
     static constexpr int S[NT][NC] =
         {{-1, 1, 0, 0}, {0, -1, 1, 0}, {0, 0, -1, 1}};
     static constexpr int G[NT][NT] =
@@ -73,6 +75,7 @@ struct ModelSEIR {
 
 struct ModelSIS {
     static constexpr int NC = 2, NT = 2, ND = 0, NLD = 2, NGD = 0;
+    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
     static constexpr int S[NT][NC] =
         {{-1, 1}, {1, -1}};
     static constexpr int G[NT][NT] =
@@ -130,6 +133,7 @@ struct ModelSISe3 {
 
 struct ModelSIRmparse {
     static constexpr int NC = 3, NT = 2, ND = 0, NLD = 0, NGD = 2;
+    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
     static constexpr int S[NT][NC] =
         {{-1, 1, 0}, {0, -1, 1}};
     static constexpr int G[NT][NT] =
@@ -230,7 +234,7 @@ struct SimInf_hip_plan {
     KParams p;
     DeviceBuffers buf;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_start = nullptr, ev_mid = nullptr, ev_stop = nullptr;
     unsigned grid = 0, block = 0;
     size_t lds = 0;
     bool has_e1 = false;
@@ -245,7 +249,10 @@ struct SimInf_hip_plan {
 
     /* Last run. */
     unsigned long long transitions = 0;
-    double kernel_ms = 0.0;
+    double kernel_ms = 0.0;     /* all kernels of the run */
+    double solver_ms = 0.0;     /* the solver kernel(s) */
+    double fill_ms = 0.0;       /* the frozen-column fill */
+    unsigned long long fill_cols = 0;
 };
 
 static const uint64_t NO_ERROR_KEY = ~0ull;
@@ -699,6 +706,12 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     } else {
         P->block = 256;
         P->grid = (unsigned)((lanes + 255) / 256);
+        if (a->U && (Nd == 0 || a->V)) {
+            p.frozen_from = B.alloc<int>((size_t)lanes);
+            p.frozen_u = B.alloc<int>((size_t)lanes * Nc);
+            p.frozen_v = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
+            p.frozen_cols = B.alloc<unsigned long long>(1);
+        }
     }
 
     if (B.status != hipSuccess) {
@@ -707,6 +720,7 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     }
     if (hipStreamCreate(&P->stream) != hipSuccess ||
         hipEventCreate(&P->ev_start) != hipSuccess ||
+        hipEventCreate(&P->ev_mid) != hipSuccess ||
         hipEventCreate(&P->ev_stop) != hipSuccess) {
         SimInf_hip_plan_destroy(P);
         return SIMINF_ERR_DEVICE;
@@ -726,7 +740,9 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
     if (hipMemcpyAsync(p.err_key, &init_key, sizeof(init_key),
                        hipMemcpyHostToDevice, P->stream) != hipSuccess ||
         hipMemcpyAsync(p.transitions, &zero, sizeof(zero),
-                       hipMemcpyHostToDevice, P->stream) != hipSuccess)
+                       hipMemcpyHostToDevice, P->stream) != hipSuccess ||
+        (p.frozen_cols && hipMemcpyAsync(p.frozen_cols, &zero, sizeof(zero),
+                                         hipMemcpyHostToDevice, P->stream) != hipSuccess))
         return SIMINF_ERR_DEVICE;
 
     (void)hipEventRecord(P->ev_start, P->stream);
@@ -752,21 +768,33 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
             return SIMINF_ERR_DEVICE;
         break;
     }
+    (void)hipEventRecord(P->ev_mid, P->stream);
+    if (p.frozen_from) {
+        hipLaunchKernelGGL(k_fill_frozen, dim3(P->grid), dim3(P->block), 0, P->stream,
+                           p, P->Nc, P->Nd);
+    }
     (void)hipEventRecord(P->ev_stop, P->stream);
     if (hipGetLastError() != hipSuccess)
         return SIMINF_ERR_DEVICE;
 
-    unsigned long long key = NO_ERROR_KEY, trans = 0;
+    unsigned long long key = NO_ERROR_KEY, trans = 0, fill_cols = 0;
     if (hipMemcpyAsync(&key, p.err_key, sizeof(key), hipMemcpyDeviceToHost,
                        P->stream) != hipSuccess ||
         hipMemcpyAsync(&trans, p.transitions, sizeof(trans),
                        hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
+        (p.frozen_cols && hipMemcpyAsync(&fill_cols, p.frozen_cols, sizeof(fill_cols),
+                                         hipMemcpyDeviceToHost, P->stream) != hipSuccess) ||
         hipStreamSynchronize(P->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
 
-    float ms = 0.0f;
+    float ms = 0.0f, ms_solver = 0.0f, ms_fill = 0.0f;
     (void)hipEventElapsedTime(&ms, P->ev_start, P->ev_stop);
+    (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_mid);
+    (void)hipEventElapsedTime(&ms_fill, P->ev_mid, P->ev_stop);
     P->kernel_ms = ms;
+    P->solver_ms = ms_solver;
+    P->fill_ms = ms_fill;
+    P->fill_cols = fill_cols;
     P->transitions = trans;
     return key_to_code(std::min<uint64_t>(key, P->host_err_key));
 }
@@ -806,6 +834,21 @@ extern "C" int SimInf_hip_plan_stats(const SimInf_hip_plan *P,
     return 0;
 }
 
+extern "C" int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *P,
+                                            double *solver_ms, double *fill_ms,
+                                            long long *fill_columns)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (solver_ms)
+        *solver_ms = P->solver_ms;
+    if (fill_ms)
+        *fill_ms = P->fill_ms;
+    if (fill_columns)
+        *fill_columns = (long long)P->fill_cols;
+    return 0;
+}
+
 extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
 {
     if (!P)
@@ -814,6 +857,8 @@ extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
         (void)hipStreamSynchronize(P->stream);
     if (P->ev_start)
         (void)hipEventDestroy(P->ev_start);
+    if (P->ev_mid)
+        (void)hipEventDestroy(P->ev_mid);
     if (P->ev_stop)
         (void)hipEventDestroy(P->ev_stop);
     if (P->stream)

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long fired = 0;
+    unsigned long long fired = 0, frozen_cols = 0;
 
     if (g < p.lanes) {
         const int rep = (int)(g / p.Nn);
@@ -133,6 +133,14 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                 bool upd = false;
                 int bad = 0;
 
+                if constexpr (pts_noop<M>::value) {
+                    if (p.frozen_from && s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end)) {
+                        record_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                        frozen_cols = (unsigned long long)(p.tlen - U_it);
+                        break;
+                    }
+                }
+
                 err = ssa_day<M>(s, cx, tt, next, fired);
                 if (err) {
                     record_error(p, rep, day, PH_SSA, node, err);
@@ -176,8 +184,12 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                 day++;
             }
         }
+        if (p.frozen_from && frozen_cols == 0)
+            p.frozen_from[g] = p.tlen;
     }
     add_transitions(p, fired);
+    if (p.frozen_cols)
+        add_count(p.frozen_cols, frozen_cols);
 }
 
 template <class M, bool EVENTS, bool CANON>
@@ -185,7 +197,7 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long fired = 0;
+    unsigned long long fired = 0, frozen_cols = 0;
 
     if (g < p.lanes) {
         const int node = (int)g;
@@ -237,6 +249,18 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
             for (;;) {
                 bool upd = false;
                 int bad = 0;
+
+                if constexpr (pts_noop<M>::value) {
+                    bool idle = p.frozen_from && (!EVENTS || ec.cur >= ec.end);
+                    static_for<NT>([&](auto J) {
+                        idle = idle && s.rate[decltype(J)::value] == 0.0;
+                    });
+                    if (idle) {
+                        record_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                        frozen_cols = (unsigned long long)(p.tlen - U_it);
+                        break;
+                    }
+                }
 
                 err = aem_day<M>(s, cx, next, chan_base, fired);
                 if (err) {
@@ -291,8 +315,12 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
                 day++;
             }
         }
+        if (p.frozen_from && frozen_cols == 0)
+            p.frozen_from[g] = p.tlen;
     }
     add_transitions(p, fired);
+    if (p.frozen_cols)
+        add_count(p.frozen_cols, frozen_cols);
 }
 
 /* ------------------------------------------------------------------ */
@@ -611,6 +639,31 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
             break;
     }
     add_transitions(p, fired);
+}
+
+/* The columns frozen nodes left unwritten (node kernels, dense U/V):
+ * one lane per (replicate, node), looping over the columns in order, so
+ * that at any time the workgroups write neighbouring stretches of one
+ * column, as a stream. */
+__global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int Nc, int Nd)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= p.lanes)
+        return;
+    const int rep = (int)(g / p.Nn);
+    const int node = (int)(g - (long long)rep * p.Nn);
+    const int from = p.frozen_from[g];
+    if (from >= p.tlen)
+        return;
+    const int *u = p.frozen_u + g * Nc;
+    const double *v = p.frozen_v + g * Nd;
+    for (int it = from; it < p.tlen; it++) {
+        const long long q = ((long long)rep * p.tlen + it) * p.Nn + node;
+        for (int c = 0; c < Nc; c++)
+            p.U[q * Nc + c] = u[c];
+        for (int c = 0; c < Nd; c++)
+            p.V[q * Nd + c] = v[c];
+    }
 }
 
 /* Layout version of SimInf_hip_model, checked when a plugin is used. */

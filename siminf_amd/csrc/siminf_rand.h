@@ -140,6 +140,29 @@ SIMINF_FN double siminf_uniform(siminf_stream *s)
     return siminf_u01(w[0], w[1]);
 }
 
+/* The next aligned pair of uniforms, 2b and 2b+1 with 2b the first even
+ * position at or after the stream's (a stream standing at an odd
+ * position drops the uniform there).  The SSA takes one pair per
+ * attempt -- the waiting time from the first, the transition from the
+ * second, which is dropped when the waiting time ends the day -- so that
+ * every attempt costs one Philox block and lanes of a wavefront never
+ * disagree about computing one. */
+SIMINF_FN void siminf_uniform2(siminf_stream *s, double *u0, double *u1)
+{
+    uint32_t w[4];
+    const uint64_t b = (s->j + 1u) >> 1;
+    if (2u * b + 1u < s->npre) {
+        *u0 = s->pre[2u * b];
+        *u1 = s->pre[2u * b + 1u];
+    } else {
+        siminf_philox4x32_10((uint32_t)b, (uint32_t)(b >> 32), s->id0, s->id1,
+                             s->key0, s->key1, w);
+        *u0 = siminf_u01(w[0], w[1]);
+        *u1 = siminf_u01(w[2], w[3]);
+    }
+    s->j = 2u * b + 2u;
+}
+
 /* Uniforms 2b and 2b+1 of a stream (one Philox block). */
 SIMINF_FN void siminf_uniform_pair(const siminf_stream *s, uint64_t b,
                                    double *u0, double *u1)

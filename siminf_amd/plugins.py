@@ -59,6 +59,8 @@ def device_source(parsed, name):
     lines += ["struct GeneratedModel {",
               "    static constexpr int NC = %d, NT = %d, ND = %d, NLD = %d, NGD = %d;"
               % (NC, NT, ND, NLD, NGD),
+              "    static constexpr bool PTS_NOOP = %s;" % ("true" if parsed.pts_fun is None
+                                                           else "false"),
               "    static constexpr int S[NT][NC] = %s;" % fmt(S),
               "    static constexpr int G[NT][NT] = %s;" % fmt(G),
               "    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)",
