@@ -155,6 +155,17 @@ struct KParams {
     double *frozen_v;
     unsigned long long *frozen_cols;
 
+    /* Node kernels run in one of three modes (NODES_*): every lane
+     * through the whole run; or a classify pass over every lane, which
+     * hands nodes frozen from the start to the fill kernel and lists the
+     * others in active (count in *active_count), then an active pass
+     * over the listed lanes, packed 1 << lpw_shift to a wavefront. */
+    int mode;
+    int lpw_shift;
+    int n_active;
+    int *active;
+    int *active_count;
+
     /* Results. */
     unsigned long long *err_key;
     unsigned long long *transitions;
@@ -1035,11 +1046,14 @@ __device__ __forceinline__ void store_sparse(const int *ir, const int *jc,
 
 template <int NC, int ND>
 __device__ __forceinline__ void store_columns(
-    const KParams &p, int rep, int node, double tt, int &U_it, int &V_it,
+    const KParams &p, int rep, int node, double tt, int &it, double &ts,
     const int (&u)[NC], const double (&vnew)[ND > 0 ? ND : 1])
 {
-    while (U_it < p.tlen && tt > p.tspan[U_it]) {
-        const long long col = (long long)rep * p.tlen + U_it;
+    /* U and V columns advance together (both while tt > tspan[it]); ts
+     * holds tspan[it], loaded when it advances, so that the load's
+     * latency overlaps the next day instead of opening this one. */
+    while (it < p.tlen && tt > ts) {
+        const long long col = (long long)rep * p.tlen + it;
         if (p.U) {
             int *dst = p.U + (col * p.Nn + node) * NC;
             static_for<NC>([&](auto C) {
@@ -1049,11 +1063,7 @@ __device__ __forceinline__ void store_columns(
         } else {
             store_sparse<NC>(p.irU, p.jcU, p.prU, col, node, u);
         }
-        U_it++;
-    }
-    while (V_it < p.tlen && tt > p.tspan[V_it]) {
         if constexpr (ND > 0) {
-            const long long col = (long long)rep * p.tlen + V_it;
             if (p.V) {
                 double *dst = p.V + (col * p.Nn + node) * ND;
                 static_for<ND>([&](auto C) {
@@ -1064,7 +1074,8 @@ __device__ __forceinline__ void store_columns(
                 store_sparse<ND>(p.irV, p.jcV, p.prV, col, node, vnew);
             }
         }
-        V_it++;
+        it++;
+        ts = it < p.tlen ? p.tspan[it] : INFINITY;
     }
 }
 
@@ -1095,6 +1106,60 @@ __device__ __forceinline__ void record_frozen(const KParams &p, long long g,
             constexpr int c = decltype(C)::value;
             p.frozen_v[g * ND + c] = vnew[c];
         });
+    }
+}
+
+enum { NODES_ALL = 0, NODES_CLASSIFY = 1, NODES_ACTIVE = 2 };
+
+/* The lane (replicate * Nn + node) a thread of a node kernel runs, or -1. */
+__device__ __forceinline__ long long node_lane(const KParams &p)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p.mode != NODES_ACTIVE)
+        return t < p.lanes ? t : -1;
+    const int l = (int)(t & 63);
+    if (l >= (1 << p.lpw_shift))
+        return -1;
+    const long long i = ((t >> 6) << p.lpw_shift) + l;
+    return i < p.n_active ? (long long)p.active[i] : -1;
+}
+
+/* Append lane g to the active list (one atomic per wavefront). */
+__device__ __forceinline__ void push_active(const KParams &p, long long g, bool push)
+{
+    const unsigned long long m = __ballot(push);
+    const int lane = (int)(threadIdx.x & 63);
+    int base = 0;
+    const int first = m ? __ffsll((long long)m) - 1 : 0;
+    if (m && lane == first)
+        base = atomicAdd(p.active_count, __popcll(m));
+    base = __shfl(base, first);
+    if (push)
+        p.active[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)g;
+}
+
+/* A frozen node of the active pass writes its remaining columns itself
+ * (the fill kernel has already been given the nodes frozen from the
+ * start). */
+template <int NC, int ND>
+__device__ __forceinline__ void store_frozen(const KParams &p, long long g, int U_it,
+                                             const int (&u)[NC],
+                                             const double (&vnew)[ND > 0 ? ND : 1])
+{
+    const int rep = (int)(g / p.Nn);
+    const int node = (int)(g - (long long)rep * p.Nn);
+    for (int it = U_it; it < p.tlen; it++) {
+        const long long q = ((long long)rep * p.tlen + it) * p.Nn + node;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            p.U[q * NC + c] = u[c];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                p.V[q * ND + c] = vnew[c];
+            });
+        }
     }
 }
 

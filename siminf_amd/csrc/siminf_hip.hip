@@ -234,6 +234,10 @@ struct SimInf_hip_plan {
     KParams p;
     DeviceBuffers buf;
     hipStream_t stream = nullptr;
+    hipStream_t fill_stream = nullptr;   /* the fill kernel, beside the active pass */
+    hipEvent_t ev_classified = nullptr, ev_fill0 = nullptr, ev_fill1 = nullptr,
+               ev_active1 = nullptr;
+    int simds = 1024;                     /* SIMDs of the device (4 per CU) */
     hipEvent_t ev_start = nullptr, ev_mid = nullptr, ev_stop = nullptr;
     unsigned grid = 0, block = 0;
     size_t lds = 0;
@@ -711,6 +715,8 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
             p.frozen_u = B.alloc<int>((size_t)lanes * Nc);
             p.frozen_v = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
             p.frozen_cols = B.alloc<unsigned long long>(1);
+            p.active = B.alloc<int>((size_t)lanes);
+            p.active_count = B.alloc<int>(1);
         }
     }
 
@@ -718,7 +724,19 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
         delete P;
         return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
     }
+    {
+        int dev = 0, cus = 256;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                hipSuccess && cus > 0)
+            P->simds = 4 * cus;
+    }
     if (hipStreamCreate(&P->stream) != hipSuccess ||
+        hipStreamCreate(&P->fill_stream) != hipSuccess ||
+        hipEventCreate(&P->ev_classified) != hipSuccess ||
+        hipEventCreate(&P->ev_fill0) != hipSuccess ||
+        hipEventCreate(&P->ev_fill1) != hipSuccess ||
+        hipEventCreate(&P->ev_active1) != hipSuccess ||
         hipEventCreate(&P->ev_start) != hipSuccess ||
         hipEventCreate(&P->ev_mid) != hipSuccess ||
         hipEventCreate(&P->ev_stop) != hipSuccess) {
@@ -746,13 +764,65 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
         return SIMINF_ERR_DEVICE;
 
     (void)hipEventRecord(P->ev_start, P->stream);
+    bool filled = false;
     switch (P->kernel) {
     case K_SSM_NODES:
-        P->model->ssm_nodes(p, P->has_e1, P->canon, P->grid, P->block, P->stream);
+    case K_AEM_NODES: {
+        auto launch = [&](const KParams &q, unsigned grid) {
+            if (P->kernel == K_SSM_NODES)
+                P->model->ssm_nodes(q, P->has_e1, P->canon, grid, P->block, P->stream);
+            else
+                P->model->aem_nodes(q, P->has_e1, P->canon, grid, P->block, P->stream);
+        };
+        if (!p.frozen_from) {
+            p.mode = NODES_ALL;
+            launch(p, P->grid);
+            (void)hipEventRecord(P->ev_mid, P->stream);
+            break;
+        }
+        /* Classify every lane; nodes frozen from the start go to the fill
+         * kernel on the second stream while the active pass runs the
+         * others, packed so that they spread over the SIMDs. */
+        const int zero_i = 0;
+        if (hipMemcpyAsync(p.active_count, &zero_i, sizeof(int), hipMemcpyHostToDevice,
+                           P->stream) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        p.mode = NODES_CLASSIFY;
+        launch(p, P->grid);
+        int n_active = 0;
+        if (hipMemcpyAsync(&n_active, p.active_count, sizeof(int), hipMemcpyDeviceToHost,
+                           P->stream) != hipSuccess ||
+            hipEventRecord(P->ev_classified, P->stream) != hipSuccess ||
+            hipStreamWaitEvent(P->fill_stream, P->ev_classified, 0) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        (void)hipEventRecord(P->ev_fill0, P->fill_stream);
+        hipLaunchKernelGGL(k_fill_frozen, dim3(P->grid), dim3(P->block), 0,
+                           P->fill_stream, p, P->Nc, P->Nd);
+        (void)hipEventRecord(P->ev_fill1, P->fill_stream);
+        if (hipStreamSynchronize(P->stream) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        if (n_active > 0) {
+            /* Lanes per wavefront: the power of two that gives about two
+             * wavefronts per SIMD, at most 64. */
+            int shift = 0;
+            while (shift < 6 && ((long long)n_active >> shift) > 2LL * P->simds)
+                shift++;
+            p.mode = NODES_ACTIVE;
+            p.n_active = n_active;
+            p.lpw_shift = shift;
+            const long long waves = ((long long)n_active + (1 << shift) - 1) >> shift;
+            const unsigned grid = (unsigned)((waves * 64 + P->block - 1) / P->block);
+            (void)hipEventRecord(P->ev_mid, P->stream);
+            launch(p, grid);
+        } else {
+            (void)hipEventRecord(P->ev_mid, P->stream);
+        }
+        (void)hipEventRecord(P->ev_active1, P->stream);
+        if (hipStreamWaitEvent(P->stream, P->ev_fill1, 0) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        filled = true;
         break;
-    case K_AEM_NODES:
-        P->model->aem_nodes(p, P->has_e1, P->canon, P->grid, P->block, P->stream);
-        break;
+    }
     case K_COUPLED_GRID:
         /* One cooperative launch per replicate. */
         for (int r = 0; r < P->Nrep; r++) {
@@ -768,11 +838,8 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
             return SIMINF_ERR_DEVICE;
         break;
     }
-    (void)hipEventRecord(P->ev_mid, P->stream);
-    if (p.frozen_from) {
-        hipLaunchKernelGGL(k_fill_frozen, dim3(P->grid), dim3(P->block), 0, P->stream,
-                           p, P->Nc, P->Nd);
-    }
+    if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES)
+        (void)hipEventRecord(P->ev_mid, P->stream);
     (void)hipEventRecord(P->ev_stop, P->stream);
     if (hipGetLastError() != hipSuccess)
         return SIMINF_ERR_DEVICE;
@@ -787,10 +854,18 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
         hipStreamSynchronize(P->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
 
-    float ms = 0.0f, ms_solver = 0.0f, ms_fill = 0.0f;
+    /* Device times: the whole run; the solver (classify and active
+     * passes, or the one kernel); the fill kernel on its own stream. */
+    float ms = 0.0f, ms_solver = 0.0f, ms_active = 0.0f, ms_fill = 0.0f;
     (void)hipEventElapsedTime(&ms, P->ev_start, P->ev_stop);
-    (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_mid);
-    (void)hipEventElapsedTime(&ms_fill, P->ev_mid, P->ev_stop);
+    if (filled) {
+        (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_classified);
+        (void)hipEventElapsedTime(&ms_active, P->ev_mid, P->ev_active1);
+        ms_solver += ms_active;
+        (void)hipEventElapsedTime(&ms_fill, P->ev_fill0, P->ev_fill1);
+    } else {
+        (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_mid);
+    }
     P->kernel_ms = ms;
     P->solver_ms = ms_solver;
     P->fill_ms = ms_fill;
@@ -859,6 +934,16 @@ extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
         (void)hipEventDestroy(P->ev_start);
     if (P->ev_mid)
         (void)hipEventDestroy(P->ev_mid);
+    if (P->ev_classified)
+        (void)hipEventDestroy(P->ev_classified);
+    if (P->ev_fill0)
+        (void)hipEventDestroy(P->ev_fill0);
+    if (P->ev_fill1)
+        (void)hipEventDestroy(P->ev_fill1);
+    if (P->ev_active1)
+        (void)hipEventDestroy(P->ev_active1);
+    if (P->fill_stream)
+        (void)hipStreamDestroy(P->fill_stream);
     if (P->ev_stop)
         (void)hipEventDestroy(P->ev_stop);
     if (P->stream)

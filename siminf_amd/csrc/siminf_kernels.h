@@ -85,10 +85,11 @@ template <class M, bool EVENTS, bool CANON>
 __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long g = node_lane(p);
     unsigned long long fired = 0, frozen_cols = 0;
+    bool push = false;
 
-    if (g < p.lanes) {
+    if (g >= 0) {
         const int rep = (int)(g / p.Nn);
         const int node = (int)(g - (long long)rep * p.Nn);
         Ctx<M, CANON> cx;
@@ -121,24 +122,37 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                 err = SIMINF_ERR_INVALID_RATE;
         });
 
+        const EvTab tab = global_tables(p);
+        E1Cursor ec;
+        if constexpr (EVENTS)
+            ec.start(p, node);
+        /* Can the node's state no longer change? */
+        auto frozen = [&]() -> bool {
+            if constexpr (pts_noop<M>::value)
+                return s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end);
+            else
+                return false;
+        };
+
         if (err) {
             record_error(p, rep, 0, PH_INIT, node, err);
+        } else if (p.mode == NODES_CLASSIFY) {
+            if (frozen()) {
+                record_frozen<NC, ND>(p, g, 0, s.u, s.vnew);
+                frozen_cols = (unsigned long long)p.tlen;
+            } else {
+                push = true;
+            }
         } else {
-            const EvTab tab = global_tables(p);
-            E1Cursor ec;
-            if constexpr (EVENTS)
-                ec.start(p, node);
-            int U_it = 0, V_it = 0, day = 0;
+            int U_it = 0, day = 0;
+            double ts = p.tspan[0];
             for (;;) {
                 bool upd = false;
                 int bad = 0;
 
-                if constexpr (pts_noop<M>::value) {
-                    if (p.frozen_from && s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end)) {
-                        record_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
-                        frozen_cols = (unsigned long long)(p.tlen - U_it);
-                        break;
-                    }
+                if (p.mode == NODES_ACTIVE && frozen()) {
+                    store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                    break;
                 }
 
                 err = ssa_day<M>(s, cx, tt, next, fired);
@@ -170,7 +184,7 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
 
                 tt = next;
                 next += 1.0;
-                store_columns<NC, ND>(p, rep, node, tt, U_it, V_it, s.u, s.vnew);
+                store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
                 if constexpr (ND > 0) {
                     static_for<ND>([&](auto C) {
                         constexpr int c = decltype(C)::value;
@@ -184,22 +198,25 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                 day++;
             }
         }
-        if (p.frozen_from && frozen_cols == 0)
+        if (p.mode == NODES_CLASSIFY && frozen_cols == 0)
             p.frozen_from[g] = p.tlen;
     }
-    add_transitions(p, fired);
-    if (p.frozen_cols)
+    if (p.mode == NODES_CLASSIFY) {
+        push_active(p, g, push);
         add_count(p.frozen_cols, frozen_cols);
+    }
+    add_transitions(p, fired);
 }
 
 template <class M, bool EVENTS, bool CANON>
 __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long g = node_lane(p);
     unsigned long long fired = 0, frozen_cols = 0;
+    bool push = false;
 
-    if (g < p.lanes) {
+    if (g >= 0) {
         const int node = (int)g;
         Ctx<M, CANON> cx;
         cx.load(p, node);
@@ -238,28 +255,42 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
             s.j[j] = 1;
         });
 
+        const EvTab tab = global_tables(p);
+        E1Cursor ec;
+        if constexpr (EVENTS)
+            ec.start(p, node);
+        /* Can the node's state no longer change (every rate zero)? */
+        auto frozen = [&]() -> bool {
+            if constexpr (pts_noop<M>::value) {
+                bool idle = !EVENTS || ec.cur >= ec.end;
+                static_for<NT>([&](auto J) {
+                    idle = idle && s.rate[decltype(J)::value] == 0.0;
+                });
+                return idle;
+            } else {
+                return false;
+            }
+        };
+
         if (err) {
             record_error(p, 0, 0, PH_INIT, node, err);
+        } else if (p.mode == NODES_CLASSIFY) {
+            if (frozen()) {
+                record_frozen<NC, ND>(p, g, 0, s.u, s.vnew);
+                frozen_cols = (unsigned long long)p.tlen;
+            } else {
+                push = true;
+            }
         } else {
-            const EvTab tab = global_tables(p);
-            E1Cursor ec;
-            if constexpr (EVENTS)
-                ec.start(p, node);
-            int U_it = 0, V_it = 0, day = 0;
+            int U_it = 0, day = 0;
+            double ts = p.tspan[0];
             for (;;) {
                 bool upd = false;
                 int bad = 0;
 
-                if constexpr (pts_noop<M>::value) {
-                    bool idle = p.frozen_from && (!EVENTS || ec.cur >= ec.end);
-                    static_for<NT>([&](auto J) {
-                        idle = idle && s.rate[decltype(J)::value] == 0.0;
-                    });
-                    if (idle) {
-                        record_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
-                        frozen_cols = (unsigned long long)(p.tlen - U_it);
-                        break;
-                    }
+                if (p.mode == NODES_ACTIVE && frozen()) {
+                    store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                    break;
                 }
 
                 err = aem_day<M>(s, cx, next, chan_base, fired);
@@ -301,7 +332,7 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
 
                 tt = next;
                 next += 1.0;
-                store_columns<NC, ND>(p, 0, node, tt, U_it, V_it, s.u, s.vnew);
+                store_columns<NC, ND>(p, 0, node, tt, U_it, ts, s.u, s.vnew);
                 if constexpr (ND > 0) {
                     static_for<ND>([&](auto C) {
                         constexpr int c = decltype(C)::value;
@@ -315,12 +346,14 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
                 day++;
             }
         }
-        if (p.frozen_from && frozen_cols == 0)
+        if (p.mode == NODES_CLASSIFY && frozen_cols == 0)
             p.frozen_from[g] = p.tlen;
     }
-    add_transitions(p, fired);
-    if (p.frozen_cols)
+    if (p.mode == NODES_CLASSIFY) {
+        push_active(p, g, push);
         add_count(p.frozen_cols, frozen_cols);
+    }
+    add_transitions(p, fired);
 }
 
 /* ------------------------------------------------------------------ */
@@ -483,7 +516,8 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     int *updw = p.upd_work + base;
     unsigned long long fired = 0;
     double tt = p.tt0, next = p.next0;
-    int U_it = 0, V_it = 0;
+    int U_it = 0;
+    double ts = p.tspan[0];
     int myerr = 0;
 
     NodeState<M> s;
@@ -622,7 +656,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         tt = next;
         next += 1.0;
         if (active) {
-            store_columns<NC, ND>(p, rep, node, tt, U_it, V_it, s.u, s.vnew);
+            store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
             if constexpr (ND > 0) {
                 static_for<ND>([&](auto C) {
                     constexpr int c = decltype(C)::value;
@@ -632,8 +666,10 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 });
             }
         } else {
-            while (U_it < p.tlen && tt > p.tspan[U_it])
+            while (U_it < p.tlen && tt > ts) {
                 U_it++;
+                ts = U_it < p.tlen ? p.tspan[U_it] : INFINITY;
+            }
         }
         if (U_it >= p.tlen)
             break;
