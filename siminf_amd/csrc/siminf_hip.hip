@@ -796,8 +796,28 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
             hipStreamWaitEvent(P->fill_stream, P->ev_classified, 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
         (void)hipEventRecord(P->ev_fill0, P->fill_stream);
-        hipLaunchKernelGGL(k_fill_frozen, dim3(P->grid), dim3(P->block), 0,
-                           P->fill_stream, p, P->Nc, P->Nd);
+        {
+            const dim3 fg(P->grid), fb(P->block);
+            hipStream_t fs = P->fill_stream;
+            const bool v = P->Nd > 0;
+#define SIMINF_FILL(NCV)                                                          \
+    do {                                                                          \
+        if (v)                                                                    \
+            hipLaunchKernelGGL((k_fill_frozen<NCV, true>), fg, fb, 0, fs, p, P->Nc, \
+                               P->Nd);                                            \
+        else                                                                      \
+            hipLaunchKernelGGL((k_fill_frozen<NCV, false>), fg, fb, 0, fs, p,      \
+                               P->Nc, P->Nd);                                     \
+    } while (0)
+            switch (P->Nc) {
+            case 2: SIMINF_FILL(2); break;
+            case 3: SIMINF_FILL(3); break;
+            case 4: SIMINF_FILL(4); break;
+            case 6: SIMINF_FILL(6); break;
+            default: SIMINF_FILL(0); break;
+            }
+#undef SIMINF_FILL
+        }
         (void)hipEventRecord(P->ev_fill1, P->fill_stream);
         if (hipStreamSynchronize(P->stream) != hipSuccess)
             return SIMINF_ERR_DEVICE;

@@ -680,8 +680,12 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 /* The columns frozen nodes left unwritten (node kernels, dense U/V):
  * one lane per (replicate, node), looping over the columns in order, so
  * that at any time the workgroups write neighbouring stretches of one
- * column, as a stream. */
-__global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int Nc, int Nd)
+ * column, as a stream.  NC > 0 gives compile-time compartment counts
+ * (the stores merge into wide ones); NC == 0 reads the count from nc.
+ * Without V (HASV false) the loop holds stores only, so none of them
+ * waits for another. */
+template <int NC, bool HASV>
+__global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int nc, int nd)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= p.lanes)
@@ -691,14 +695,30 @@ __global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int Nc, int Nd)
     const int from = p.frozen_from[g];
     if (from >= p.tlen)
         return;
-    const int *u = p.frozen_u + g * Nc;
-    const double *v = p.frozen_v + g * Nd;
-    for (int it = from; it < p.tlen; it++) {
-        const long long q = ((long long)rep * p.tlen + it) * p.Nn + node;
-        for (int c = 0; c < Nc; c++)
-            p.U[q * Nc + c] = u[c];
-        for (int c = 0; c < Nd; c++)
-            p.V[q * Nd + c] = v[c];
+    const int Nc = NC > 0 ? NC : nc;
+    int u[NC > 0 ? NC : 1];
+    if constexpr (NC > 0) {
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            u[c] = p.frozen_u[g * NC + c];
+        });
+    }
+    const long long stride = (long long)p.Nn;
+    long long q = ((long long)rep * p.tlen + from) * stride + node;
+    for (int it = from; it < p.tlen; it++, q += stride) {
+        if constexpr (NC > 0) {
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                p.U[q * NC + c] = u[c];
+            });
+        } else {
+            for (int c = 0; c < Nc; c++)
+                p.U[q * Nc + c] = p.frozen_u[g * Nc + c];
+        }
+        if constexpr (HASV) {
+            for (int c = 0; c < nd; c++)
+                p.V[q * nd + c] = p.frozen_v[g * nd + c];
+        }
     }
 }
 
