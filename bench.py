@@ -39,6 +39,26 @@ def algorithmic_bytes(model):
             + 36 * len(ev) + 2 * (2 * 4 * Nc) * len_e2)
 
 
+def pmc_traffic(workload, dominant):
+    """HBM bytes per launch of the dominant kernel from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/pmc_traffic.json
+    (each counter scaled by its reading for a copy kernel of known size);
+    None when no run was recorded for this workload and kernel."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(workload)
+    if entry is None:
+        return None
+    if dominant == "k_fill_frozen":
+        entry = entry.get("fill_kernel")
+        if entry is None:
+            return None
+    return entry.get("traffic_bytes")
+
+
 def make_workload(name):
     from siminf_amd import workloads as W
     if name == "sise3_herds":
@@ -170,6 +190,7 @@ def main():
         dominant, kernel_s = "solver", solver_ms / 1e3
         bytes_per_launch = total_bytes - fill_bytes
     achieved = bytes_per_launch / kernel_s / 1e9
+    traffic = pmc_traffic(args.workload, dominant)
     out = {
         "metric": "node-days/sec (SSA transitions/sec and p50/p99 simulation latency alongside)",
         "value": node_days / wall,
@@ -193,7 +214,7 @@ def main():
                             "fill_node_columns": fill_cols},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": dominant,
+                     "traffic": traffic, "kernel": dominant,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if not args.no_cpu_baseline:
