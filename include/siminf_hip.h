@@ -66,7 +66,12 @@ typedef enum {
     SIMINF_ERR_INVALID_CRN          = -23,
     SIMINF_ERR_INVALID_RNG_TYPE     = -24,
     /* Not in the reference: the GPU runtime failed (allocation, launch). */
-    SIMINF_ERR_DEVICE               = -100
+    SIMINF_ERR_DEVICE               = -100,
+    /* Particle filter (no code in the reference's C API: R-level errors
+     * of pfilter.R:455-458 and SimInf_particle_filter.c:42-48). */
+    SIMINF_ERR_INVALID_OBS_PROCESS  = -101,
+    SIMINF_ERR_INVALID_WEIGHT       = -102,
+    SIMINF_ERR_WEIGHT_SUM           = -103
 } SimInf_error_code;
 
 /* Plugin signatures (SimInf.h:92-107). */
@@ -210,6 +215,29 @@ int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *plan, double *solver_ms,
                                  double *fill_ms, long long *fill_columns);
 
 void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
+
+/* Bootstrap particle filter with the state of every particle kept in HBM
+ * between observation intervals (pfilter_internal, R/pfilter.R:315-499;
+ * systematic resampling, SimInf_particle_filter.c:32-66).
+ *
+ * args describes one particle (Nrep 1, no scheduled events); the filter
+ * runs n_particles replicates of it.  Interval i runs from tspan2[2i] to
+ * tspan2[2i+1], or over the one time point tspan2[2i+1] when tspan2[2i]
+ * is NaN (pfilter_tspan, R/pfilter.R:177-193).  The observation process
+ * is binomial: the observed count of compartment obs_compartment
+ * (zero-based) with detection probability obs_prob, either per node
+ * (obs_nodes == Nn) or of the sum over the nodes (obs_nodes == 1); obs is
+ * n_intervals x obs_nodes, and a negative count is missing.  The weight of
+ * a particle is the sum of the log-probabilities of its observations.
+ * Outputs: loglik (the log-likelihood estimate), ess (n_intervals
+ * effective sample sizes), ancestors (n_intervals x n_particles
+ * zero-based indices, or NULL).  Returns 0 or an error code. */
+int SimInf_hip_pfilter(const SimInf_solver_args *args,
+                       const SimInf_hip_model *model, int n_particles,
+                       int n_intervals, const double *tspan2,
+                       int obs_compartment, double obs_prob, int obs_nodes,
+                       const int *obs, unsigned long int seed, double *loglik,
+                       double *ess, int *ancestors);
 
 #ifdef __cplusplus
 }

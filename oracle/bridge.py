@@ -48,6 +48,14 @@ def load():
         lib.oracle_prefetch_mismatches.restype = ctypes.c_int
         lib.oracle_prefetch_mismatches.argtypes = [ctypes.c_ulonglong, ctypes.c_int,
                                                    ctypes.c_int]
+        lib.oracle_pfilter.restype = ctypes.c_int
+        lib.oracle_pfilter.argtypes = [
+            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double,
+            ctypes.c_int, ip, ctypes.c_ulonglong, dp, dp, ip]
+        lib.oracle_exp.argtypes = [dp, dp, ctypes.c_longlong]
+        lib.oracle_lfact.argtypes = [ip, dp, ctypes.c_longlong]
+        lib.oracle_log_dbinom.argtypes = [ip, ip, ctypes.c_double, dp, ctypes.c_longlong]
+        lib.oracle_resample.restype = ctypes.c_int
         lib.oracle_sample.restype = ctypes.c_int
         lib.oracle_sample.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ip,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -67,11 +75,8 @@ class OracleResult:
         self.aem_ties = ties
 
 
-def run(model, solver="ssm", seed=0, crn=False):
-    """Run the checker on the same SimInf_solver_args the GPU path gets."""
-    lib = load()
-    b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
-                           crn=1 if crn else 0, seed=int(seed))
+def _set_plugin(lib, model, b):
+    """tr_fun / pts_fun of the model's host plugin into b.args."""
     if isinstance(model.plugin, str):
         if lib.oracle_set_model(ctypes.byref(b.args), model.plugin.encode()):
             raise ValueError("no host plugin for %s" % model.plugin)
@@ -84,8 +89,74 @@ def run(model, solver="ssm", seed=0, crn=False):
         h = _host_plugins[so]
         b.args.tr_fun = ctypes.cast(h.siminf_tr_fun, ctypes.c_void_p)
         b.args.pts_fun = ctypes.c_void_p.in_dll(h, "siminf_pts_fun")
+
+
+def run(model, solver="ssm", seed=0, crn=False):
+    """Run the checker on the same SimInf_solver_args the GPU path gets."""
+    lib = load()
+    b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
+                           crn=1 if crn else 0, seed=int(seed))
+    _set_plugin(lib, model, b)
     err = lib.oracle_run(ctypes.byref(b.args))
     return OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
+
+
+def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5, seed=0):
+    """The checker's filter on the same inputs as siminf_amd.pfilter;
+    returns (error code, loglik, ess, ancestors)."""
+    from siminf_amd import particle_filter as pf
+    lib = load()
+    tspan2, obs, obs_nodes = pf.prepare(model, obs_compartment, times, obs)
+    b = _native.ArgsBundle(model, solver=0, seed=int(seed))
+    _set_plugin(lib, model, b)
+    n_int = len(tspan2)
+    loglik = ctypes.c_double()
+    ess = np.zeros(n_int)
+    anc = np.zeros((n_int, int(n_particles)), dtype=np.int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    err = lib.oracle_pfilter(ctypes.byref(b.args), int(n_particles), n_int,
+                             tspan2.ctypes.data_as(dp), int(obs_compartment),
+                             float(obs_prob), obs_nodes, obs.ctypes.data_as(ip),
+                             ctypes.c_ulonglong(int(seed)), ctypes.byref(loglik),
+                             ess.ctypes.data_as(dp), anc.ctypes.data_as(ip))
+    return err, loglik.value, ess, anc
+
+
+def _arr(fn, x, *extra, dtype=np.float64, out_dtype=np.float64):
+    x = np.ascontiguousarray(x, dtype=dtype)
+    y = np.empty(x.shape, dtype=out_dtype)
+    ptype = ctypes.POINTER(ctypes.c_double if dtype == np.float64 else ctypes.c_int)
+    fn(x.ctypes.data_as(ptype), *extra, y.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+       x.size)
+    return y
+
+
+def exp(x):
+    return _arr(load().oracle_exp, x)
+
+
+def lfact(n):
+    return _arr(load().oracle_lfact, n, dtype=np.int32)
+
+
+def log_dbinom(x, n, p):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    y = np.empty(x.shape)
+    ip = ctypes.POINTER(ctypes.c_int)
+    load().oracle_log_dbinom(x.ctypes.data_as(ip), n.ctypes.data_as(ip), ctypes.c_double(p),
+                             y.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), x.size)
+    return y
+
+
+def resample(w, u01):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    idx = np.zeros(len(w), dtype=np.int32)
+    err = load().oracle_resample(w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 len(w), ctypes.c_double(u01),
+                                 idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return err, idx
 
 
 def uniforms(seed, domain, sid, count):

@@ -14,6 +14,8 @@
  *   src/misc/binheap.c:15-105                    (indexed binary heap)
  *   src/solvers/SimInf_solver.c:62-324, :658-950 (sampling, events,
  *                                                 sparse output)
+ *   R/pfilter.R:315-499, src/misc/SimInf_particle_filter.c:32-66
+ *                                                (bootstrap particle filter)
  * with one thread.  Random numbers come from the injected streams of
  * siminf_rand.h instead of GSL:
  *   - SSA draws and E1-event draws of (replicate r, node i) use the node
@@ -37,6 +39,7 @@
 #include "../include/siminf_hip.h"
 #include "../siminf_amd/csrc/siminf_rand.h"
 #include "../siminf_amd/csrc/siminf_models.h"
+#include "../siminf_amd/csrc/siminf_pfilter.h"
 
 #include <math.h>
 #include <stddef.h>
@@ -1089,4 +1092,157 @@ int oracle_sample(int enter, const int *irE, const int *jcE, const double *prE,
             return err;
     }
     return 0;
+}
+
+/* ---------------------------------------------------------------- */
+/* Bootstrap particle filter (R/pfilter.R:315-499).                   */
+/* ---------------------------------------------------------------- */
+
+/* The device filter's computation on the host: per interval one mssm run
+ * of all particles from their ancestors' states (pfilter.R:385-430), the
+ * weights (the sum of log dbinom over the observed nodes, added in the
+ * device's order: node k into partial sum k mod 64, then the partial sums
+ * in order), log-sum-exp, ESS and systematic resampling (pfilter.R:455-
+ * 474).  a describes one particle, with tr_fun/pts_fun set.  Arguments as
+ * SimInf_hip_pfilter. */
+int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
+                   const double *tspan2, int obs_c, double obs_p, int obs_nodes,
+                   const int *obs, unsigned long long seed, double *loglik,
+                   double *ess, int *ancestors)
+{
+    const int Nn = a->Nn, Nc = a->Nc, Nd = a->Nd;
+    const size_t lanes = (size_t)Nn * P;
+    int *u_cur = malloc(lanes * Nc * sizeof(int));
+    double *v_cur = malloc(lanes * (Nd > 0 ? Nd : 1) * sizeof(double));
+    int *U = malloc(lanes * Nc * 2 * sizeof(int));
+    double *V = malloc(lanes * (Nd > 0 ? Nd : 1) * 2 * sizeof(double));
+    double *w = malloc((size_t)P * sizeof(double));
+    int *anc = malloc((size_t)P * sizeof(int));
+    int err = 0;
+    *loglik = 0.0;
+    if (!u_cur || !v_cur || !U || !V || !w || !anc) {
+        err = SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+        goto done;
+    }
+    for (int p = 0; p < P; p++) {
+        memcpy(&u_cur[(size_t)p * Nn * Nc], a->u0, (size_t)Nn * Nc * sizeof(int));
+        if (Nd > 0)
+            memcpy(&v_cur[(size_t)p * Nn * Nd], a->v0, (size_t)Nn * Nd * sizeof(double));
+    }
+    for (int i = 0; i < n_int && !err; i++) {
+        SimInf_solver_args b = *a;
+        const double tb = tspan2[2 * i];
+        const int tlen = isnan(tb) ? 1 : 2;
+        b.Nrep = P;
+        b.solver = 0;
+        b.tspan = isnan(tb) ? &tspan2[2 * i + 1] : &tspan2[2 * i];
+        b.tlen = tlen;
+        b.u0 = u_cur;
+        b.v0 = v_cur;
+        b.U = U;
+        b.V = V;
+        b.seed = siminf_interval_seed(seed, i);
+        err = oracle_run(&b);
+        if (err)
+            break;
+        for (int p = 0; p < P; p++) {
+            const size_t col = ((size_t)p * tlen + (tlen - 1)) * Nn;
+            if (obs_nodes == 1) {
+                long long tot = 0;
+                for (int k = 0; k < Nn; k++)
+                    tot += U[(col + k) * Nc + obs_c];
+                w[p] = obs[(size_t)i] < 0 ? 0.0
+                                          : siminf_log_dbinom(obs[(size_t)i], (int)tot, obs_p);
+            } else {
+                double partial[64], sum = 0.0;
+                for (int l = 0; l < 64; l++)
+                    partial[l] = 0.0;
+                for (int k = 0; k < Nn; k++) {
+                    const int x = obs[(size_t)i * Nn + k];
+                    if (x >= 0)
+                        partial[k % 64] = partial[k % 64] +
+                            siminf_log_dbinom(x, U[(col + k) * Nc + obs_c], obs_p);
+                }
+                for (int l = 0; l < 64; l++)
+                    sum = sum + partial[l];
+                w[p] = sum;
+            }
+        }
+        {
+            double max_w = -INFINITY, sum = 0.0, s2 = 0.0;
+            for (int p = 0; p < P; p++) {
+                if (!(w[p] > -INFINITY && w[p] < INFINITY)) {
+                    err = SIMINF_ERR_INVALID_OBS_PROCESS;
+                    break;
+                }
+                max_w = w[p] > max_w ? w[p] : max_w;
+            }
+            if (err)
+                break;
+            for (int p = 0; p < P; p++)
+                w[p] = siminf_exp(w[p] - max_w);
+            for (int p = 0; p < P; p++)
+                sum = sum + w[p];
+            *loglik = ((*loglik + max_w) + siminf_log(sum)) - siminf_log((double)P);
+            for (int p = 0; p < P; p++)
+                w[p] = w[p] / sum;
+            for (int p = 0; p < P; p++)
+                s2 = s2 + w[p] * w[p];
+            ess[i] = 1.0 / s2;
+            {
+                const siminf_stream st = siminf_stream_make(seed, SIMINF_DOMAIN_PF,
+                                                            (uint64_t)i);
+                const int r = siminf_systematic_resampling(w, P, siminf_uniform_at(&st, 0),
+                                                           anc);
+                if (r) {
+                    err = r == -1 ? SIMINF_ERR_INVALID_WEIGHT : SIMINF_ERR_WEIGHT_SUM;
+                    break;
+                }
+            }
+        }
+        if (ancestors)
+            memcpy(&ancestors[(size_t)i * P], anc, (size_t)P * sizeof(int));
+        for (int p = 0; p < P; p++) {
+            const size_t src = ((size_t)anc[p] * tlen + (tlen - 1)) * Nn;
+            memcpy(&u_cur[(size_t)p * Nn * Nc], &U[src * Nc], (size_t)Nn * Nc * sizeof(int));
+            if (Nd > 0)
+                memcpy(&v_cur[(size_t)p * Nn * Nd], &V[src * Nd],
+                       (size_t)Nn * Nd * sizeof(double));
+        }
+    }
+done:
+    free(u_cur);
+    free(v_cur);
+    free(U);
+    free(V);
+    free(w);
+    free(anc);
+    return err;
+}
+
+/* siminf_exp, siminf_lfact and siminf_log_dbinom over arrays, for the
+ * accuracy tests. */
+void oracle_exp(const double *x, double *y, long long n)
+{
+    for (long long i = 0; i < n; i++)
+        y[i] = siminf_exp(x[i]);
+}
+
+void oracle_lfact(const int *n, double *y, long long count)
+{
+    for (long long i = 0; i < count; i++)
+        y[i] = siminf_lfact(n[i]);
+}
+
+void oracle_log_dbinom(const int *x, const int *n, double p, double *y, long long count)
+{
+    for (long long i = 0; i < count; i++)
+        y[i] = siminf_log_dbinom(x[i], n[i], p);
+}
+
+/* Systematic resampling with a given uniform; returns the error code of
+ * siminf_systematic_resampling. */
+int oracle_resample(const double *w, int n, double u01, int *idx)
+{
+    return siminf_systematic_resampling(w, n, u01, idx);
 }

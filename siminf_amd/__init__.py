@@ -5,7 +5,9 @@ SimInf_solver_args C boundary (include/siminf_hip.h)."""
 from .model import SimInfModel, SimInfError, make_events, run, trajectory
 from .models import SIR, SEIR, SIS, SISe, SISe3, SIR_mparse
 from .modelparser import mparse
+from .particle_filter import pfilter, PFilterResult
 from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
-           "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse"]
+           "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse",
+           "pfilter", "PFilterResult"]

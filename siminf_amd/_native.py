@@ -82,6 +82,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_download",
     "SimInf_hip_plan_stats",
     "SimInf_hip_plan_kernel_times",
+    "SimInf_hip_pfilter",
     "SimInf_hip_plan_destroy",
 )
 
@@ -128,6 +129,12 @@ def load(path=None):
     lib.SimInf_hip_plan_kernel_times.argtypes = [
         vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
         ctypes.POINTER(ctypes.c_longlong)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.SimInf_hip_pfilter.restype = ctypes.c_int
+    lib.SimInf_hip_pfilter.argtypes = [
+        args_p, vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double,
+        ctypes.c_int, ip, ctypes.c_ulong, dp, dp, ip]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
     if path == LIB_PATH:

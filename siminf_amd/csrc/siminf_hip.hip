@@ -19,6 +19,7 @@
  *                   reference, SimInf_solver_ssm.c:244-250).
  */
 #include "siminf_kernels.h"
+#include "siminf_pfilter.h"
 
 #include <hip/hip_runtime.h>
 
@@ -348,9 +349,10 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
     return std::max<size_t>(off, 16);
 }
 
-extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
-                                      const SimInf_solver_args *a,
-                                      const SimInf_hip_model *model)
+/* With replicate_from_one, u0 and v0 hold one replicate (Nn columns)
+ * and are replicated on the device to a->Nrep replicates. */
+static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
+                            const SimInf_hip_model *model, bool replicate_from_one)
 {
     if (!out || !a || !model)
         return SIMINF_ERR_INVALID_MODEL;
@@ -609,8 +611,29 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     p.next0 = std::floor(tt0) + 1.0;
     p.ndays = ndays;
 
-    p.u0 = B.upload(a->u0, (size_t)lanes * Nc);
-    p.v0 = B.upload(a->v0, (size_t)lanes * Nd);
+    if (!replicate_from_one) {
+        p.u0 = B.upload(a->u0, (size_t)lanes * Nc);
+        p.v0 = B.upload(a->v0, (size_t)lanes * Nd);
+    } else {
+        /* Upload one replicate, then double the filled part. */
+        int *u0 = B.alloc<int>((size_t)lanes * Nc);
+        double *v0 = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
+        const size_t nu = (size_t)Nn * Nc, nv = (size_t)Nn * Nd;
+        if (B.status == hipSuccess)
+            B.status = hipMemcpy(u0, a->u0, nu * sizeof(int), hipMemcpyHostToDevice);
+        if (B.status == hipSuccess && nv)
+            B.status = hipMemcpy(v0, a->v0, nv * sizeof(double), hipMemcpyHostToDevice);
+        for (long long have = 1; have < Nrep && B.status == hipSuccess; have *= 2) {
+            const long long n = std::min<long long>(have, Nrep - have);
+            B.status = hipMemcpy(u0 + have * nu, u0, n * nu * sizeof(int),
+                                 hipMemcpyDeviceToDevice);
+            if (B.status == hipSuccess && nv)
+                B.status = hipMemcpy(v0 + have * nv, v0, n * nv * sizeof(double),
+                                     hipMemcpyDeviceToDevice);
+        }
+        p.u0 = u0;
+        p.v0 = v0;
+    }
     if (a->Nld > 0) {
         p.ldata = B.upload(a->ldata, (size_t)a->Nld * Nn);
     } else {
@@ -745,6 +768,13 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     }
     *out = P;
     return 0;
+}
+
+extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
+                                      const SimInf_solver_args *a,
+                                      const SimInf_hip_model *model)
+{
+    return plan_create_impl(out, a, model, false);
 }
 
 extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
@@ -1074,6 +1104,273 @@ extern "C" const char *SimInf_hip_error_message(int code)
     case SIMINF_ERR_INVALID_CRN: return "Invalid 'crn' value.";
     case SIMINF_ERR_INVALID_RNG_TYPE: return "Invalid 'rng_type' value.";
     case SIMINF_ERR_DEVICE: return "GPU runtime error.";
+    case SIMINF_ERR_INVALID_OBS_PROCESS: return "Invalid observation process.";
+    case SIMINF_ERR_INVALID_WEIGHT: return "Invalid weight detected (non-finite or < 0.0).";
+    case SIMINF_ERR_WEIGHT_SUM: return "Non-positive sum of weights detected.";
     default: return "Unknown error code.";
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* Bootstrap particle filter (R/pfilter.R:315-499).                     */
+/* ------------------------------------------------------------------ */
+
+/* Weight of each particle: the log-probability of the interval's
+ * observations given its state at the interval's end.  One workgroup of
+ * 64 lanes per particle; lane l adds the nodes l, l+64, ... in order and
+ * lane 0 adds the 64 partial sums in lane order, the order the checker
+ * uses.  With one observation for all nodes the count observed is the
+ * sum over the nodes (exact in any order). */
+__global__ __launch_bounds__(64) void k_pf_weights(const int *U, int Nn, int Nc,
+                                                   int tlen, int c_obs, double prob,
+                                                   const int *obs, int obs_nodes,
+                                                   double *w)
+{
+    __shared__ double partial[64];
+    __shared__ long long total[64];
+    const int part = (int)blockIdx.x, lane = (int)threadIdx.x;
+    const long long col = ((long long)part * tlen + (tlen - 1)) * Nn;
+    if (obs_nodes == 1) {
+        long long t = 0;
+        for (int node = lane; node < Nn; node += 64)
+            t += U[(col + node) * Nc + c_obs];
+        total[lane] = t;
+        __syncthreads();
+        if (lane == 0) {
+            long long sum = 0;
+            for (int l = 0; l < 64; l++)
+                sum += total[l];
+            w[part] = obs[0] < 0 ? 0.0
+                                 : siminf_log_dbinom(obs[0], (int)sum, prob);
+        }
+        return;
+    }
+    double acc = 0.0;
+    for (int node = lane; node < Nn; node += 64) {
+        const int x = obs[node];
+        if (x >= 0)
+            acc = acc + siminf_log_dbinom(x, U[(col + node) * Nc + c_obs], prob);
+    }
+    partial[lane] = acc;
+    __syncthreads();
+    if (lane == 0) {
+        double sum = 0.0;
+        for (int l = 0; l < 64; l++)
+            sum = sum + partial[l];
+        w[part] = sum;
+    }
+}
+
+/* Log-sum-exp of the weights, the log-likelihood increment, the ESS and
+ * systematic resampling (pfilter.R:455-474), in one workgroup.  The
+ * order-dependent sums run in particle order on one lane. */
+__global__ __launch_bounds__(256) void k_pf_resample(double *w, int n, int interval,
+                                                     unsigned long long seed,
+                                                     double *loglik, double *ess,
+                                                     int *anc, int *err)
+{
+    __shared__ double s_max[256];
+    __shared__ int s_bad[256];
+    __shared__ double s_sum;
+    const int tid = (int)threadIdx.x;
+    const double NEG_INF = -INFINITY;
+    double m = NEG_INF;
+    int bad = 0;
+    for (int i = tid; i < n; i += 256) {
+        const double x = w[i];
+        if (!(x > NEG_INF && x < INFINITY))
+            bad = 1;
+        m = x > m ? x : m;
+    }
+    s_max[tid] = m;
+    s_bad[tid] = bad;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            s_max[tid] = s_max[tid + h] > s_max[tid] ? s_max[tid + h] : s_max[tid];
+            s_bad[tid] |= s_bad[tid + h];
+        }
+        __syncthreads();
+    }
+    if (s_bad[0]) {
+        /* Keep every particle (the run reports the error). */
+        for (int i = tid; i < n; i += 256)
+            anc[i] = i;
+        if (tid == 0)
+            *err = SIMINF_ERR_INVALID_OBS_PROCESS;
+        return;
+    }
+    const double max_w = s_max[0];
+    for (int i = tid; i < n; i += 256)
+        w[i] = siminf_exp(w[i] - max_w);
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+        for (int i = 0; i < n; i++)
+            sum = sum + w[i];
+        *loglik = ((*loglik + max_w) + siminf_log(sum)) - siminf_log((double)n);
+        s_sum = sum;
+    }
+    __syncthreads();
+    const double sum = s_sum;
+    for (int i = tid; i < n; i += 256)
+        w[i] = w[i] / sum;
+    __syncthreads();
+    if (tid == 0) {
+        double s2 = 0.0;
+        for (int i = 0; i < n; i++)
+            s2 = s2 + w[i] * w[i];
+        ess[interval] = 1.0 / s2;
+        const siminf_stream st = siminf_stream_make(seed, SIMINF_DOMAIN_PF,
+                                                    (uint64_t)interval);
+        const int r = siminf_systematic_resampling(w, n, siminf_uniform_at(&st, 0),
+                                                   anc);
+        if (r) {
+            for (int i = 0; i < n; i++)
+                anc[i] = i;
+            *err = r == -1 ? SIMINF_ERR_INVALID_WEIGHT : SIMINF_ERR_WEIGHT_SUM;
+        }
+    }
+}
+
+/* Each particle continues from its ancestor's state at the interval's
+ * end (pfilter.R:398-410). */
+__global__ __launch_bounds__(256) void k_pf_gather(const int *U, const double *V,
+                                                   int Nn, int Nc, int Nd, int tlen,
+                                                   const int *anc, long long lanes,
+                                                   int *u_cur, double *v_cur)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= lanes)
+        return;
+    const long long part = g / Nn;
+    const long long node = g - part * Nn;
+    const long long src = ((long long)anc[part] * tlen + (tlen - 1)) * Nn + node;
+    for (int c = 0; c < Nc; c++)
+        u_cur[g * Nc + c] = U[src * Nc + c];
+    for (int c = 0; c < Nd; c++)
+        v_cur[g * Nd + c] = V[src * Nd + c];
+}
+
+extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
+                                  const SimInf_hip_model *model, int n_particles,
+                                  int n_intervals, const double *tspan2,
+                                  int obs_compartment, double obs_prob, int obs_nodes,
+                                  const int *obs, unsigned long int seed,
+                                  double *loglik, double *ess, int *ancestors)
+{
+    if (!a || !model || !tspan2 || !obs || !loglik || !ess || n_particles < 1 ||
+        n_intervals < 1 || a->Nrep > 1 || a->len != 0 || obs_compartment < 0 ||
+        obs_compartment >= a->Nc || (obs_nodes != 1 && obs_nodes != a->Nn) ||
+        !(obs_prob >= 0.0 && obs_prob <= 1.0))
+        return SIMINF_ERR_INVALID_MODEL;
+    for (int i = 0; i < n_intervals; i++) {
+        const double tb = tspan2[2 * i], te = tspan2[2 * i + 1];
+        if (!std::isfinite(te) || (!std::isnan(tb) && !(tb < te)))
+            return SIMINF_ERR_INVALID_MODEL;
+    }
+
+    /* A plan for n_particles replicates of the particle, two columns. */
+    SimInf_solver_args b = *a;
+    double ts0[2] = {0.0, 1.0};
+    int dummy_u = 0;
+    double dummy_v = 0.0;
+    b.Nrep = n_particles;
+    b.solver = 0;
+    b.tspan = ts0;
+    b.tlen = 2;
+    b.U = &dummy_u;
+    b.V = &dummy_v;
+    b.irU = b.jcU = b.irV = b.jcV = nullptr;
+    b.prU = b.prV = nullptr;
+    SimInf_hip_plan *P = nullptr;
+    int err = plan_create_impl(&P, &b, model, true);
+    if (err)
+        return err;
+
+    const int Nn = a->Nn, Nc = a->Nc, Nd = a->Nd;
+    const long long lanes = (long long)Nn * n_particles;
+    DeviceBuffers &B = P->buf;
+    int *u_cur = B.alloc<int>((size_t)lanes * Nc);
+    double *v_cur = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
+    double *w = B.alloc<double>((size_t)n_particles);
+    int *anc = B.alloc<int>((size_t)n_particles * n_intervals);
+    int *obs_d = B.upload(obs, (size_t)n_intervals * obs_nodes);
+    double *ts_d = B.upload(tspan2, (size_t)2 * n_intervals);
+    double *ll_d = B.alloc<double>(1);
+    double *ess_d = B.alloc<double>((size_t)n_intervals);
+    /* Errors per interval (solver keys and filter codes), so that the
+     * first interval that failed is the one reported. */
+    std::vector<unsigned long long> keys(n_intervals, NO_ERROR_KEY);
+    std::vector<int> perrs(n_intervals, 0);
+    unsigned long long *keys_d = B.upload(keys.data(), keys.size());
+    int *perr_d = B.upload(perrs.data(), perrs.size());
+    const double zero = 0.0;
+    if (B.status != hipSuccess ||
+        hipMemcpyAsync(ll_d, &zero, sizeof(zero), hipMemcpyHostToDevice, P->stream) !=
+            hipSuccess) {
+        SimInf_hip_plan_destroy(P);
+        return SIMINF_ERR_DEVICE;
+    }
+
+    const unsigned grid = (unsigned)((lanes + 255) / 256);
+    for (int i = 0; i < n_intervals; i++) {
+        KParams q = P->p;
+        q.mode = NODES_ALL;
+        q.frozen_from = nullptr;
+        q.u0 = i == 0 ? P->p.u0 : u_cur;
+        q.v0 = i == 0 ? P->p.v0 : v_cur;
+        const double tb = tspan2[2 * i], te = tspan2[2 * i + 1];
+        if (std::isnan(tb)) {
+            q.tlen = 1;
+            q.tspan = ts_d + 2 * i + 1;
+            q.tt0 = te;
+        } else {
+            q.tlen = 2;
+            q.tspan = ts_d + 2 * i;
+            q.tt0 = tb;
+        }
+        q.next0 = std::floor(q.tt0) + 1.0;
+        q.seed = siminf_interval_seed((uint64_t)seed, i);
+        q.err_key = keys_d + i;
+        model->ssm_nodes(q, false, P->canon, grid, 256, P->stream);
+        hipLaunchKernelGGL(k_pf_weights, dim3(n_particles), dim3(64), 0, P->stream, q.U,
+                           Nn, Nc, q.tlen, obs_compartment, obs_prob,
+                           obs_d + (size_t)i * obs_nodes, obs_nodes, w);
+        hipLaunchKernelGGL(k_pf_resample, dim3(1), dim3(256), 0, P->stream, w,
+                           n_particles, i, (unsigned long long)seed, ll_d, ess_d,
+                           anc + (size_t)i * n_particles, perr_d + i);
+        hipLaunchKernelGGL(k_pf_gather, dim3(grid), dim3(256), 0, P->stream, q.U, q.V,
+                           Nn, Nc, Nd, q.tlen, anc + (size_t)i * n_particles, lanes,
+                           u_cur, v_cur);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        SimInf_hip_plan_destroy(P);
+        return SIMINF_ERR_DEVICE;
+    }
+    hipError_t e = hipMemcpyAsync(keys.data(), keys_d, keys.size() * sizeof(keys[0]),
+                                  hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(perrs.data(), perr_d, perrs.size() * sizeof(int),
+                           hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(loglik, ll_d, sizeof(double), hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ess, ess_d, sizeof(double) * n_intervals, hipMemcpyDeviceToHost,
+                           P->stream);
+    if (e == hipSuccess && ancestors)
+        e = hipMemcpyAsync(ancestors, anc, sizeof(int) * (size_t)n_particles * n_intervals,
+                           hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(P->stream);
+    SimInf_hip_plan_destroy(P);
+    if (e != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    for (int i = 0; i < n_intervals; i++) {
+        if (keys[i] != NO_ERROR_KEY)
+            return key_to_code(keys[i]);
+        if (perrs[i])
+            return perrs[i];
+    }
+    return 0;
 }

@@ -1,0 +1,66 @@
+"""GPU parity of the device particle filter: log-likelihood, effective
+sample sizes and ancestors must equal the checker's bit for bit (both run
+the same SSA streams and the same weight, log-sum-exp and resampling
+arithmetic)."""
+import numpy as np
+import pytest
+
+import siminf_amd as si
+
+pytestmark = pytest.mark.gpu
+
+
+def _sir(n_nodes, seed=0):
+    rng = np.random.default_rng(seed)
+    S = rng.integers(40, 200, n_nodes)
+    I = rng.integers(1, 8, n_nodes)
+    return si.SIR_mparse(u0={"S": S, "I": I, "R": np.zeros(n_nodes, int)},
+                         tspan=np.array([1.0, 2.0]), beta=0.25, gamma=0.1)
+
+
+def _check(oracle, model, c, times, obs, P, seed, p=0.5):
+    print("case: pfilter %s Nn=%d P=%d intervals=%d obs=%s seed=%d" % (
+        getattr(model.plugin, "name", model.plugin), model.Nn, P, len(times),
+        np.shape(obs), seed), flush=True)
+    err, ll, ess, anc = oracle.pfilter(model, c, times, obs, P, obs_prob=p, seed=seed)
+    if err:
+        with pytest.raises(si.SimInfError) as e:
+            si.pfilter(model, c, times, obs, P, obs_prob=p, seed=seed)
+        assert e.value.code == err
+        return err, None
+    r = si.pfilter(model, c, times, obs, P, obs_prob=p, seed=seed)
+    assert r.loglik == ll
+    np.testing.assert_array_equal(r.ess, ess)
+    np.testing.assert_array_equal(r.ancestors, anc)
+    return 0, r
+
+
+def test_pfilter_per_node_observations(oracle):
+    m = _sir(40)
+    times = [1, 2, 3, 4, 8, 9, 15]
+    rng = np.random.default_rng(1)
+    obs = np.where(rng.random((len(times), 40)) < 0.5, 0, -1)   # zeros and missing
+    err, r = _check(oracle, m, 1, times, obs, 64, seed=3)
+    assert err == 0 and np.isfinite(r.loglik) and r.loglik < 0
+
+
+def test_pfilter_aggregate_observations(oracle):
+    m = _sir(120, seed=2)
+    times = np.arange(1, 31, 7.0)
+    obs = np.array([[20], [25], [30], [20], [10]])
+    err, r = _check(oracle, m, 1, times, obs, 256, seed=9)
+    assert err == 0
+
+
+def test_pfilter_seir_builtin(oracle):
+    m = si.SEIR(u0={"S": [300] * 16, "E": [0] * 16, "I": [4] * 16, "R": [0] * 16},
+                tspan=np.array([1.0, 2.0]), beta=0.3, epsilon=0.2, gamma=0.1)
+    times = np.arange(3, 40, 4.0)
+    obs = np.zeros((len(times), 1), dtype=int)
+    _check(oracle, m, 2, times, obs, 100, seed=5, p=0.1)
+
+
+def test_pfilter_invalid_observation(oracle):
+    m = _sir(3)
+    err, _ = _check(oracle, m, 1, [1, 2], [[0, 0, 0], [10**6, 0, 0]], 16, seed=1)
+    assert err == -101
