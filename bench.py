@@ -10,6 +10,13 @@ Prints one JSON line: node-days/s (value), transitions/s, p50/p99 step
 latency, the roofline of the solver kernel (algorithmic bytes over the
 kernel time measured with HIP events on the plan's stream) and a CPU
 baseline (the C checker, one core, on the same workload).
+
+--workload pfilter_stream measures the particle-filter serving loop
+instead: a stream of filter requests (SIR on log-uniform node counts,
+2000 particles, 30 weekly observations) with Poisson arrivals through
+siminf_amd.serving.PFilterServer, reported as particle-simulations/s
+with p50/p99 request latency, beside the plain path (one run() per
+interval, resampling on the host) on a sample of the same distribution.
 """
 import argparse
 import ctypes
@@ -105,6 +112,116 @@ def _ndays(model):
     return k + 1
 
 
+def bench_pfilter_stream(args):
+    """Particle-filter request stream through the serving loop.  A
+    particle-simulation is one particle carried through all observation
+    intervals of its request."""
+    from siminf_amd import _native, serving
+    from siminf_amd import workloads as W
+    lib = _native.load()
+
+    def request(spec):
+        return serving.Request(spec.model, 1, spec.times, spec.obs, spec.n_particles,
+                               obs_prob=spec.obs_prob, seed=spec.seed,
+                               obs_dist=args.obs, obs_offset=1e-6)
+
+    # Warm-up: the first launches of each kernel, outside the timed stream.
+    for spec in W.pfilter_requests(max(args.warmup, 1), seed=99, nodes=(100, 1000),
+                                   n_particles=args.particles):
+        serving._run_filter(request(spec), lib)
+
+    specs = W.pfilter_requests(args.requests, seed=5, n_particles=args.particles,
+                               rate=args.rate)
+    server = serving.PFilterServer(hbm_budget=args.hbm_budget_gib << 30,
+                                   max_inflight=args.inflight, lib=lib)
+    reqs = []
+    with server:
+        t0 = time.perf_counter()
+        for spec in specs:
+            delay = t0 + spec.arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            reqs.append(server.submit(request(spec)))
+        for k, r in enumerate(reqs):
+            r.done.wait()
+            print("request %d: %d nodes x %d particles, %.3f s latency%s"
+                  % (k, r.model.Nn, r.n_particles, r.latency,
+                     "" if r.error is None else " error: %s" % r.error),
+                  file=sys.stderr, flush=True)
+        wall = max(r.t_done for r in reqs) - t0
+    errors = [str(r.error) for r in reqs if r.error is not None]
+    lat = [r.latency for r in reqs]
+    psims = sum(r.n_particles for r in reqs)
+    node_days = sum(r.n_particles * r.model.Nn * (r.times[-1] - r.model.tspan[0])
+                    for r in reqs)
+
+    # The same comparison sample through both paths: the served loop (all
+    # submitted at once) and the plain path (one run() per interval).
+    sample = W.pfilter_requests(args.plain_sample, seed=17, nodes=(100, 10000),
+                                n_particles=args.particles, rate=args.rate)
+    sreqs = []
+    with serving.PFilterServer(hbm_budget=args.hbm_budget_gib << 30,
+                               max_inflight=args.inflight, lib=lib) as srv:
+        ts0 = time.perf_counter()
+        for spec in sample:
+            sreqs.append(srv.submit(request(spec)))
+        for r in sreqs:
+            r.done.wait()
+        served_s = max(r.t_done for r in sreqs) - ts0
+    tp0 = time.perf_counter()
+    plain_ll = []
+    for spec in sample:
+        t1 = time.perf_counter()
+        ll, _ = serving.host_pfilter(spec.model, 1, spec.times, spec.obs, spec.n_particles,
+                                     obs_prob=spec.obs_prob, seed=spec.seed, lib=lib,
+                                     obs_dist=args.obs, obs_offset=1e-6)
+        plain_ll.append(ll)
+        print("plain path: %d nodes, %.2f s" % (spec.model.Nn, time.perf_counter() - t1),
+              file=sys.stderr, flush=True)
+    plain_s = time.perf_counter() - tp0
+    sample_psims = sum(s.n_particles for s in sample)
+
+    out = {
+        "metric": "particle-simulations/sec of the filter request stream (p50/p99 "
+                  "request latency alongside)",
+        "value": psims / wall,
+        "unit": "particle-simulations/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": wall / len(reqs) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 state, f64 rates/weights",
+        "data": "synthetic (seeded request generator; observations thinned from the "
+                "mean-field SIR)",
+        "config": {"workload": "pfilter request stream: SIR, nodes log-uniform in "
+                               "[100, 100000], %d particles, 30 weekly observations of "
+                               "the infected total, %s observation process, Poisson "
+                               "arrivals at %.1f/s" % (args.particles, args.obs, args.rate),
+                   "requests": len(reqs), "max_inflight": args.inflight,
+                   "hbm_budget_gib": args.hbm_budget_gib},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "particle_node_days_per_s": node_days / wall,
+        "wall_s": wall,
+        "errors": errors,
+        "peak_inflight": server.peak_inflight,
+        "peak_hbm_bytes_admitted": server.peak_bytes,
+        "plain_path_sample": {
+            "requests": len(sample),
+            "nodes": [int(s.model.Nn) for s in sample],
+            "served_s": served_s, "plain_s": plain_s,
+            "served_particle_sims_per_s": sample_psims / served_s,
+            "plain_particle_sims_per_s": sample_psims / plain_s,
+            "speedup": plain_s / served_s,
+            "served_loglik": [r.result.loglik if r.result else None for r in sreqs],
+            "plain_loglik": plain_ll},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +229,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sise3_herds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--requests", type=int, default=24,
+                    help="pfilter_stream: requests in the timed stream")
+    ap.add_argument("--particles", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=4.0, help="pfilter_stream: arrivals/s")
+    ap.add_argument("--inflight", type=int, default=8)
+    ap.add_argument("--hbm-budget-gib", type=int, default=96)
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--obs", default="poisson", choices=("poisson", "binomial"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (torch), for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -131,6 +256,9 @@ def main():
               "(the multi-GPU sharding is a design, not a launch path)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "pfilter_stream":
+        bench_pfilter_stream(args)
+        return
 
     from siminf_amd import _native
     model, config = make_workload(args.workload)

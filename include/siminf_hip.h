@@ -224,19 +224,23 @@ void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
  * runs n_particles replicates of it.  Interval i runs from tspan2[2i] to
  * tspan2[2i+1], or over the one time point tspan2[2i+1] when tspan2[2i]
  * is NaN (pfilter_tspan, R/pfilter.R:177-193).  The observation process
- * is binomial: the observed count of compartment obs_compartment
- * (zero-based) with detection probability obs_prob, either per node
- * (obs_nodes == Nn) or of the sum over the nodes (obs_nodes == 1); obs is
- * n_intervals x obs_nodes, and a negative count is missing.  The weight of
- * a particle is the sum of the log-probabilities of its observations.
+ * (the binomial and poisson forms of pfilter's formula, R/pfilter.R:279-295)
+ * relates the observed count x to the count X of compartment
+ * obs_compartment (zero-based), either per node (obs_nodes == Nn) or of
+ * the sum over the nodes (obs_nodes == 1): obs_dist 0 is
+ * x ~ Binomial(size = X, prob = obs_p1), obs_dist 1 is
+ * x ~ Poisson(lambda = obs_p1 * X + obs_p2).  obs is n_intervals x
+ * obs_nodes, and a negative count is missing.  The weight of a particle is
+ * the sum of the log-probabilities of its observations.
  * Outputs: loglik (the log-likelihood estimate), ess (n_intervals
  * effective sample sizes), ancestors (n_intervals x n_particles
  * zero-based indices, or NULL).  Returns 0 or an error code. */
 int SimInf_hip_pfilter(const SimInf_solver_args *args,
                        const SimInf_hip_model *model, int n_particles,
                        int n_intervals, const double *tspan2,
-                       int obs_compartment, double obs_prob, int obs_nodes,
-                       const int *obs, unsigned long int seed, double *loglik,
+                       int obs_compartment, int obs_dist, double obs_p1,
+                       double obs_p2, int obs_nodes, const int *obs,
+                       unsigned long int seed, double *loglik,
                        double *ess, int *ancestors);
 
 #ifdef __cplusplus

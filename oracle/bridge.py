@@ -50,11 +50,13 @@ def load():
                                                    ctypes.c_int]
         lib.oracle_pfilter.restype = ctypes.c_int
         lib.oracle_pfilter.argtypes = [
-            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double,
-            ctypes.c_int, ip, ctypes.c_ulonglong, dp, dp, ip]
+            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+            ctypes.c_double, ctypes.c_double, ctypes.c_int, ip, ctypes.c_ulonglong, dp, dp,
+            ip]
         lib.oracle_exp.argtypes = [dp, dp, ctypes.c_longlong]
         lib.oracle_lfact.argtypes = [ip, dp, ctypes.c_longlong]
         lib.oracle_log_dbinom.argtypes = [ip, ip, ctypes.c_double, dp, ctypes.c_longlong]
+        lib.oracle_log_dpois.argtypes = [ip, dp, dp, ctypes.c_longlong]
         lib.oracle_resample.restype = ctypes.c_int
         lib.oracle_sample.restype = ctypes.c_int
         lib.oracle_sample.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ip,
@@ -101,11 +103,13 @@ def run(model, solver="ssm", seed=0, crn=False):
     return OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
 
 
-def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5, seed=0):
+def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5, seed=0,
+            obs_dist="binomial", obs_offset=0.0):
     """The checker's filter on the same inputs as siminf_amd.pfilter;
     returns (error code, loglik, ess, ancestors)."""
     from siminf_amd import particle_filter as pf
     lib = load()
+    dist = pf.OBS_DISTRIBUTIONS[obs_dist]
     tspan2, obs, obs_nodes = pf.prepare(model, obs_compartment, times, obs)
     b = _native.ArgsBundle(model, solver=0, seed=int(seed))
     _set_plugin(lib, model, b)
@@ -116,8 +120,9 @@ def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5, seed=
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
     err = lib.oracle_pfilter(ctypes.byref(b.args), int(n_particles), n_int,
-                             tspan2.ctypes.data_as(dp), int(obs_compartment),
-                             float(obs_prob), obs_nodes, obs.ctypes.data_as(ip),
+                             tspan2.ctypes.data_as(dp), int(obs_compartment), dist,
+                             float(obs_prob), float(obs_offset), obs_nodes,
+                             obs.ctypes.data_as(ip),
                              ctypes.c_ulonglong(int(seed)), ctypes.byref(loglik),
                              ess.ctypes.data_as(dp), anc.ctypes.data_as(ip))
     return err, loglik.value, ess, anc
@@ -147,6 +152,16 @@ def log_dbinom(x, n, p):
     ip = ctypes.POINTER(ctypes.c_int)
     load().oracle_log_dbinom(x.ctypes.data_as(ip), n.ctypes.data_as(ip), ctypes.c_double(p),
                              y.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), x.size)
+    return y
+
+
+def log_dpois(x, lam):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    lam = np.ascontiguousarray(np.broadcast_to(lam, x.shape), dtype=np.float64)
+    y = np.empty(x.shape)
+    dp = ctypes.POINTER(ctypes.c_double)
+    load().oracle_log_dpois(x.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            lam.ctypes.data_as(dp), y.ctypes.data_as(dp), x.size)
     return y
 
 

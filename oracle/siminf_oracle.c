@@ -1106,7 +1106,8 @@ int oracle_sample(int enter, const int *irE, const int *jcE, const double *prE,
  * 474).  a describes one particle, with tr_fun/pts_fun set.  Arguments as
  * SimInf_hip_pfilter. */
 int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
-                   const double *tspan2, int obs_c, double obs_p, int obs_nodes,
+                   const double *tspan2, int obs_c, int obs_dist, double obs_p1,
+                   double obs_p2, int obs_nodes,
                    const int *obs, unsigned long long seed, double *loglik,
                    double *ess, int *ancestors)
 {
@@ -1152,7 +1153,8 @@ int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
                 for (int k = 0; k < Nn; k++)
                     tot += U[(col + k) * Nc + obs_c];
                 w[p] = obs[(size_t)i] < 0 ? 0.0
-                                          : siminf_log_dbinom(obs[(size_t)i], (int)tot, obs_p);
+                                          : siminf_obs_logw(obs_dist, obs[(size_t)i], tot,
+                                                            obs_p1, obs_p2);
             } else {
                 double partial[64], sum = 0.0;
                 for (int l = 0; l < 64; l++)
@@ -1161,7 +1163,8 @@ int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
                     const int x = obs[(size_t)i * Nn + k];
                     if (x >= 0)
                         partial[k % 64] = partial[k % 64] +
-                            siminf_log_dbinom(x, U[(col + k) * Nc + obs_c], obs_p);
+                            siminf_obs_logw(obs_dist, x, U[(col + k) * Nc + obs_c],
+                                            obs_p1, obs_p2);
                 }
                 for (int l = 0; l < 64; l++)
                     sum = sum + partial[l];
@@ -1232,6 +1235,12 @@ void oracle_lfact(const int *n, double *y, long long count)
 {
     for (long long i = 0; i < count; i++)
         y[i] = siminf_lfact(n[i]);
+}
+
+void oracle_log_dpois(const int *x, const double *lambda, double *y, long long count)
+{
+    for (long long i = 0; i < count; i++)
+        y[i] = siminf_log_dpois(x[i], lambda[i]);
 }
 
 void oracle_log_dbinom(const int *x, const int *n, double p, double *y, long long count)

@@ -133,8 +133,8 @@ def load(path=None):
     dp = ctypes.POINTER(ctypes.c_double)
     lib.SimInf_hip_pfilter.restype = ctypes.c_int
     lib.SimInf_hip_pfilter.argtypes = [
-        args_p, vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double,
-        ctypes.c_int, ip, ctypes.c_ulong, dp, dp, ip]
+        args_p, vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
+        ctypes.c_double, ctypes.c_double, ctypes.c_int, ip, ctypes.c_ulong, dp, dp, ip]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
     if path == LIB_PATH:

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace siminf;
@@ -193,8 +194,39 @@ namespace {
 
 enum Kernel { K_SSM_NODES, K_AEM_NODES, K_COUPLED_GRID, K_COUPLED_REPS };
 
+/* Freed pool memory the process keeps for reuse (per device). */
+constexpr uint64_t SIMINF_POOL_KEEP = 16ull << 30;
+
+/* Stream-ordered allocation where the device supports memory pools:
+ * hipFree waits for the whole device, which would serialise plans that
+ * run on their own streams at the same time (concurrent filter
+ * requests); hipFreeAsync orders the free on the plan's stream only. */
+static bool pool_allocation(int dev)
+{
+    static std::mutex mu;
+    static signed char ready[64];
+    if (dev < 0 || dev >= 64)
+        return false;
+    std::lock_guard<std::mutex> lock(mu);
+    if (ready[dev] == 0) {
+        int ok = 0;
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = SIMINF_POOL_KEEP;
+        ready[dev] = -1;
+        if (hipDeviceGetAttribute(&ok, hipDeviceAttributeMemoryPoolsSupported, dev) ==
+                hipSuccess && ok &&
+            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess &&
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) ==
+                hipSuccess)
+            ready[dev] = 1;
+    }
+    return ready[dev] == 1;
+}
+
 struct DeviceBuffers {
     std::vector<void *> ptrs;
+    hipStream_t stream = nullptr;   /* copies (and pooled allocations) go here */
+    bool pooled = false;
     hipError_t status = hipSuccess;
 
     template <class T>
@@ -204,11 +236,22 @@ struct DeviceBuffers {
         if (status != hipSuccess)
             return nullptr;
         const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
-        status = hipMalloc(&p, bytes);
+        status = pooled ? hipMallocAsync(&p, bytes, stream) : hipMalloc(&p, bytes);
         if (status != hipSuccess)
             return nullptr;
         ptrs.push_back(p);
         return (T *)p;
+    }
+
+    /* A copy on the buffers' stream, complete on return (the host side may
+     * be a temporary). */
+    void copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+    {
+        if (status != hipSuccess || bytes == 0)
+            return;
+        status = hipMemcpyAsync(dst, src, bytes, kind, stream);
+        if (status == hipSuccess)
+            status = hipStreamSynchronize(stream);
     }
 
     template <class T>
@@ -216,15 +259,21 @@ struct DeviceBuffers {
     {
         T *d = alloc<T>(count);
         if (d && count && host)
-            status = hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice);
+            copy(d, host, count * sizeof(T), hipMemcpyHostToDevice);
         return d;
     }
 
-    ~DeviceBuffers()
+    /* Frees everything; the stream must still exist. */
+    void release()
     {
         for (void *p : ptrs)
-            (void)hipFree(p);
+            (void)(pooled ? hipFreeAsync(p, stream) : hipFree(p));
+        if (pooled && !ptrs.empty())
+            (void)hipStreamSynchronize(stream);
+        ptrs.clear();
     }
+
+    ~DeviceBuffers() { release(); }
 };
 
 } // namespace
@@ -349,10 +398,11 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
     return std::max<size_t>(off, 16);
 }
 
-/* With replicate_from_one, u0 and v0 hold one replicate (Nn columns)
- * and are replicated on the device to a->Nrep replicates. */
+/* filter_plan: u0/v0 hold one replicate (Nn columns), copied on the
+ * device to all a->Nrep replicates, and the caller runs every lane in one
+ * pass (no frozen-node buffers). */
 static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
-                            const SimInf_hip_model *model, bool replicate_from_one)
+                            const SimInf_hip_model *model, bool filter_plan)
 {
     if (!out || !a || !model)
         return SIMINF_ERR_INVALID_MODEL;
@@ -595,8 +645,28 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                                                          : K_COUPLED_REPS;
     }
 
+    /* Streams first: every copy of the plan, and its allocations where the
+     * device has memory pools, are ordered on its own stream, which does
+     * not wait for the legacy default stream. */
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&P->fill_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&P->ev_classified) != hipSuccess ||
+        hipEventCreate(&P->ev_fill0) != hipSuccess ||
+        hipEventCreate(&P->ev_fill1) != hipSuccess ||
+        hipEventCreate(&P->ev_active1) != hipSuccess ||
+        hipEventCreate(&P->ev_start) != hipSuccess ||
+        hipEventCreate(&P->ev_mid) != hipSuccess ||
+        hipEventCreate(&P->ev_stop) != hipSuccess) {
+        SimInf_hip_plan_destroy(P);
+        return SIMINF_ERR_DEVICE;
+    }
+
     /* Device buffers. */
     DeviceBuffers &B = P->buf;
+    B.stream = P->stream;
+    B.pooled = pool_allocation(dev);
     KParams &p = P->p;
     memset(&p, 0, sizeof(p));
     const long long lanes = (long long)Nn * Nrep;
@@ -611,7 +681,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     p.next0 = std::floor(tt0) + 1.0;
     p.ndays = ndays;
 
-    if (!replicate_from_one) {
+    if (!filter_plan) {
         p.u0 = B.upload(a->u0, (size_t)lanes * Nc);
         p.v0 = B.upload(a->v0, (size_t)lanes * Nd);
     } else {
@@ -619,17 +689,12 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         int *u0 = B.alloc<int>((size_t)lanes * Nc);
         double *v0 = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
         const size_t nu = (size_t)Nn * Nc, nv = (size_t)Nn * Nd;
-        if (B.status == hipSuccess)
-            B.status = hipMemcpy(u0, a->u0, nu * sizeof(int), hipMemcpyHostToDevice);
-        if (B.status == hipSuccess && nv)
-            B.status = hipMemcpy(v0, a->v0, nv * sizeof(double), hipMemcpyHostToDevice);
+        B.copy(u0, a->u0, nu * sizeof(int), hipMemcpyHostToDevice);
+        B.copy(v0, a->v0, nv * sizeof(double), hipMemcpyHostToDevice);
         for (long long have = 1; have < Nrep && B.status == hipSuccess; have *= 2) {
             const long long n = std::min<long long>(have, Nrep - have);
-            B.status = hipMemcpy(u0 + have * nu, u0, n * nu * sizeof(int),
-                                 hipMemcpyDeviceToDevice);
-            if (B.status == hipSuccess && nv)
-                B.status = hipMemcpy(v0 + have * nv, v0, n * nv * sizeof(double),
-                                     hipMemcpyDeviceToDevice);
+            B.copy(u0 + have * nu, u0, n * nu * sizeof(int), hipMemcpyDeviceToDevice);
+            B.copy(v0 + have * nv, v0, n * nv * sizeof(double), hipMemcpyDeviceToDevice);
         }
         p.u0 = u0;
         p.v0 = v0;
@@ -694,8 +759,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         /* LDS budget: at most 128 KiB, and small enough that the
          * workgroups a grid-wide launch needs fit on the CUs together
          * (every workgroup gets the same allocation). */
-        int dev = 0, lds_max = 64 * 1024, lds_cu = 64 * 1024, cus = 1;
-        (void)hipGetDevice(&dev);
+        int lds_max = 64 * 1024, lds_cu = 64 * 1024, cus = 1;
         if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                   dev) != hipSuccess)
             lds_max = 64 * 1024;
@@ -723,7 +787,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                                nullptr, &max_blocks) != hipSuccess ||
                 (int)P->grid > max_blocks) {
                 /* Too many nodes for one co-resident grid. */
-                delete P;
+                SimInf_hip_plan_destroy(P);
                 return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
             }
         } else {
@@ -733,7 +797,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     } else {
         P->block = 256;
         P->grid = (unsigned)((lanes + 255) / 256);
-        if (a->U && (Nd == 0 || a->V)) {
+        if (a->U && (Nd == 0 || a->V) && !filter_plan) {
             p.frozen_from = B.alloc<int>((size_t)lanes);
             p.frozen_u = B.alloc<int>((size_t)lanes * Nc);
             p.frozen_v = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
@@ -744,27 +808,14 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     }
 
     if (B.status != hipSuccess) {
-        delete P;
+        SimInf_hip_plan_destroy(P);
         return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
     }
     {
-        int dev = 0, cus = 256;
-        (void)hipGetDevice(&dev);
+        int cus = 256;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
                 hipSuccess && cus > 0)
             P->simds = 4 * cus;
-    }
-    if (hipStreamCreate(&P->stream) != hipSuccess ||
-        hipStreamCreate(&P->fill_stream) != hipSuccess ||
-        hipEventCreate(&P->ev_classified) != hipSuccess ||
-        hipEventCreate(&P->ev_fill0) != hipSuccess ||
-        hipEventCreate(&P->ev_fill1) != hipSuccess ||
-        hipEventCreate(&P->ev_active1) != hipSuccess ||
-        hipEventCreate(&P->ev_start) != hipSuccess ||
-        hipEventCreate(&P->ev_mid) != hipSuccess ||
-        hipEventCreate(&P->ev_stop) != hipSuccess) {
-        SimInf_hip_plan_destroy(P);
-        return SIMINF_ERR_DEVICE;
     }
     *out = P;
     return 0;
@@ -931,16 +982,21 @@ extern "C" int SimInf_hip_plan_download(SimInf_hip_plan *P, int *U, double *V,
         return SIMINF_ERR_INVALID_MODEL;
     const size_t ncols = (size_t)P->Nrep * P->tlen;
     hipError_t e = hipSuccess;
+    hipStream_t st = P->stream;
     if (U && P->dense_U)
-        e = hipMemcpy(U, P->p.U, (size_t)P->Nn * P->Nc * ncols * sizeof(int),
-                      hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(U, P->p.U, (size_t)P->Nn * P->Nc * ncols * sizeof(int),
+                           hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && prU && !P->dense_U && P->nnzU)
-        e = hipMemcpy(prU, P->p.prU, P->nnzU * sizeof(double), hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(prU, P->p.prU, P->nnzU * sizeof(double), hipMemcpyDeviceToHost,
+                           st);
     if (e == hipSuccess && V && P->dense_V && P->Nd > 0)
-        e = hipMemcpy(V, P->p.V, (size_t)P->Nn * P->Nd * ncols * sizeof(double),
-                      hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(V, P->p.V, (size_t)P->Nn * P->Nd * ncols * sizeof(double),
+                           hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && prV && !P->dense_V && P->nnzV)
-        e = hipMemcpy(prV, P->p.prV, P->nnzV * sizeof(double), hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(prV, P->p.prV, P->nnzV * sizeof(double), hipMemcpyDeviceToHost,
+                           st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
     return e == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
 }
 
@@ -978,8 +1034,12 @@ extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
 {
     if (!P)
         return;
-    if (P->stream)
+    if (P->fill_stream)
+        (void)hipStreamSynchronize(P->fill_stream);
+    if (P->stream) {
         (void)hipStreamSynchronize(P->stream);
+        P->buf.release();
+    }
     if (P->ev_start)
         (void)hipEventDestroy(P->ev_start);
     if (P->ev_mid)
@@ -1122,7 +1182,8 @@ extern "C" const char *SimInf_hip_error_message(int code)
  * uses.  With one observation for all nodes the count observed is the
  * sum over the nodes (exact in any order). */
 __global__ __launch_bounds__(64) void k_pf_weights(const int *U, int Nn, int Nc,
-                                                   int tlen, int c_obs, double prob,
+                                                   int tlen, int c_obs, int dist,
+                                                   double p1, double p2,
                                                    const int *obs, int obs_nodes,
                                                    double *w)
 {
@@ -1140,8 +1201,7 @@ __global__ __launch_bounds__(64) void k_pf_weights(const int *U, int Nn, int Nc,
             long long sum = 0;
             for (int l = 0; l < 64; l++)
                 sum += total[l];
-            w[part] = obs[0] < 0 ? 0.0
-                                 : siminf_log_dbinom(obs[0], (int)sum, prob);
+            w[part] = obs[0] < 0 ? 0.0 : siminf_obs_logw(dist, obs[0], sum, p1, p2);
         }
         return;
     }
@@ -1149,7 +1209,7 @@ __global__ __launch_bounds__(64) void k_pf_weights(const int *U, int Nn, int Nc,
     for (int node = lane; node < Nn; node += 64) {
         const int x = obs[node];
         if (x >= 0)
-            acc = acc + siminf_log_dbinom(x, U[(col + node) * Nc + c_obs], prob);
+            acc = acc + siminf_obs_logw(dist, x, U[(col + node) * Nc + c_obs], p1, p2);
     }
     partial[lane] = acc;
     __syncthreads();
@@ -1255,14 +1315,17 @@ __global__ __launch_bounds__(256) void k_pf_gather(const int *U, const double *V
 extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
                                   const SimInf_hip_model *model, int n_particles,
                                   int n_intervals, const double *tspan2,
-                                  int obs_compartment, double obs_prob, int obs_nodes,
-                                  const int *obs, unsigned long int seed,
+                                  int obs_compartment, int obs_dist, double obs_p1,
+                                  double obs_p2, int obs_nodes, const int *obs,
+                                  unsigned long int seed,
                                   double *loglik, double *ess, int *ancestors)
 {
     if (!a || !model || !tspan2 || !obs || !loglik || !ess || n_particles < 1 ||
         n_intervals < 1 || a->Nrep > 1 || a->len != 0 || obs_compartment < 0 ||
         obs_compartment >= a->Nc || (obs_nodes != 1 && obs_nodes != a->Nn) ||
-        !(obs_prob >= 0.0 && obs_prob <= 1.0))
+        (obs_dist != SIMINF_OBS_BINOMIAL && obs_dist != SIMINF_OBS_POISSON) ||
+        (obs_dist == SIMINF_OBS_BINOMIAL && !(obs_p1 >= 0.0 && obs_p1 <= 1.0)) ||
+        (obs_dist == SIMINF_OBS_POISSON && !(std::isfinite(obs_p1) && std::isfinite(obs_p2))))
         return SIMINF_ERR_INVALID_MODEL;
     for (int i = 0; i < n_intervals; i++) {
         const double tb = tspan2[2 * i], te = tspan2[2 * i + 1];
@@ -1335,7 +1398,7 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
         q.err_key = keys_d + i;
         model->ssm_nodes(q, false, P->canon, grid, 256, P->stream);
         hipLaunchKernelGGL(k_pf_weights, dim3(n_particles), dim3(64), 0, P->stream, q.U,
-                           Nn, Nc, q.tlen, obs_compartment, obs_prob,
+                           Nn, Nc, q.tlen, obs_compartment, obs_dist, obs_p1, obs_p2,
                            obs_d + (size_t)i * obs_nodes, obs_nodes, w);
         hipLaunchKernelGGL(k_pf_resample, dim3(1), dim3(256), 0, P->stream, w,
                            n_particles, i, (unsigned long long)seed, ll_d, ess_d,

@@ -7,9 +7,9 @@
  *
  *   - siminf_exp: exp with a fixed sequence of IEEE operations (the
  *     log-sum-exp of the weights, pfilter.R:468-470);
- *   - siminf_lfact / siminf_log_dbinom: log n! and the binomial
- *     log-probability of the observation process (stats::dbinom(...,
- *     log = TRUE), pfilter.R:286-292);
+ *   - siminf_lfact / siminf_log_dbinom / siminf_log_dpois: log n! and the
+ *     log-probabilities of the observation processes (stats::dbinom and
+ *     stats::dpois with log = TRUE, pfilter.R:279-295);
  *   - siminf_interval_seed: the seed of interval i's run (the reference
  *     draws a new seed for every run() from R's RNG);
  *   - siminf_systematic_resampling: SimInf_particle_filter.c:32-66.
@@ -108,6 +108,29 @@ SIMINF_FN double siminf_log_dbinom(int x, int n, double p)
         return x == n ? 0.0 : NEG_INF;
     return (siminf_lfact(n) - siminf_lfact(x) - siminf_lfact(n - x)) +
            ((double)x * siminf_log(p) + (double)(n - x) * siminf_log(1.0 - p));
+}
+
+/* log P(X = x) for X ~ Poisson(lambda); lambda <= 0 puts all mass on 0. */
+SIMINF_FN double siminf_log_dpois(int x, double lambda)
+{
+    const double NEG_INF = -1.0 / 0.0;
+    if (x < 0)
+        return NEG_INF;
+    if (!(lambda > 0.0))
+        return x == 0 ? 0.0 : NEG_INF;
+    return ((double)x * siminf_log(lambda) - lambda) - siminf_lfact(x);
+}
+
+/* Observation processes of the filter: the log-weight of observing x
+ * when the particle holds X individuals in the observed compartment. */
+#define SIMINF_OBS_BINOMIAL 0   /* x ~ Binomial(size = X, prob = p1) */
+#define SIMINF_OBS_POISSON 1    /* x ~ Poisson(lambda = p1 * X + p2) */
+
+SIMINF_FN double siminf_obs_logw(int dist, int x, long long X, double p1, double p2)
+{
+    if (dist == SIMINF_OBS_POISSON)
+        return siminf_log_dpois(x, p1 * (double)X + p2);
+    return siminf_log_dbinom(x, (int)X, p1);
 }
 
 /* Seed of interval i's run: splitmix64 of seed + i. */

@@ -5,9 +5,9 @@ The state of every particle stays in HBM for the whole filter: per
 observation interval one launch runs all particles' SSA, and the weights,
 the log-likelihood increment, the effective sample size, systematic
 resampling (SimInf_particle_filter.c:32-66) and the ancestor gather run on
-the device behind it.  The observation process is binomial: the observed
-count of one compartment with a detection probability, per node or of the
-sum over the nodes.
+the device behind it.  The observation process is the binomial or the
+poisson form of pfilter's formula (R/pfilter.R:279-295) on the count of one
+compartment, per node or of the sum over the nodes.
 """
 import ctypes
 
@@ -15,6 +15,9 @@ import numpy as np
 
 from . import _native
 from .model import SimInfError, _raise, device_plugin
+
+
+OBS_DISTRIBUTIONS = {"binomial": 0, "poisson": 1}
 
 
 class PFilterResult:
@@ -68,11 +71,15 @@ def prepare(model, obs_compartment, times, obs):
 
 
 def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5,
-            seed=None, lib=None):
+            seed=None, lib=None, obs_dist="binomial", obs_offset=0.0):
     """Run the filter on the GPU.  obs_compartment is zero-based; obs is
     (number of times) x Nn, or one column for the sum over the nodes;
-    missing observations are negative or NaN.  Raises SimInfError with the
-    reference's message on failure."""
+    missing observations are negative or NaN.  The observed count x of a
+    compartment holding X is Binomial(size = X, prob = obs_prob), or with
+    obs_dist "poisson" Poisson(lambda = obs_prob * X + obs_offset).  Raises
+    SimInfError with the reference's message on failure."""
+    if obs_dist not in OBS_DISTRIBUTIONS:
+        raise ValueError("Unknown distribution: '%s'." % obs_dist)
     lib = lib or _native.load()
     tspan2, obs, obs_nodes = prepare(model, obs_compartment, times, obs)
     if seed is None:
@@ -87,12 +94,14 @@ def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5,
     ip = ctypes.POINTER(ctypes.c_int)
     err = lib.SimInf_hip_pfilter(
         ctypes.byref(b.args), ctypes.c_void_p(dev), int(n_particles), n_int,
-        tspan2.ctypes.data_as(dp), int(obs_compartment), float(obs_prob), obs_nodes,
-        obs.ctypes.data_as(ip), int(seed) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(loglik),
-        ess.ctypes.data_as(dp), anc.ctypes.data_as(ip))
+        tspan2.ctypes.data_as(dp), int(obs_compartment), OBS_DISTRIBUTIONS[obs_dist],
+        float(obs_prob), float(obs_offset), obs_nodes, obs.ctypes.data_as(ip),
+        int(seed) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(loglik), ess.ctypes.data_as(dp),
+        anc.ctypes.data_as(ip))
     if err:
         _raise(lib, err)
     return PFilterResult(loglik.value, ess, anc, int(n_particles))
 
 
-__all__ = ["pfilter", "pfilter_tspan", "PFilterResult", "SimInfError"]
+__all__ = ["pfilter", "pfilter_tspan", "PFilterResult", "SimInfError",
+           "OBS_DISTRIBUTIONS"]

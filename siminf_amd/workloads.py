@@ -13,6 +13,10 @@
                       (parameters of man/examples/SISe3.R)
   seir_million        SEIR on Nn ~ Poisson(500) nodes, 0.5 % seeded with
                       5 infected, beta 0.16, epsilon 0.25, gamma 0.077
+  pfilter_requests    particle-filter requests: SIR on node counts
+                      log-uniform in [100, 100000], beta ~ U(0.05, 0.4),
+                      gamma ~ U(0.03, 0.2), 30 weekly observations of the
+                      infected total thinned with p = 0.5, Poisson arrivals
 
 Scheduled events must stay feasible: the reference aborts when an event
 asks for more individuals than a node holds (SimInf_solver.c:102-105).
@@ -223,3 +227,61 @@ def sise3_herds(Nn=1600, days=1460, seed=11, events=True, replicates=1,
     return models.SISe3(u0=u0, tspan=np.arange(1, days + 1, dtype=float),
                         events=ev, phi=0.0, end_t1=91, end_t2=182, end_t3=273,
                         end_t4=365, replicates=replicates, **SISE3_GDATA)
+
+
+class FilterSpec:
+    """A particle-filter request of the request-stream workload: the model,
+    the observation times and counts, and the arrival offset (seconds)."""
+
+    def __init__(self, model, times, obs, n_particles, obs_prob, seed, arrival, beta,
+                 gamma):
+        self.model = model
+        self.times = times
+        self.obs = obs
+        self.n_particles = n_particles
+        self.obs_prob = obs_prob
+        self.seed = seed
+        self.arrival = arrival
+        self.beta = beta
+        self.gamma = gamma
+
+
+def _sir_mean_infected(beta, gamma, S0, I0, R0, times, t0, substeps=16):
+    """I(t) of the mean-field SIR of one node (Euler, substeps per day)."""
+    S, I, R = float(S0), float(I0), float(R0)
+    N = S + I + R
+    out = []
+    t = float(t0)
+    h = 1.0 / substeps
+    for te in times:
+        while t < te - 1e-12:
+            inf = beta * S * I / N * h
+            rec = gamma * I * h
+            S, I, R = S - inf, I + inf - rec, R + rec
+            t += h
+        out.append(I)
+    return np.asarray(out)
+
+
+def pfilter_requests(n_requests=24, seed=5, nodes=(100, 100000), n_particles=2000,
+                     n_obs=30, every=7, obs_prob=0.5, rate=4.0):
+    """Requests of the particle-filter stream: SIR (S=99, I=5, R=0 per
+    node) with node counts log-uniform in ``nodes``; observations are the
+    binomially thinned mean-field infected total at days 1 + every*k."""
+    rng = np.random.default_rng(seed)
+    times = 1.0 + every * np.arange(1, n_obs + 1, dtype=float)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    arrivals -= arrivals[0]
+    out = []
+    for k in range(n_requests):
+        Nn = int(round(np.exp(rng.uniform(np.log(nodes[0]), np.log(nodes[1])))))
+        beta = float(rng.uniform(0.05, 0.4))
+        gamma = float(rng.uniform(0.03, 0.2))
+        I = _sir_mean_infected(beta, gamma, 99, 5, 0, times, 1.0)
+        obs = rng.binomial(np.round(I * Nn).astype(np.int64), obs_prob)
+        model = models.SIR_mparse(u0={"S": np.full(Nn, 99), "I": np.full(Nn, 5),
+                                      "R": np.zeros(Nn, int)},
+                                  tspan=np.array([1.0, times[-1]]), beta=beta, gamma=gamma)
+        out.append(FilterSpec(model, times, obs.reshape(-1, 1), n_particles, obs_prob,
+                              int(rng.integers(0, 2**62)), float(arrivals[k]), beta, gamma))
+    return out

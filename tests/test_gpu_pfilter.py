@@ -18,17 +18,18 @@ def _sir(n_nodes, seed=0):
                          tspan=np.array([1.0, 2.0]), beta=0.25, gamma=0.1)
 
 
-def _check(oracle, model, c, times, obs, P, seed, p=0.5):
-    print("case: pfilter %s Nn=%d P=%d intervals=%d obs=%s seed=%d" % (
+def _check(oracle, model, c, times, obs, P, seed, p=0.5, dist="binomial", offset=0.0):
+    print("case: pfilter %s Nn=%d P=%d intervals=%d obs=%s %s seed=%d" % (
         getattr(model.plugin, "name", model.plugin), model.Nn, P, len(times),
-        np.shape(obs), seed), flush=True)
-    err, ll, ess, anc = oracle.pfilter(model, c, times, obs, P, obs_prob=p, seed=seed)
+        np.shape(obs), dist, seed), flush=True)
+    kw = dict(obs_prob=p, seed=seed, obs_dist=dist, obs_offset=offset)
+    err, ll, ess, anc = oracle.pfilter(model, c, times, obs, P, **kw)
     if err:
         with pytest.raises(si.SimInfError) as e:
-            si.pfilter(model, c, times, obs, P, obs_prob=p, seed=seed)
+            si.pfilter(model, c, times, obs, P, **kw)
         assert e.value.code == err
         return err, None
-    r = si.pfilter(model, c, times, obs, P, obs_prob=p, seed=seed)
+    r = si.pfilter(model, c, times, obs, P, **kw)
     assert r.loglik == ll
     np.testing.assert_array_equal(r.ess, ess)
     np.testing.assert_array_equal(r.ancestors, anc)
@@ -49,6 +50,20 @@ def test_pfilter_aggregate_observations(oracle):
     times = np.arange(1, 31, 7.0)
     obs = np.array([[20], [25], [30], [20], [10]])
     err, r = _check(oracle, m, 1, times, obs, 256, seed=9)
+    assert err == 0
+
+
+def test_pfilter_poisson_observations(oracle):
+    m = _sir(50, seed=4)
+    times = np.arange(1, 43, 7.0)
+    rng = np.random.default_rng(8)
+    obs = rng.poisson(2.0, size=(len(times), 50))
+    err, r = _check(oracle, m, 1, times, obs, 128, seed=11, p=0.5, dist="poisson",
+                    offset=0.1)
+    assert err == 0 and np.isfinite(r.loglik)
+    agg = obs.sum(axis=1, keepdims=True)
+    err, r = _check(oracle, m, 1, times, agg, 128, seed=12, p=0.5, dist="poisson",
+                    offset=1e-6)
     assert err == 0
 
 

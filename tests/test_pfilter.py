@@ -1,6 +1,7 @@
 """The particle filter's arithmetic and the checker's filter.
 
-Pins: exp, log n! and the binomial log-probability against numpy/scipy;
+Pins: exp, log n! and the binomial and Poisson log-probabilities against
+numpy/scipy;
 systematic resampling against its defining property (every particle gets
 floor(n w) or floor(n w) + 1 offspring, ancestors in order) and the error
 cases of SimInf_particle_filter.c:42-48; the interval split of
@@ -47,6 +48,18 @@ def test_lfact_and_log_dbinom(oracle):
     assert oracle.log_dbinom([5], [3], 0.5)[0] == -np.inf
     assert oracle.log_dbinom([0], [7], 0.0)[0] == 0.0
     assert oracle.log_dbinom([7], [7], 1.0)[0] == 0.0
+
+
+def test_log_dpois(oracle):
+    rng = np.random.default_rng(6)
+    lam = np.concatenate([rng.uniform(1e-6, 5, 10000), rng.uniform(5, 2e5, 10000)])
+    x = rng.poisson(lam)
+    x[::7] = rng.integers(0, 400, len(x[::7]))
+    y = oracle.log_dpois(x, lam)
+    assert np.allclose(y, stats.poisson.logpmf(x, lam), rtol=1e-12, atol=1e-8)
+    assert oracle.log_dpois([0], 0.0)[0] == 0.0
+    assert oracle.log_dpois([3], 0.0)[0] == -np.inf
+    assert oracle.log_dpois([-1], 2.0)[0] == -np.inf
 
 
 def test_systematic_resampling(oracle):
@@ -109,6 +122,11 @@ def test_pfilter_frozen_model(oracle):
     # the sum over the nodes as one observation
     err, ll, _, _ = oracle.pfilter(m, 1, [1, 2, 4, 8], obs.sum(axis=1), 20, seed=2)
     assert ll == pytest.approx(stats.binom.logpmf(obs.sum(axis=1), 48, 0.5).sum(), rel=1e-12)
+    # Poisson(lambda = 0.3 X + 0.2)
+    err, ll, _, _ = oracle.pfilter(m, 1, [1, 2, 4, 8], obs, 20, obs_prob=0.3, seed=2,
+                                   obs_dist="poisson", obs_offset=0.2)
+    assert err == 0
+    assert ll == pytest.approx(stats.poisson.logpmf(obs, 0.3 * 8 + 0.2).sum(), rel=1e-12)
 
 
 def test_pfilter_impossible_observation(oracle):
@@ -134,6 +152,39 @@ def _sir_generator(N, beta, gamma):
             Q[k, index[(s, i - 1)]] += rec
         Q[k, k] = -(inf + rec)
     return states, index, Q
+
+
+def _exact_likelihood(N, S0, I0, beta, gamma, times, emission):
+    """Forward algorithm: alpha <- alpha exp(Q elapsed) diag(emission(k, I))."""
+    states, index, Q = _sir_generator(N, beta, gamma)
+    tsp = pfilter_tspan(1.0, times)
+    alpha = np.zeros(len(states))
+    alpha[index[(S0, I0)]] = 1.0
+    for k, (tb, te) in enumerate(tsp):
+        elapsed = 1.0 if math.isnan(tb) else te + 1.0 - tb
+        alpha = alpha @ linalg.expm(Q * elapsed)
+        alpha = alpha * np.array([emission(k, i) for (s, i) in states])
+    return alpha.sum()
+
+
+def test_pfilter_unbiased_likelihood_poisson(oracle):
+    """As below with Poisson(lambda = p I + c) emissions, which give every
+    particle a positive weight for any count."""
+    N, S0, I0, beta, gamma, p, c = 8, 6, 2, 0.9, 0.3, 0.5, 0.3
+    times = [2.0, 4.0, 5.0]
+    obs = np.array([[1], [3], [0]])
+    m = _sir(1, S0, I0, beta, gamma, t0=1.0)
+    exact = _exact_likelihood(N, S0, I0, beta, gamma, times,
+                              lambda k, i: stats.poisson.pmf(obs[k, 0], p * i + c))
+    est = []
+    for seed in range(400):
+        err, ll, _, _ = oracle.pfilter(m, 1, times, obs, 50, obs_prob=p, seed=seed,
+                                       obs_dist="poisson", obs_offset=c)
+        assert err == 0
+        est.append(math.exp(ll))
+    est = np.array(est)
+    se = est.std(ddof=1) / math.sqrt(len(est))
+    assert abs(est.mean() - exact) < 4 * se + 1e-12, (est.mean(), exact, se)
 
 
 def test_pfilter_unbiased_likelihood(oracle):
