@@ -1221,17 +1221,44 @@ __global__ __launch_bounds__(64) void k_pf_weights(const int *U, int Nn, int Nc,
     }
 }
 
+/* Particles whose weights the resampling kernel keeps in LDS (8 bytes
+ * each, 56 KiB in all, within the default dynamic LDS limit); more use a
+ * global buffer. */
+#define SIMINF_PF_LDS 7168
+
+/* Sum of x[0..n) in index order, from 0.0 (the loads run ahead of the
+ * dependent adds). */
+__device__ __forceinline__ double pf_serial_sum(const double *x, int n)
+{
+    double s = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < n; i++)
+        s = s + x[i];
+    return s;
+}
+
 /* Log-sum-exp of the weights, the log-likelihood increment, the ESS and
- * systematic resampling (pfilter.R:455-474), in one workgroup.  The
- * order-dependent sums run in particle order on one lane. */
-__global__ __launch_bounds__(256) void k_pf_resample(double *w, int n, int interval,
-                                                     unsigned long long seed,
+ * systematic resampling (pfilter.R:455-474), in one workgroup, with the
+ * reference's floating-point sequence: the sums run in particle order on
+ * one lane, over weights staged in LDS.  Resampling
+ * (SimInf_particle_filter.c:32-66) walks u_i = u_0 + du + ... + du against
+ * the running sums c_j = w_0 + ... + w_j; since both sequences are
+ * nondecreasing, the walk's ancestor of particle i is the first j with
+ * u_i <= c_j (at most n-1), found here by a binary search per particle
+ * over the stored running sums, after one lane has written them and the
+ * u_i (the same additions, in the same order, as the walk). */
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_pf_resample(double *w, double *c_global, int n,
+                                                     int interval, unsigned long long seed,
                                                      double *loglik, double *ess,
                                                      int *anc, int *err)
 {
+    extern __shared__ double c_lds[];
     __shared__ double s_max[256];
     __shared__ int s_bad[256];
     __shared__ double s_sum;
+    __shared__ int s_res;
+    double *c = LDS ? c_lds : c_global;
     const int tid = (int)threadIdx.x;
     const double NEG_INF = -INFINITY;
     double m = NEG_INF;
@@ -1262,34 +1289,66 @@ __global__ __launch_bounds__(256) void k_pf_resample(double *w, int n, int inter
     }
     const double max_w = s_max[0];
     for (int i = tid; i < n; i += 256)
-        w[i] = siminf_exp(w[i] - max_w);
+        c[i] = siminf_exp(w[i] - max_w);
     __syncthreads();
     if (tid == 0) {
-        double sum = 0.0;
-        for (int i = 0; i < n; i++)
-            sum = sum + w[i];
+        const double sum = pf_serial_sum(c, n);
         *loglik = ((*loglik + max_w) + siminf_log(sum)) - siminf_log((double)n);
         s_sum = sum;
     }
     __syncthreads();
     const double sum = s_sum;
     for (int i = tid; i < n; i += 256)
-        w[i] = w[i] / sum;
+        c[i] = c[i] / sum;
     __syncthreads();
     if (tid == 0) {
-        double s2 = 0.0;
-        for (int i = 0; i < n; i++)
-            s2 = s2 + w[i] * w[i];
-        ess[interval] = 1.0 / s2;
-        const siminf_stream st = siminf_stream_make(seed, SIMINF_DOMAIN_PF,
-                                                    (uint64_t)interval);
-        const int r = siminf_systematic_resampling(w, n, siminf_uniform_at(&st, 0),
-                                                   anc);
-        if (r) {
-            for (int i = 0; i < n; i++)
-                anc[i] = i;
-            *err = r == -1 ? SIMINF_ERR_INVALID_WEIGHT : SIMINF_ERR_WEIGHT_SUM;
+        /* ESS, the validity pass and the running sums; then the u_i. */
+        double s2 = 0.0, cs = 0.0;
+        int r = 0;
+#pragma unroll 8
+        for (int i = 0; i < n; i++) {
+            const double x = c[i];
+            s2 = s2 + x * x;
+            if (!(x >= 0.0) || x == INFINITY)
+                r = -1;
+            cs = cs + x;
+            c[i] = cs;
         }
+        ess[interval] = 1.0 / s2;
+        if (r == 0 && cs <= 0.0)
+            r = -2;
+        if (r == 0) {
+            const siminf_stream st = siminf_stream_make(seed, SIMINF_DOMAIN_PF,
+                                                        (uint64_t)interval);
+            const double du = cs / (double)n;
+            double u = du * siminf_uniform_at(&st, 0);
+            for (int i = 0; i < n; i++) {
+                w[i] = u;
+                u += du;
+            }
+        }
+        s_res = r;
+    }
+    __syncthreads();
+    const int r = s_res;
+    if (r) {
+        for (int i = tid; i < n; i += 256)
+            anc[i] = i;
+        if (tid == 0)
+            *err = r == -1 ? SIMINF_ERR_INVALID_WEIGHT : SIMINF_ERR_WEIGHT_SUM;
+        return;
+    }
+    for (int i = tid; i < n; i += 256) {
+        const double u = w[i];
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (u > c[mid])
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        anc[i] = lo;
     }
 }
 
@@ -1357,6 +1416,8 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
     int *u_cur = B.alloc<int>((size_t)lanes * Nc);
     double *v_cur = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
     double *w = B.alloc<double>((size_t)n_particles);
+    double *c_global =
+        n_particles > SIMINF_PF_LDS ? B.alloc<double>((size_t)n_particles) : nullptr;
     int *anc = B.alloc<int>((size_t)n_particles * n_intervals);
     int *obs_d = B.upload(obs, (size_t)n_intervals * obs_nodes);
     double *ts_d = B.upload(tspan2, (size_t)2 * n_intervals);
@@ -1400,9 +1461,15 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
         hipLaunchKernelGGL(k_pf_weights, dim3(n_particles), dim3(64), 0, P->stream, q.U,
                            Nn, Nc, q.tlen, obs_compartment, obs_dist, obs_p1, obs_p2,
                            obs_d + (size_t)i * obs_nodes, obs_nodes, w);
-        hipLaunchKernelGGL(k_pf_resample, dim3(1), dim3(256), 0, P->stream, w,
-                           n_particles, i, (unsigned long long)seed, ll_d, ess_d,
-                           anc + (size_t)i * n_particles, perr_d + i);
+        if (c_global)
+            hipLaunchKernelGGL(k_pf_resample<false>, dim3(1), dim3(256), 0, P->stream, w,
+                               c_global, n_particles, i, (unsigned long long)seed, ll_d,
+                               ess_d, anc + (size_t)i * n_particles, perr_d + i);
+        else
+            hipLaunchKernelGGL(k_pf_resample<true>, dim3(1), dim3(256),
+                               (size_t)n_particles * sizeof(double), P->stream, w, c_global,
+                               n_particles, i, (unsigned long long)seed, ll_d, ess_d,
+                               anc + (size_t)i * n_particles, perr_d + i);
         hipLaunchKernelGGL(k_pf_gather, dim3(grid), dim3(256), 0, P->stream, q.U, q.V,
                            Nn, Nc, Nd, q.tlen, anc + (size_t)i * n_particles, lanes,
                            u_cur, v_cur);

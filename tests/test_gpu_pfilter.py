@@ -75,6 +75,19 @@ def test_pfilter_seir_builtin(oracle):
     _check(oracle, m, 2, times, obs, 100, seed=5, p=0.1)
 
 
+@pytest.mark.parametrize("P", [1, 7168, 7169, 9000])
+def test_pfilter_particle_counts(oracle, P):
+    """Weights staged in LDS up to 7168 particles, in a global buffer
+    above; one particle resamples itself."""
+    m = _sir(3, seed=6)
+    times = np.arange(1, 20, 3.0)
+    obs = np.random.default_rng(P).poisson(20.0, size=(len(times), 1))
+    err, r = _check(oracle, m, 1, times, obs, P, seed=P, p=0.3, dist="poisson", offset=1.0)
+    assert err == 0
+    if P == 1:
+        assert np.all(r.ancestors == 0)
+
+
 def test_pfilter_invalid_observation(oracle):
     m = _sir(3)
     err, _ = _check(oracle, m, 1, [1, 2], [[0, 0, 0], [10**6, 0, 0]], 16, seed=1)
