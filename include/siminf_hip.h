@@ -71,7 +71,11 @@ typedef enum {
      * of pfilter.R:455-458 and SimInf_particle_filter.c:42-48). */
     SIMINF_ERR_INVALID_OBS_PROCESS  = -101,
     SIMINF_ERR_INVALID_WEIGHT       = -102,
-    SIMINF_ERR_WEIGHT_SUM           = -103
+    SIMINF_ERR_WEIGHT_SUM           = -103,
+    /* Summaries of a plan's trajectory (R-level errors of prevalence()
+     * and trajectory(), R/prevalence.R:233-244). */
+    SIMINF_ERR_EMPTY_TRAJECTORY     = -104,
+    SIMINF_ERR_INVALID_SUMMARY      = -105
 } SimInf_error_code;
 
 /* Plugin signatures (SimInf.h:92-107). */
@@ -242,6 +246,43 @@ int SimInf_hip_pfilter(const SimInf_solver_args *args,
                        double obs_p2, int obs_nodes, const int *obs,
                        unsigned long int seed, double *loglik,
                        double *ess, int *ancestors);
+
+/* On-device summaries of a plan's resident dense U after a successful
+ * SimInf_hip_plan_run; only the summary returns to the host.
+ *
+ * Prevalence (calculate_prevalence, R/prevalence.R:70-110): per node and
+ * U column, cases = the sum of the compartments in cases_mask (bit c =
+ * compartment c) and population = the sum of those in pop_mask, both set
+ * to zero where the condition is false.  level 1: sum(cases) /
+ * sum(population) per column; level 2: #(cases > 0) / #(population > 0)
+ * per column; level 3: cases / population per node and column.  index
+ * holds n_index zero-based nodes (sorted, unique), or is NULL for all
+ * nodes.  out: Nrep*tlen values (levels 1, 2), or n_index x (Nrep*tlen)
+ * column-major (level 3).  The condition is a postfix program of cond_len
+ * (op, arg) pairs, at most SIMINF_COND_MAX, stack depth at most
+ * SIMINF_COND_STACK, ending with one value (nonzero = TRUE); cond_len 0
+ * means no condition. */
+#define SIMINF_COND_MAX 64
+#define SIMINF_COND_STACK 16
+enum {
+    SIMINF_OP_U = 0,        /* push u[arg] */
+    SIMINF_OP_CONST = 1,    /* push arg */
+    SIMINF_OP_ADD, SIMINF_OP_SUB, SIMINF_OP_MUL, SIMINF_OP_DIV, SIMINF_OP_NEG,
+    SIMINF_OP_EQ, SIMINF_OP_NE, SIMINF_OP_LT, SIMINF_OP_LE, SIMINF_OP_GT,
+    SIMINF_OP_GE, SIMINF_OP_AND, SIMINF_OP_OR, SIMINF_OP_NOT
+};
+int SimInf_hip_plan_prevalence(SimInf_hip_plan *plan, unsigned int cases_mask,
+                               unsigned int pop_mask, const int *cond_ops,
+                               const double *cond_args, int cond_len, int level,
+                               const int *index, int n_index, double *out);
+
+/* Trajectory of the compartments in comp_mask (in model order) for the
+ * nodes in index (zero-based, or NULL for all) from the resident U, in
+ * the layout of trajectory(..., format = "matrix")
+ * (SimInf_trajectory.c:566-650): rows node-major, then compartment;
+ * columns the Nrep*tlen U columns; column-major int32. */
+int SimInf_hip_plan_trajectory(SimInf_hip_plan *plan, unsigned int comp_mask,
+                               const int *index, int n_index, int *out);
 
 #ifdef __cplusplus
 }

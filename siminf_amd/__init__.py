@@ -6,8 +6,10 @@ from .model import SimInfModel, SimInfError, make_events, run, trajectory
 from .models import SIR, SEIR, SIS, SISe, SISe3, SIR_mparse
 from .modelparser import mparse
 from .particle_filter import pfilter, PFilterResult
+from .plan import Plan
+from .summaries import SummaryError
 from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
            "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse",
-           "pfilter", "PFilterResult"]
+           "pfilter", "PFilterResult", "Plan", "SummaryError"]

@@ -83,6 +83,8 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_stats",
     "SimInf_hip_plan_kernel_times",
     "SimInf_hip_pfilter",
+    "SimInf_hip_plan_prevalence",
+    "SimInf_hip_plan_trajectory",
     "SimInf_hip_plan_destroy",
 )
 
@@ -135,6 +137,12 @@ def load(path=None):
     lib.SimInf_hip_pfilter.argtypes = [
         args_p, vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int,
         ctypes.c_double, ctypes.c_double, ctypes.c_int, ip, ctypes.c_ulong, dp, dp, ip]
+    lib.SimInf_hip_plan_prevalence.restype = ctypes.c_int
+    lib.SimInf_hip_plan_prevalence.argtypes = [
+        vp, ctypes.c_uint, ctypes.c_uint, ip, dp, ctypes.c_int, ctypes.c_int, ip,
+        ctypes.c_int, dp]
+    lib.SimInf_hip_plan_trajectory.restype = ctypes.c_int
+    lib.SimInf_hip_plan_trajectory.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int, ip]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
     if path == LIB_PATH:

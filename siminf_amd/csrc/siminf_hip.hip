@@ -302,6 +302,7 @@ struct SimInf_hip_plan {
     uint64_t host_err_key;     /* errors found while validating events */
 
     /* Last run. */
+    bool has_run = false;       /* a run completed without error */
     unsigned long long transitions = 0;
     double kernel_ms = 0.0;     /* all kernels of the run */
     double solver_ms = 0.0;     /* the solver kernel(s) */
@@ -832,6 +833,7 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
 {
     if (!P)
         return SIMINF_ERR_INVALID_MODEL;
+    P->has_run = false;
     KParams p = P->p;
     p.seed = (unsigned long long)seed;
 
@@ -972,7 +974,9 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
     P->fill_ms = ms_fill;
     P->fill_cols = fill_cols;
     P->transitions = trans;
-    return key_to_code(std::min<uint64_t>(key, P->host_err_key));
+    const int code = key_to_code(std::min<uint64_t>(key, P->host_err_key));
+    P->has_run = code == 0;
+    return code;
 }
 
 extern "C" int SimInf_hip_plan_download(SimInf_hip_plan *P, int *U, double *V,
@@ -1167,6 +1171,9 @@ extern "C" const char *SimInf_hip_error_message(int code)
     case SIMINF_ERR_INVALID_OBS_PROCESS: return "Invalid observation process.";
     case SIMINF_ERR_INVALID_WEIGHT: return "Invalid weight detected (non-finite or < 0.0).";
     case SIMINF_ERR_WEIGHT_SUM: return "Non-positive sum of weights detected.";
+    case SIMINF_ERR_EMPTY_TRAJECTORY:
+        return "Please run the model first, the trajectory is empty.";
+    case SIMINF_ERR_INVALID_SUMMARY: return "Invalid summary specification.";
     default: return "Unknown error code.";
     }
 }
@@ -1503,4 +1510,286 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
             return perrs[i];
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* On-device summaries of a resident trajectory (R/prevalence.R:24-110, */
+/* SimInf_trajectory.c:566-650).                                        */
+/* ------------------------------------------------------------------ */
+
+/* A validated condition program, passed by value in the kernel
+ * arguments (uniform for all lanes). */
+struct CondProgram {
+    int n;
+    int op[SIMINF_COND_MAX];
+    double arg[SIMINF_COND_MAX];
+};
+
+/* The condition for one node's state.  Values are doubles (exact for the
+ * integer sums and products of compartment counts); comparisons and the
+ * logical operators give 1 or 0, and nonzero counts as TRUE, as R's
+ * coercion of numbers to logicals.  Each lane's stack is a column of an
+ * LDS array. */
+__device__ __forceinline__ bool cond_holds(const CondProgram &c, const int *u,
+                                           double (*stk)[256], int lane)
+{
+    if (c.n == 0)
+        return true;
+    int sp = 0;
+    for (int k = 0; k < c.n; k++) {
+        const int op = c.op[k];
+        if (op == SIMINF_OP_U) {
+            stk[sp++][lane] = (double)u[(int)c.arg[k]];
+            continue;
+        }
+        if (op == SIMINF_OP_CONST) {
+            stk[sp++][lane] = c.arg[k];
+            continue;
+        }
+        if (op == SIMINF_OP_NEG) {
+            stk[sp - 1][lane] = -stk[sp - 1][lane];
+            continue;
+        }
+        if (op == SIMINF_OP_NOT) {
+            stk[sp - 1][lane] = stk[sp - 1][lane] == 0.0 ? 1.0 : 0.0;
+            continue;
+        }
+        sp--;
+        const double b = stk[sp][lane];
+        const double a = stk[sp - 1][lane];
+        double r;
+        switch (op) {
+        case SIMINF_OP_ADD: r = a + b; break;
+        case SIMINF_OP_SUB: r = a - b; break;
+        case SIMINF_OP_MUL: r = a * b; break;
+        case SIMINF_OP_DIV: r = a / b; break;
+        case SIMINF_OP_EQ: r = a == b; break;
+        case SIMINF_OP_NE: r = a != b; break;
+        case SIMINF_OP_LT: r = a < b; break;
+        case SIMINF_OP_LE: r = a <= b; break;
+        case SIMINF_OP_GT: r = a > b; break;
+        case SIMINF_OP_GE: r = a >= b; break;
+        case SIMINF_OP_AND: r = (a != 0.0) && (b != 0.0); break;
+        default: r = (a != 0.0) || (b != 0.0); break;
+        }
+        stk[sp - 1][lane] = r;
+    }
+    return stk[0][lane] != 0.0;
+}
+
+/* Cases and population of one node's state. */
+__device__ __forceinline__ void node_counts(const int *u, int Nc, unsigned cases_mask,
+                                            unsigned pop_mask, long long &x, long long &y)
+{
+    x = 0;
+    y = 0;
+    for (int c = 0; c < Nc; c++) {
+        const int v = u[c];
+        if ((cases_mask >> c) & 1u)
+            x += v;
+        if ((pop_mask >> c) & 1u)
+            y += v;
+    }
+}
+
+/* Levels 1 and 2: one workgroup per U column; exact integer sums. */
+__global__ __launch_bounds__(256) void k_prevalence_cols(const int *U, int Nn, int Nc,
+                                                         unsigned cases_mask,
+                                                         unsigned pop_mask, CondProgram cond,
+                                                         int level, const int *index,
+                                                         int n_sel, double *out)
+{
+    __shared__ double stk[SIMINF_COND_STACK][256];
+    __shared__ long long sa[256], sb[256];
+    const int lane = (int)threadIdx.x;
+    const long long col = blockIdx.x;
+    long long a = 0, b = 0;
+    for (int k = lane; k < n_sel; k += 256) {
+        const int node = index ? index[k] : k;
+        const int *u = U + (col * Nn + node) * (long long)Nc;
+        long long x, y;
+        node_counts(u, Nc, cases_mask, pop_mask, x, y);
+        if (!cond_holds(cond, u, stk, lane))
+            x = y = 0;
+        if (level == 1) {
+            a += x;
+            b += y;
+        } else {
+            a += x > 0;
+            b += y > 0;
+        }
+    }
+    sa[lane] = a;
+    sb[lane] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (lane < h) {
+            sa[lane] += sa[lane + h];
+            sb[lane] += sb[lane + h];
+        }
+        __syncthreads();
+    }
+    if (lane == 0)
+        out[col] = (double)sa[0] / (double)sb[0];
+}
+
+/* Level 3: one lane per (selected node, U column). */
+__global__ __launch_bounds__(256) void k_prevalence_nodes(const int *U, int Nn, int Nc,
+                                                          unsigned cases_mask,
+                                                          unsigned pop_mask,
+                                                          CondProgram cond,
+                                                          const int *index, int n_sel,
+                                                          long long total, double *out)
+{
+    __shared__ double stk[SIMINF_COND_STACK][256];
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= total)
+        return;
+    const long long col = g / n_sel;
+    const int k = (int)(g - col * n_sel);
+    const int node = index ? index[k] : k;
+    const int *u = U + (col * Nn + node) * (long long)Nc;
+    long long x, y;
+    node_counts(u, Nc, cases_mask, pop_mask, x, y);
+    if (!cond_holds(cond, u, stk, (int)threadIdx.x))
+        x = y = 0;
+    out[g] = (double)x / (double)y;
+}
+
+/* Selected compartments of selected nodes, one lane per (node, column). */
+__global__ __launch_bounds__(256) void k_trajectory(const int *U, int Nn, int Nc,
+                                                    unsigned mask, int ncomp,
+                                                    const int *index, int n_sel,
+                                                    long long total, int *out)
+{
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= total)
+        return;
+    const long long col = g / n_sel;
+    const int k = (int)(g - col * n_sel);
+    const int node = index ? index[k] : k;
+    const int *u = U + (col * Nn + node) * (long long)Nc;
+    int *o = out + g * ncomp;
+    int j = 0;
+    for (int c = 0; c < Nc; c++)
+        if ((mask >> c) & 1u)
+            o[j++] = u[c];
+}
+
+static int summary_checks(const SimInf_hip_plan *P, const int *index, int n_index)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (!P->has_run)
+        return SIMINF_ERR_EMPTY_TRAJECTORY;
+    if (!P->dense_U)
+        return SIMINF_ERR_INVALID_SUMMARY;
+    if (index) {
+        if (n_index < 1)
+            return SIMINF_ERR_INVALID_SUMMARY;
+        for (int k = 0; k < n_index; k++)
+            if (index[k] < 0 || index[k] >= P->Nn || (k > 0 && index[k] <= index[k - 1]))
+                return SIMINF_ERR_INVALID_SUMMARY;
+    }
+    return 0;
+}
+
+static unsigned compartment_bits(int Nc)
+{
+    return Nc >= 32 ? 0xffffffffu : ((1u << Nc) - 1u);
+}
+
+extern "C" int SimInf_hip_plan_prevalence(SimInf_hip_plan *P, unsigned int cases_mask,
+                                          unsigned int pop_mask, const int *cond_ops,
+                                          const double *cond_args, int cond_len, int level,
+                                          const int *index, int n_index, double *out)
+{
+    int err = summary_checks(P, index, n_index);
+    if (err)
+        return err;
+    const unsigned all = compartment_bits(P->Nc);
+    if (!out || level < 1 || level > 3 || !cases_mask || !pop_mask ||
+        (cases_mask & ~all) || (pop_mask & ~all) || cond_len < 0 ||
+        cond_len > SIMINF_COND_MAX || (cond_len && (!cond_ops || !cond_args)))
+        return SIMINF_ERR_INVALID_SUMMARY;
+    CondProgram cond;
+    memset(&cond, 0, sizeof(cond));
+    cond.n = cond_len;
+    int depth = 0;
+    for (int k = 0; k < cond_len; k++) {
+        const int op = cond_ops[k];
+        const double arg = cond_args[k];
+        if (op == SIMINF_OP_U) {
+            if (!(arg >= 0.0 && arg < (double)P->Nc && arg == std::floor(arg)))
+                return SIMINF_ERR_INVALID_SUMMARY;
+            depth++;
+        } else if (op == SIMINF_OP_CONST) {
+            depth++;
+        } else if (op == SIMINF_OP_NEG || op == SIMINF_OP_NOT) {
+            if (depth < 1)
+                return SIMINF_ERR_INVALID_SUMMARY;
+        } else if (op >= SIMINF_OP_ADD && op <= SIMINF_OP_OR) {
+            if (depth < 2)
+                return SIMINF_ERR_INVALID_SUMMARY;
+            depth--;
+        } else {
+            return SIMINF_ERR_INVALID_SUMMARY;
+        }
+        if (depth > SIMINF_COND_STACK)
+            return SIMINF_ERR_INVALID_SUMMARY;
+        cond.op[k] = op;
+        cond.arg[k] = arg;
+    }
+    if (cond_len && depth != 1)
+        return SIMINF_ERR_INVALID_SUMMARY;
+
+    const int n_sel = index ? n_index : P->Nn;
+    const long long ncols = (long long)P->Nrep * P->tlen;
+    const size_t nout = level == 3 ? (size_t)n_sel * (size_t)ncols : (size_t)ncols;
+    DeviceBuffers T;
+    T.stream = P->stream;
+    T.pooled = P->buf.pooled;
+    const int *idx_d = index ? T.upload(index, (size_t)n_index) : nullptr;
+    double *out_d = T.alloc<double>(nout);
+    if (T.status != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    if (level < 3)
+        hipLaunchKernelGGL(k_prevalence_cols, dim3((unsigned)ncols), dim3(256), 0, P->stream,
+                           P->p.U, P->Nn, P->Nc, cases_mask, pop_mask, cond, level, idx_d,
+                           n_sel, out_d);
+    else
+        hipLaunchKernelGGL(k_prevalence_nodes, dim3((unsigned)((nout + 255) / 256)), dim3(256),
+                           0, P->stream, P->p.U, P->Nn, P->Nc, cases_mask, pop_mask, cond,
+                           idx_d, n_sel, (long long)nout, out_d);
+    if (hipGetLastError() != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    T.copy(out, out_d, nout * sizeof(double), hipMemcpyDeviceToHost);
+    return T.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
+}
+
+extern "C" int SimInf_hip_plan_trajectory(SimInf_hip_plan *P, unsigned int comp_mask,
+                                          const int *index, int n_index, int *out)
+{
+    int err = summary_checks(P, index, n_index);
+    if (err)
+        return err;
+    if (!out || !comp_mask || (comp_mask & ~compartment_bits(P->Nc)))
+        return SIMINF_ERR_INVALID_SUMMARY;
+    const int ncomp = __builtin_popcount(comp_mask);
+    const int n_sel = index ? n_index : P->Nn;
+    const long long total = (long long)n_sel * P->Nrep * P->tlen;
+    DeviceBuffers T;
+    T.stream = P->stream;
+    T.pooled = P->buf.pooled;
+    const int *idx_d = index ? T.upload(index, (size_t)n_index) : nullptr;
+    int *out_d = T.alloc<int>((size_t)total * ncomp);
+    if (T.status != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    hipLaunchKernelGGL(k_trajectory, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       P->stream, P->p.U, P->Nn, P->Nc, comp_mask, ncomp, idx_d, n_sel,
+                       total, out_d);
+    if (hipGetLastError() != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    T.copy(out, out_d, (size_t)total * ncomp * sizeof(int), hipMemcpyDeviceToHost);
+    return T.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
 }

@@ -77,7 +77,7 @@ class SimInfModel:
 
     def __init__(self, plugin, G, S, tspan, u0, ldata=None, gdata=None,
                  v0=None, E=None, N=None, events=None, replicates=1,
-                 U_sparse=None, V_sparse=None):
+                 U_sparse=None, V_sparse=None, compartments=None):
         self.plugin = plugin
         self.G = _csc(G)
         self.S = _csc(S)
@@ -90,6 +90,12 @@ class SimInfModel:
         self.Nt = self.S.shape[1]
         if u0.shape[0] != self.Nc:
             raise ValueError("u0 must have one row per compartment")
+        # The rownames of S: the compartment names.
+        if compartments is None:
+            compartments = ["C%d" % (c + 1) for c in range(self.Nc)]
+        if len(compartments) != self.Nc:
+            raise ValueError("one name per compartment")
+        self.compartments = list(compartments)
         self._u0 = np.asfortranarray(u0.astype(np.int32))
         ncols = self._u0.shape[1]
         if ncols % self.replicates:

@@ -599,7 +599,8 @@ def mparse(transitions, compartments, ldata=None, gdata=None, u0=None, v0=None,
     model = SimInfModel(GeneratedPlugin(parsed), parsed.G, parsed.S, tspan, u,
                         ldata=ld, gdata=gvec, v0=v if v.shape[0] else None,
                         E=E if E is not None else np.zeros((len(compartments), 0)),
-                        N=N, events=events, replicates=replicates)
+                        N=N, events=events, replicates=replicates,
+                        compartments=compartments)
     model.C_code = parsed.C_code
     model.G_rownames = parsed.G_rownames
     return model

@@ -76,7 +76,8 @@ def SIR(u0, tspan, events=None, beta=None, gamma=None, replicates=1):
     S = np.array([[-1, 0], [1, -1], [0, 1]], float)
     u, _ = _tile_reps(u, None, replicates)
     return SimInfModel("SIR", G, S, tspan, u, ldata=ldata, E=select_matrix_SIR(),
-                       events=events, replicates=replicates)
+                       events=events, replicates=replicates,
+                       compartments=("S", "I", "R"))
 
 
 def SEIR(u0, tspan, events=None, beta=None, epsilon=None, gamma=None,
@@ -89,7 +90,8 @@ def SEIR(u0, tspan, events=None, beta=None, epsilon=None, gamma=None,
     u, _ = _tile_reps(u, None, replicates)
     return SimInfModel("SEIR", G, S, tspan, u, ldata=ldata,
                        E=select_matrix_SEIR(), events=events,
-                       replicates=replicates)
+                       replicates=replicates,
+                       compartments=("S", "E", "I", "R"))
 
 
 def SIS(u0, tspan, events=None, beta=None, gamma=None, replicates=1):
@@ -100,7 +102,8 @@ def SIS(u0, tspan, events=None, beta=None, gamma=None, replicates=1):
     S = np.array([[-1, 1], [1, -1]], float)
     u, _ = _tile_reps(u, None, replicates)
     return SimInfModel("SIS", G, S, tspan, u, ldata=ldata, E=select_matrix_SIS(),
-                       events=events, replicates=replicates)
+                       events=events, replicates=replicates,
+                       compartments=("S", "I"))
 
 
 def SISe(u0, tspan, events=None, phi=0.0, upsilon=None, gamma=None, alpha=None,
@@ -118,7 +121,8 @@ def SISe(u0, tspan, events=None, phi=0.0, upsilon=None, gamma=None, alpha=None,
     u, v0 = _tile_reps(u, v0, replicates)
     return SimInfModel("SISe", G, S, tspan, u, ldata=ldata, gdata=gdata, v0=v0,
                        E=select_matrix_SIS(), events=events,
-                       replicates=replicates)
+                       replicates=replicates,
+                       compartments=("S", "I"))
 
 
 def SISe3(u0, tspan, events=None, phi=0.0, upsilon_1=None, upsilon_2=None,
@@ -142,7 +146,8 @@ def SISe3(u0, tspan, events=None, phi=0.0, upsilon_1=None, upsilon_2=None,
     u, v0 = _tile_reps(u, v0, replicates)
     return SimInfModel("SISe3", G, S, tspan, u, ldata=ldata, gdata=gdata, v0=v0,
                        E=select_matrix_SISe3(), N=shift_matrix_SISe3(),
-                       events=events, replicates=replicates)
+                       events=events, replicates=replicates,
+                       compartments=("S_1", "I_1", "S_2", "I_2", "S_3", "I_3"))
 
 
 def SIR_mparse(u0, tspan, beta, gamma, events=None, E=None, N=None,
@@ -159,4 +164,5 @@ def SIR_mparse(u0, tspan, beta, gamma, events=None, E=None, N=None,
     u, _ = _tile_reps(u, None, replicates)
     return SimInfModel("SIR_mparse", G, S, tspan, u, gdata=[beta, gamma],
                        E=E if E is not None else np.zeros((3, 0)), N=N,
-                       events=events, replicates=replicates)
+                       events=events, replicates=replicates,
+                       compartments=("S", "I", "R"))

@@ -1,0 +1,114 @@
+"""A model resident on the GPU: inputs uploaded once, runs with new seeds,
+and summaries of the trajectory computed where it lies.
+
+`Plan` wraps the `SimInf_hip_plan_*` entry points (no reference
+counterpart; they serve callers that run one model many times).  Its
+`prevalence` and `trajectory` mirror the reference's prevalence() and
+trajectory(format = "matrix") (R/prevalence.R:70-110,
+SimInf_trajectory.c:566-650) on the device-resident U, so a summary costs
+a kernel and a copy of the summary, never a copy of U.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native, summaries
+from .model import SimInfError, _attach, _raise, device_plugin
+
+
+class Plan:
+    def __init__(self, model, solver="ssm", lib=None):
+        self.lib = lib or _native.load()
+        if solver not in ("ssm", "aem"):
+            _raise(self.lib, -12)
+        self.model = model
+        self._bundle = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1, seed=1)
+        self._handle = ctypes.c_void_p()
+        err = self.lib.SimInf_hip_plan_create(ctypes.byref(self._handle),
+                                              ctypes.byref(self._bundle.args),
+                                              ctypes.c_void_p(device_plugin(model, self.lib)))
+        if err:
+            _raise(self.lib, err)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        if self._handle:
+            self.lib.SimInf_hip_plan_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, err):
+        if err:
+            _raise(self.lib, err)
+
+    def run(self, seed):
+        """One run with this seed; returns (transitions, node_days,
+        kernel_ms)."""
+        self._check(self.lib.SimInf_hip_plan_run(self._handle,
+                                                 int(seed) & 0xFFFFFFFFFFFFFFFF))
+        tr, nd, ms = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        self.lib.SimInf_hip_plan_stats(self._handle, ctypes.byref(tr), ctypes.byref(nd),
+                                       ctypes.byref(ms))
+        return tr.value, nd.value, ms.value
+
+    def download(self):
+        """Copy the last run's U/V into the model (as run() leaves them)."""
+        b = self._bundle
+        ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        self._check(self.lib.SimInf_hip_plan_download(
+            self._handle,
+            None if b.U is None else _native.ptr(b.U, ctypes.c_int),
+            None if b.V is None else _native.ptr(b.V, ctypes.c_double),
+            None if b.prU is None else b.prU.ctypes.data_as(dp),
+            None if b.prV is None else b.prV.ctypes.data_as(dp)))
+        _attach(self.model, b)
+        return self.model
+
+    def _columns(self):
+        return self.model.replicates * len(self.model.tspan)
+
+    def prevalence(self, formula, level=1, index=None):
+        """prevalence(model, formula, level, index, format = "matrix") of the
+        last run, computed on the device.  index is one-based like the
+        reference's.  Returns Nrep*tlen values (levels 1, 2) or a
+        len(index) x Nrep*tlen matrix (level 3)."""
+        cases, pop, ops, args, level = summaries.prevalence_spec(
+            formula, self.model.compartments, level)
+        idx = summaries.node_index(index, self.model.Nn)
+        n_sel = self.model.Nn if idx is None else len(idx)
+        ncols = self._columns()
+        out = np.zeros((n_sel, ncols) if level == 3 else ncols, dtype=np.float64, order="F")
+        ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        self._check(self.lib.SimInf_hip_plan_prevalence(
+            self._handle, cases, pop, ops.ctypes.data_as(ip), args.ctypes.data_as(dp),
+            len(ops), level, None if idx is None else idx.ctypes.data_as(ip),
+            0 if idx is None else len(idx), out.ctypes.data_as(dp)))
+        return out
+
+    def trajectory(self, compartments=None, index=None):
+        """trajectory(model, compartments, index, format = "matrix") of the
+        last run, gathered on the device: rows node-major then compartment
+        (model order), one column per U column."""
+        m = summaries.trajectory_mask(compartments, self.model.compartments)
+        idx = summaries.node_index(index, self.model.Nn)
+        n_sel = self.model.Nn if idx is None else len(idx)
+        ncomp = bin(m).count("1")
+        out = np.zeros((n_sel * ncomp, self._columns()), dtype=np.int32, order="F")
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._check(self.lib.SimInf_hip_plan_trajectory(
+            self._handle, m, None if idx is None else idx.ctypes.data_as(ip),
+            0 if idx is None else len(idx), out.ctypes.data_as(ip)))
+        return out
+
+
+__all__ = ["Plan", "SimInfError"]

@@ -177,7 +177,8 @@ def host_pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5,
     for i, (tb, te) in enumerate(tspan2):
         ts = np.array([te]) if np.isnan(tb) else np.array([tb, te])
         m = SimInfModel(model.plugin, model.G, model.S, ts, u, ldata=model.ldata,
-                        gdata=model.gdata, v0=v, E=model.E, replicates=P)
+                        gdata=model.gdata, v0=v, E=model.E, replicates=P,
+                        compartments=model.compartments)
         run_model(m, seed=int(rng.integers(0, 2**63 - 1)), lib=lib)
         tlen = len(ts)
         last = m.U.reshape(Nn * Nc, P, tlen)[:, :, -1]          # row node*Nc+c

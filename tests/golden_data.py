@@ -31,7 +31,7 @@ def seir_enter_model():
     m = si.SEIR(u0=u0, tspan=np.arange(0, 10.0), events=None, beta=0, epsilon=0,
                 gamma=0)
     return si.SimInfModel("SEIR", m.G, m.S, m.tspan, m.u0, ldata=m.ldata,
-                          E=np.eye(4), events=ev)
+                          E=np.eye(4), events=ev, compartments=m.compartments)
 
 
 def seir_enter_expected():
