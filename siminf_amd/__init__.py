@@ -7,9 +7,10 @@ from .models import SIR, SEIR, SIS, SISe, SISe3, SIR_mparse
 from .modelparser import mparse
 from .particle_filter import pfilter, PFilterResult
 from .plan import Plan
+from .punchcard import punchcard
 from .summaries import SummaryError
 from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
            "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse",
-           "pfilter", "PFilterResult", "Plan", "SummaryError"]
+           "pfilter", "PFilterResult", "Plan", "SummaryError", "punchcard"]

@@ -77,7 +77,7 @@ class SimInfModel:
 
     def __init__(self, plugin, G, S, tspan, u0, ldata=None, gdata=None,
                  v0=None, E=None, N=None, events=None, replicates=1,
-                 U_sparse=None, V_sparse=None, compartments=None):
+                 U_sparse=None, V_sparse=None, compartments=None, v0_names=None):
         self.plugin = plugin
         self.G = _csc(G)
         self.S = _csc(S)
@@ -108,6 +108,12 @@ class SimInfModel:
             v0 = v0.reshape(1, -1)
         self._v0 = np.asfortranarray(v0)
         self.Nd = self._v0.shape[0]
+        # The rownames of v0.
+        if v0_names is None:
+            v0_names = ["V%d" % (d + 1) for d in range(self.Nd)]
+        if len(v0_names) != self.Nd:
+            raise ValueError("one name per continuous state")
+        self.v0_names = list(v0_names)
         if ldata is None:
             ldata = np.zeros((0, self.Nn))
         ldata = np.asarray(ldata, dtype=np.float64)

@@ -122,7 +122,7 @@ def SISe(u0, tspan, events=None, phi=0.0, upsilon=None, gamma=None, alpha=None,
     return SimInfModel("SISe", G, S, tspan, u, ldata=ldata, gdata=gdata, v0=v0,
                        E=select_matrix_SIS(), events=events,
                        replicates=replicates,
-                       compartments=("S", "I"))
+                       compartments=("S", "I"), v0_names=("phi",))
 
 
 def SISe3(u0, tspan, events=None, phi=0.0, upsilon_1=None, upsilon_2=None,
@@ -147,7 +147,8 @@ def SISe3(u0, tspan, events=None, phi=0.0, upsilon_1=None, upsilon_2=None,
     return SimInfModel("SISe3", G, S, tspan, u, ldata=ldata, gdata=gdata, v0=v0,
                        E=select_matrix_SISe3(), N=shift_matrix_SISe3(),
                        events=events, replicates=replicates,
-                       compartments=("S_1", "I_1", "S_2", "I_2", "S_3", "I_3"))
+                       compartments=("S_1", "I_1", "S_2", "I_2", "S_3", "I_3"),
+                       v0_names=("phi",))
 
 
 def SIR_mparse(u0, tspan, beta, gamma, events=None, E=None, N=None,
