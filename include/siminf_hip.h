@@ -224,8 +224,10 @@ void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
  * between observation intervals (pfilter_internal, R/pfilter.R:315-499;
  * systematic resampling, SimInf_particle_filter.c:32-66).
  *
- * args describes one particle (Nrep 1, no scheduled events); the filter
- * runs n_particles replicates of it.  Interval i runs from tspan2[2i] to
+ * args describes one particle (Nrep 1); the filter runs n_particles
+ * replicates of it.  Interval i runs the scheduled events due after the
+ * end of interval i-1 and at or before its own end (pfilter_events,
+ * R/pfilter.R:200-220), the first interval also those before it.  Interval i runs from tspan2[2i] to
  * tspan2[2i+1], or over the one time point tspan2[2i+1] when tspan2[2i]
  * is NaN (pfilter_tspan, R/pfilter.R:177-193).  The observation process
  * (the binomial and poisson forms of pfilter's formula, R/pfilter.R:279-295)

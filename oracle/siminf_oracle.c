@@ -347,6 +347,11 @@ static int oracle_apply_event(
     return 0;
 }
 
+/* Stream identity of the E2 events of a run over part of an event list
+ * (a filter interval's events): the full list's length and the part's
+ * offset in it; -1 means the run's own list. */
+static int g_ev_total = -1, g_ev_offset = 0;
+
 /* Process the events of replicate rep due at tt: first the E1 events in
  * list order (each drawing from its node's stream), then the E2 events
  * in list order (each drawing from its own stream).  *cursor1 / *cursor2
@@ -383,8 +388,9 @@ static int oracle_process_events(
         if (a->time[k] > tt)
             break;
         rng = siminf_stream_make(a->seed, SIMINF_DOMAIN_E2,
-                                 (uint64_t)rep * (uint64_t)a->len +
-                                 (uint64_t)k);
+                                 (uint64_t)rep *
+                                     (uint64_t)(g_ev_total >= 0 ? g_ev_total : a->len) +
+                                 (uint64_t)(g_ev_offset + k));
         err = oracle_apply_event(c, k, u, update_node, &rng);
         if (err) {
             *cursor2 = k;
@@ -1099,11 +1105,11 @@ int oracle_sample(int enter, const int *irE, const int *jcE, const double *prE,
 /* ---------------------------------------------------------------- */
 
 /* The device filter's computation on the host: per interval one mssm run
- * of all particles from their ancestors' states (pfilter.R:385-430), the
- * weights (the sum of log dbinom over the observed nodes, added in the
- * device's order: node k into partial sum k mod 64, then the partial sums
- * in order), log-sum-exp, ESS and systematic resampling (pfilter.R:455-
- * 474).  a describes one particle, with tr_fun/pts_fun set.  Arguments as
+ * of all particles from their ancestors' states over the interval's
+ * events (pfilter.R:373-430), the weights (the sum of the observations'
+ * log-probabilities over the observed nodes, added in the device's
+ * order: node k into partial sum k mod 64, then the partial sums in
+ * order), log-sum-exp, ESS and systematic resampling (pfilter.R:455-474).  a describes one particle, with tr_fun/pts_fun set.  Arguments as
  * SimInf_hip_pfilter. */
 int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
                    const double *tspan2, int obs_c, int obs_dist, double obs_p1,
@@ -1130,10 +1136,32 @@ int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
         if (Nd > 0)
             memcpy(&v_cur[(size_t)p * Nn * Nd], a->v0, (size_t)Nn * Nd * sizeof(double));
     }
+    /* The events of each interval (SimInf_split_events,
+     * SimInf_particle_filter.c:80-118): those due after the previous
+     * interval's end and at or before this one's, in list order. */
+    int ev_first = 0;
     for (int i = 0; i < n_int && !err; i++) {
         SimInf_solver_args b = *a;
         const double tb = tspan2[2 * i];
         const int tlen = isnan(tb) ? 1 : 2;
+        const int t_end = (int)tspan2[2 * i + 1];
+        int ev_n = 0;
+        while (ev_first + ev_n < a->len && a->time[ev_first + ev_n] <= t_end)
+            ev_n++;
+        b.len = ev_n;
+        if (a->len > 0) {
+            b.event = a->event + ev_first;
+            b.time = a->time + ev_first;
+            b.node = a->node + ev_first;
+            b.dest = a->dest + ev_first;
+            b.n = a->n + ev_first;
+            b.proportion = a->proportion + ev_first;
+            b.select = a->select + ev_first;
+            b.shift = a->shift + ev_first;
+        }
+        g_ev_total = a->len;
+        g_ev_offset = ev_first;
+        ev_first += ev_n;
         b.Nrep = P;
         b.solver = 0;
         b.tspan = isnan(tb) ? &tspan2[2 * i + 1] : &tspan2[2 * i];
@@ -1144,6 +1172,8 @@ int oracle_pfilter(const SimInf_solver_args *a, int P, int n_int,
         b.V = V;
         b.seed = siminf_interval_seed(seed, i);
         err = oracle_run(&b);
+        g_ev_total = -1;
+        g_ev_offset = 0;
         if (err)
             break;
         for (int p = 0; p < P; p++) {

@@ -118,6 +118,11 @@ struct KParams {
     const int *e2_tn_ptr, *e2_tn;
     int ndays;
     int Nselect;
+    /* A run over part of the plan's days (the intervals of a particle
+     * filter): the plan's day index of the run's first day, E1 events at
+     * or before time ev_floor skipped, E2 days at or before plan day
+     * e2_floor skipped (INT_MIN / -1 / 0 for a whole run). */
+    int day0, ev_floor, e2_floor;
 
     /* LDS layout of the coupled kernel (byte offsets into the dynamic
      * LDS block) for one day's E2 work, used when the day has at most

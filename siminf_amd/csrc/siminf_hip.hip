@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -681,6 +682,9 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     p.tt0 = tt0;
     p.next0 = std::floor(tt0) + 1.0;
     p.ndays = ndays;
+    p.day0 = 0;
+    p.ev_floor = INT_MIN;
+    p.e2_floor = -1;
 
     if (!filter_plan) {
         p.u0 = B.upload(a->u0, (size_t)lanes * Nc);
@@ -1387,7 +1391,7 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
                                   double *loglik, double *ess, int *ancestors)
 {
     if (!a || !model || !tspan2 || !obs || !loglik || !ess || n_particles < 1 ||
-        n_intervals < 1 || a->Nrep > 1 || a->len != 0 || obs_compartment < 0 ||
+        n_intervals < 1 || a->Nrep > 1 || a->len < 0 || obs_compartment < 0 ||
         obs_compartment >= a->Nc || (obs_nodes != 1 && obs_nodes != a->Nn) ||
         (obs_dist != SIMINF_OBS_BINOMIAL && obs_dist != SIMINF_OBS_POISSON) ||
         (obs_dist == SIMINF_OBS_BINOMIAL && !(obs_p1 >= 0.0 && obs_p1 <= 1.0)) ||
@@ -1399,9 +1403,16 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
             return SIMINF_ERR_INVALID_MODEL;
     }
 
-    /* A plan for n_particles replicates of the particle, two columns. */
+    /* A plan for n_particles replicates of the particle, two columns,
+     * whose days span all intervals: the scheduled events are laid out
+     * once, and each interval runs its own days and only the events due
+     * after the previous interval's end (pfilter_events and
+     * SimInf_split_events, R/pfilter.R:200-220, SimInf_particle_filter.c:
+     * 80-118). */
     SimInf_solver_args b = *a;
-    double ts0[2] = {0.0, 1.0};
+    const double t_first = std::isnan(tspan2[0]) ? tspan2[1] : tspan2[0];
+    const double t_last = tspan2[2 * (n_intervals - 1) + 1];
+    double ts0[2] = {t_first, std::max(t_last, t_first + 1.0)};
     int dummy_u = 0;
     double dummy_v = 0.0;
     b.Nrep = n_particles;
@@ -1416,6 +1427,11 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
     int err = plan_create_impl(&P, &b, model, true);
     if (err)
         return err;
+    if (P->host_err_key != NO_ERROR_KEY) {
+        err = key_to_code(P->host_err_key);
+        SimInf_hip_plan_destroy(P);
+        return err;
+    }
 
     const int Nn = a->Nn, Nc = a->Nc, Nd = a->Nd;
     const long long lanes = (long long)Nn * n_particles;
@@ -1464,7 +1480,23 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
         q.next0 = std::floor(q.tt0) + 1.0;
         q.seed = siminf_interval_seed((uint64_t)seed, i);
         q.err_key = keys_d + i;
-        model->ssm_nodes(q, false, P->canon, grid, 256, P->stream);
+        q.day0 = (int)(std::floor(q.tt0) - std::floor(t_first));
+        if (i > 0) {
+            const double t_prev = tspan2[2 * (i - 1) + 1];
+            q.ev_floor = (int)t_prev;
+            q.e2_floor = (int)(t_prev - std::floor(t_first));
+        }
+        if (P->kernel == K_SSM_NODES) {
+            model->ssm_nodes(q, P->has_e1, P->canon, grid, 256, P->stream);
+        } else if (P->kernel == K_COUPLED_REPS) {
+            model->coupled(q, P->canon, false, P->grid, P->block, P->lds, P->stream, nullptr);
+        } else {
+            for (int r = 0; r < n_particles; r++) {
+                q.rep0 = r;
+                model->coupled(q, P->canon, true, P->grid, P->block, P->lds, P->stream,
+                               nullptr);
+            }
+        }
         hipLaunchKernelGGL(k_pf_weights, dim3(n_particles), dim3(64), 0, P->stream, q.U,
                            Nn, Nc, q.tlen, obs_compartment, obs_dist, obs_p1, obs_p2,
                            obs_d + (size_t)i * obs_nodes, obs_nodes, w);

@@ -45,6 +45,8 @@ struct E1Cursor {
     {
         cur = p.e1_ptr[node];
         end = p.e1_ptr[node + 1];
+        while (cur < end && p.e1_rec[2 * cur].x <= p.ev_floor)
+            cur++;
         fetch(p);
     }
     __device__ __forceinline__ void fetch(const KParams &p)
@@ -425,13 +427,15 @@ template <int NC>
 __device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int rep, int day,
                       int *uw, int *updw, unsigned char *lds)
 {
-    if (day >= p.ndays)
+    /* The plan's day (tables) and the run's day (error order). */
+    const int pd = day + p.day0;
+    if (pd >= p.ndays || pd <= p.e2_floor)
         return 0;
-    const int l0 = p.e2_day[day], nlv = p.e2_day[day + 1] - l0;
+    const int l0 = p.e2_day[pd], nlv = p.e2_day[pd + 1] - l0;
     if (nlv == 0)
         return 0;
     const int i0 = p.e2_lvl[l0], nev = p.e2_lvl[l0 + nlv] - i0;
-    const int t0 = p.e2_tn_ptr[day], nt = p.e2_tn_ptr[day + 1] - t0;
+    const int t0 = p.e2_tn_ptr[pd], nt = p.e2_tn_ptr[pd + 1] - t0;
     const int4 *rec = p.e2_rec + 2 * (long long)i0;
     const double *prop = p.e2_prop + i0;
     const int *lvl = p.e2_lvl + l0;

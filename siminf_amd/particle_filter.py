@@ -55,8 +55,6 @@ def prepare(model, obs_compartment, times, obs):
     if model.replicates != 1:
         raise ValueError("the filter runs its particles as replicates of a "
                          "model with one replicate")
-    if len(model.events["event"]):
-        raise ValueError("scheduled events are not supported by the device filter")
     if not 0 <= int(obs_compartment) < model.Nc:
         raise ValueError("obs_compartment out of range")
     tspan2 = pfilter_tspan(model.tspan[0], times)

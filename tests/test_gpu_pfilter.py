@@ -92,3 +92,31 @@ def test_pfilter_invalid_observation(oracle):
     m = _sir(3)
     err, _ = _check(oracle, m, 1, [1, 2], [[0, 0, 0], [10**6, 0, 0]], 16, seed=1)
     assert err == -101
+
+
+def test_pfilter_interval_events(oracle):
+    """Scheduled events split per interval: enter events (E1, k_ssm_nodes)
+    and external transfers (E2, one workgroup per particle), with
+    individual sampling drawing from the streams."""
+    from siminf_amd import workloads as W
+    rng = np.random.default_rng(3)
+    m = _sir(20, seed=5)
+    rows = [(1, int(t), int(nd), 0, int(rng.integers(1, 4)), 0.0, 4, 0)
+            for t, nd in zip(rng.integers(1, 40, 60), rng.integers(1, 21, 60))]
+    rows.sort(key=lambda r: r[1])
+    ev = {k: np.array([r[i] for r in rows]) for i, k in enumerate(
+        ("event", "time", "node", "dest", "n", "proportion", "select", "shift"))}
+    me = si.SIR_mparse(u0={"S": m.u0[0], "I": m.u0[1], "R": m.u0[2]},
+                       tspan=np.array([1.0, 2.0]), beta=0.25, gamma=0.1, events=ev,
+                       E=si.models.select_matrix_SIR())
+    times = np.array([3, 4, 8, 15, 22, 29, 36, 40.0])
+    obs = rng.poisson(20.0, size=(len(times), 1))
+    err, _ = _check(oracle, me, 1, times, obs, 96, seed=21, p=0.5, dist="poisson",
+                    offset=1.0)
+    assert err == 0
+    h = W.sise3_herds(Nn=40, days=50, seed=6)      # the filter reads tspan[0] only
+    times = np.arange(5, 51, 5.0)
+    obs = rng.poisson(5.0, size=(len(times), 1))
+    err, _ = _check(oracle, h, 1, times, obs, 48, seed=7, p=0.5, dist="poisson",
+                    offset=1.0)
+    assert err == 0

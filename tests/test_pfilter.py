@@ -218,3 +218,51 @@ def test_pfilter_unbiased_likelihood(oracle):
     est = np.array(est)
     se = est.std(ddof=1) / math.sqrt(len(est))
     assert abs(est.mean() - exact) < 4 * se + 1e-12, (est.mean(), exact, se)
+
+
+def _enter_events(node_times, select, n=1, event=1, dest=None):
+    rows = [(event, t, nd, 0 if dest is None else dest, n, 0.0, select, 0)
+            for nd, t in node_times]
+    rows.sort(key=lambda r: r[1])
+    return {k: np.array([r[i] for r in rows]) for i, k in enumerate(
+        ("event", "time", "node", "dest", "n", "proportion", "select", "shift"))}
+
+
+def test_pfilter_interval_events_closed_form(oracle):
+    """Zero rates and one individual entering I at node 1 on days 2..20:
+    at an observation time t the particle holds I0 + #(events due at or
+    before t), so the log-likelihood is a sum of Poisson log-probabilities.
+    An interval that ran an earlier interval's events again, or dropped
+    its own, changes I (pfilter_events, R/pfilter.R:200-220)."""
+    times = [3.0, 4.0, 9.0, 15.0, 16.0]
+    ev = _enter_events([(1, t) for t in range(2, 21)], select=2)
+    m = si.SIR(u0={"S": [10, 10], "I": [2, 0], "R": [0, 0]}, tspan=[1.0, 2.0],
+               beta=0.0, gamma=0.0, events=ev)
+    obs = np.array([[3, 0], [4, 1], [7, 0], [15, 2], [16, 0]])
+    err, ll, ess, _ = oracle.pfilter(m, 1, times, obs, 8, obs_prob=0.5, seed=3,
+                                     obs_dist="poisson", obs_offset=0.25)
+    assert err == 0
+    I1 = [2 + sum(1 for d in range(2, 21) if d <= t) for t in times]
+    exact = sum(stats.poisson.logpmf(obs[k, 0], 0.5 * I1[k] + 0.25) +
+                stats.poisson.logpmf(obs[k, 1], 0.25) for k in range(len(times)))
+    assert ll == pytest.approx(exact, rel=1e-12)
+    assert np.allclose(ess, 8.0)
+
+
+def test_pfilter_interval_external_transfers_closed_form(oracle):
+    """External transfers of one infected individual from node 1 to node 2
+    on days 2, 5, 6 and 12, zero rates: per-node I is known at every
+    observation time."""
+    days = [2, 5, 6, 12]
+    ev = _enter_events([(1, t) for t in days], select=2, event=3, dest=2)
+    m = si.SIR(u0={"S": [5, 5], "I": [6, 0], "R": [0, 0]}, tspan=[1.0, 2.0],
+               beta=0.0, gamma=0.0, events=ev)
+    times = [2.0, 5.0, 6.0, 10.0, 13.0]
+    moved = [sum(1 for d in days if d <= t) for t in times]
+    obs = np.array([[6 - k, k] for k in moved])
+    err, ll, _, _ = oracle.pfilter(m, 1, times, obs, 5, obs_prob=1.0, seed=1,
+                                   obs_dist="poisson", obs_offset=0.5)
+    assert err == 0
+    exact = sum(stats.poisson.logpmf(6 - k, 6 - k + 0.5) + stats.poisson.logpmf(k, k + 0.5)
+                for k in moved)
+    assert ll == pytest.approx(exact, rel=1e-12)
