@@ -75,7 +75,10 @@ typedef enum {
     /* Summaries of a plan's trajectory (R-level errors of prevalence()
      * and trajectory(), R/prevalence.R:233-244). */
     SIMINF_ERR_EMPTY_TRAJECTORY     = -104,
-    SIMINF_ERR_INVALID_SUMMARY      = -105
+    SIMINF_ERR_INVALID_SUMMARY      = -105,
+    /* ABC-SMC weights (SimInf_abc_error, SimInf_abc.c:30-52). */
+    SIMINF_ERR_UNKNOWN_DISTRIBUTION = -106,
+    SIMINF_ERR_ABC_WEIGHTS          = -107
 } SimInf_error_code;
 
 /* Plugin signatures (SimInf.h:92-107). */
@@ -285,6 +288,24 @@ int SimInf_hip_plan_prevalence(SimInf_hip_plan *plan, unsigned int cases_mask,
  * columns the Nrep*tlen U columns; column-major int32. */
 int SimInf_hip_plan_trajectory(SimInf_hip_plan *plan, unsigned int comp_mask,
                                const int *index, int n_index, int *out);
+
+/* ABC-SMC weights of the current generation (SimInf_abc_weights,
+ * SimInf_abc.c:283-406), on the device.  distribution holds one letter
+ * per parameter: 'g' gamma (p1 shape, p2 rate), 'l' lognormal (p2
+ * sdlog), 'n' normal (p2 sd), 'u' uniform (p1, p2 bounds); the lognormal
+ * and normal priors are centred on the previous generation's particle of
+ * the same row, as in the reference.  x (previous, rows ordered by the
+ * current particles' ancestors) and xx (current) are n_particles x
+ * n_parameters column-major; w the previous weights (NULL: every weight
+ * 1/n_particles); sigma the proposal covariance (column-major).  ww gets
+ * the normalised weights.  Errors: SIMINF_ERR_UNKNOWN_DISTRIBUTION,
+ * SIMINF_ERR_INVALID_WEIGHT (a prior density of 0 or a non-finite one),
+ * SIMINF_ERR_ABC_WEIGHTS (a kernel sum that is not finite and positive,
+ * or sigma not positive definite), at the first particle in order. */
+int SimInf_hip_abc_weights(const char *distribution, const double *p1,
+                           const double *p2, int n_particles, int n_parameters,
+                           const double *x, const double *xx, const double *w,
+                           const double *sigma, double *ww);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@
 #include "../siminf_amd/csrc/siminf_rand.h"
 #include "../siminf_amd/csrc/siminf_models.h"
 #include "../siminf_amd/csrc/siminf_pfilter.h"
+#include "../siminf_amd/csrc/siminf_abc.h"
 
 #include <math.h>
 #include <stddef.h>
@@ -1284,4 +1285,76 @@ void oracle_log_dbinom(const int *x, const int *n, double p, double *y, long lon
 int oracle_resample(const double *w, int n, double u01, int *idx)
 {
     return siminf_systematic_resampling(w, n, u01, idx);
+}
+
+/* ABC-SMC weights (SimInf_abc_weights, SimInf_abc.c:283-406), in the
+ * reference's loop order: per current particle i, the prior terms, then
+ * the kernel sum over the previous particles j in order; the maximum
+ * starts at 0; normalisation in particle order.  The proposal density is
+ * evaluated on points transformed by L^-1 (siminf_abc.h).  Arguments and
+ * error codes as SimInf_hip_abc_weights. */
+int oracle_abc_weights(const char *distribution, const double *p1, const double *p2,
+                       int n, int d, const double *x, const double *xx, const double *w,
+                       const double *sigma, double *ww)
+{
+    double L[SIMINF_ABC_MAX_PARS * SIMINF_ABC_MAX_PARS], lg[SIMINF_ABC_MAX_PARS];
+    double lognorm = 0.0, max_ww = 0.0, sum;
+    double *zc = NULL, *zp = NULL;
+    int err = 0;
+    if (!w) {
+        for (int i = 0; i < n; i++)
+            ww[i] = 1.0 / (double)n;
+        return 0;
+    }
+    for (int k = 0; k < d; k++) {
+        const int c = distribution[k];
+        if (c != SIMINF_PRIOR_GAMMA && c != SIMINF_PRIOR_LOGNORMAL &&
+            c != SIMINF_PRIOR_NORMAL && c != SIMINF_PRIOR_UNIFORM)
+            return SIMINF_ERR_UNKNOWN_DISTRIBUTION;
+        lg[k] = c == SIMINF_PRIOR_GAMMA ? lgamma(p1[k]) : 0.0;
+    }
+    if (siminf_cholesky(sigma, d, L, &lognorm))
+        return SIMINF_ERR_ABC_WEIGHTS;
+    zc = malloc((size_t)n * d * sizeof(double));
+    zp = malloc((size_t)n * d * sizeof(double));
+    if (!zc || !zp) {
+        err = SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+        goto done;
+    }
+    for (int i = 0; i < n; i++) {
+        siminf_forward_subst(xx + i, n, L, d, zc + i, n);
+        siminf_forward_subst(x + i, n, L, d, zp + i, n);
+    }
+    for (int i = 0; i < n; i++) {
+        double lp = 0.0;
+        for (int k = 0; k < d; k++)
+            lp = lp + siminf_prior_logdens(distribution[k], xx[(size_t)k * n + i],
+                                           x[(size_t)k * n + i], p1[k], p2[k], lg[k]);
+        if (!(lp > -INFINITY && lp < INFINITY)) {
+            err = SIMINF_ERR_INVALID_WEIGHT;
+            goto done;
+        }
+        sum = 0.0;
+        for (int j = 0; j < n; j++)
+            sum = sum + w[j] * siminf_exp(siminf_mvn_logpdf_z(zc + i, n, zp + j, n, d,
+                                                              lognorm));
+        if (!(sum > 0.0 && sum < INFINITY)) {
+            err = SIMINF_ERR_ABC_WEIGHTS;
+            goto done;
+        }
+        ww[i] = lp - siminf_log(sum);
+        if (ww[i] > max_ww)
+            max_ww = ww[i];
+    }
+    sum = 0.0;
+    for (int i = 0; i < n; i++) {
+        ww[i] = siminf_exp(ww[i] - max_ww);
+        sum = sum + ww[i];
+    }
+    for (int i = 0; i < n; i++)
+        ww[i] = ww[i] / sum;
+done:
+    free(zc);
+    free(zp);
+    return err;
 }

@@ -8,9 +8,10 @@ from .modelparser import mparse
 from .particle_filter import pfilter, PFilterResult
 from .plan import Plan
 from .punchcard import punchcard
+from .abc_smc import abc, ABCResult
 from .summaries import SummaryError
 from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
            "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse",
-           "pfilter", "PFilterResult", "Plan", "SummaryError", "punchcard"]
+           "pfilter", "PFilterResult", "Plan", "SummaryError", "punchcard", "abc", "ABCResult"]

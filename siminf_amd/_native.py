@@ -85,6 +85,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_pfilter",
     "SimInf_hip_plan_prevalence",
     "SimInf_hip_plan_trajectory",
+    "SimInf_hip_abc_weights",
     "SimInf_hip_plan_destroy",
 )
 
@@ -143,6 +144,9 @@ def load(path=None):
         ctypes.c_int, dp]
     lib.SimInf_hip_plan_trajectory.restype = ctypes.c_int
     lib.SimInf_hip_plan_trajectory.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int, ip]
+    lib.SimInf_hip_abc_weights.restype = ctypes.c_int
+    lib.SimInf_hip_abc_weights.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int,
+                                           ctypes.c_int, dp, dp, dp, dp, dp]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
     if path == LIB_PATH:

@@ -20,6 +20,7 @@
  */
 #include "siminf_kernels.h"
 #include "siminf_pfilter.h"
+#include "siminf_abc.h"
 
 #include <hip/hip_runtime.h>
 
@@ -1178,6 +1179,8 @@ extern "C" const char *SimInf_hip_error_message(int code)
     case SIMINF_ERR_EMPTY_TRAJECTORY:
         return "Please run the model first, the trajectory is empty.";
     case SIMINF_ERR_INVALID_SUMMARY: return "Invalid summary specification.";
+    case SIMINF_ERR_UNKNOWN_DISTRIBUTION: return "Unknown distribution.";
+    case SIMINF_ERR_ABC_WEIGHTS: return "Unable to calculate weights.";
     default: return "Unknown error code.";
     }
 }
@@ -1824,4 +1827,176 @@ extern "C" int SimInf_hip_plan_trajectory(SimInf_hip_plan *P, unsigned int comp_
         return SIMINF_ERR_DEVICE;
     T.copy(out, out_d, (size_t)total * ncomp * sizeof(int), hipMemcpyDeviceToHost);
     return T.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
+}
+
+/* ------------------------------------------------------------------ */
+/* ABC-SMC weights (SimInf_abc_weights, SimInf_abc.c:283-406).          */
+/* ------------------------------------------------------------------ */
+
+/* z = L^-1 x for every particle (columns of a particle matrix). */
+__global__ __launch_bounds__(256) void k_abc_transform(const double *x, int n, int d,
+                                                       const double *L, double *z)
+{
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i < n)
+        siminf_forward_subst(x + i, n, L, d, z + i, n);
+}
+
+/* Unnormalised log-weight of each current particle: the sum of its prior
+ * log-densities less the log of sum_j w_j N(xx_i; x_j, sigma), the
+ * kernel sum in particle order; flag 1 for a non-finite prior term, 2 for
+ * a kernel sum that is not finite and positive. */
+__global__ __launch_bounds__(256) void k_abc_logw(const double *xx, const double *x,
+                                                  const double *zc, const double *zp,
+                                                  const double *w, int n, int d,
+                                                  const int *kind, const double *p1,
+                                                  const double *p2, const double *lg,
+                                                  double lognorm, double *logw, int *flag)
+{
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= n)
+        return;
+    double lp = 0.0;
+    for (int k = 0; k < d; k++)
+        lp = lp + siminf_prior_logdens(kind[k], xx[(long long)k * n + i],
+                                       x[(long long)k * n + i], p1[k], p2[k], lg[k]);
+    if (!(lp > -INFINITY && lp < INFINITY)) {
+        logw[i] = 0.0;
+        flag[i] = 1;
+        return;
+    }
+    double sum = 0.0;
+    for (int j = 0; j < n; j++)
+        sum = sum + w[j] * siminf_exp(siminf_mvn_logpdf_z(zc + i, n, zp + j, n, d, lognorm));
+    if (!(sum > 0.0 && sum < INFINITY)) {
+        logw[i] = 0.0;
+        flag[i] = 2;
+        return;
+    }
+    logw[i] = lp - siminf_log(sum);
+    flag[i] = 0;
+}
+
+/* Normalisation in one workgroup with the reference's sequence: the
+ * maximum starts at 0, exp(ww - max) summed in particle order on one
+ * lane, then divided; or the error of the first flagged particle. */
+__global__ __launch_bounds__(256) void k_abc_normalize(double *ww, const int *flag, int n,
+                                                       int *err)
+{
+    __shared__ double s_max[256];
+    __shared__ int s_first[256];
+    __shared__ double s_sum;
+    const int tid = (int)threadIdx.x;
+    double m = 0.0;
+    int first = INT_MAX;
+    for (int i = tid; i < n; i += 256) {
+        if (flag[i] && i < first)
+            first = i;
+        m = ww[i] > m ? ww[i] : m;
+    }
+    s_max[tid] = m;
+    s_first[tid] = first;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            s_max[tid] = s_max[tid + h] > s_max[tid] ? s_max[tid + h] : s_max[tid];
+            s_first[tid] = s_first[tid + h] < s_first[tid] ? s_first[tid + h] : s_first[tid];
+        }
+        __syncthreads();
+    }
+    if (s_first[0] != INT_MAX) {
+        if (tid == 0)
+            *err = flag[s_first[0]] == 1 ? SIMINF_ERR_INVALID_WEIGHT : SIMINF_ERR_ABC_WEIGHTS;
+        return;
+    }
+    const double mx = s_max[0];
+    for (int i = tid; i < n; i += 256)
+        ww[i] = siminf_exp(ww[i] - mx);
+    __syncthreads();
+    if (tid == 0)
+        s_sum = pf_serial_sum(ww, n);
+    __syncthreads();
+    const double sum = s_sum;
+    for (int i = tid; i < n; i += 256)
+        ww[i] = ww[i] / sum;
+}
+
+extern "C" int SimInf_hip_abc_weights(const char *distribution, const double *p1,
+                                      const double *p2, int n_particles, int n_parameters,
+                                      const double *x, const double *xx, const double *w,
+                                      const double *sigma, double *ww)
+{
+    const int n = n_particles, d = n_parameters;
+    if (n < 1 || !ww)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (!w) {
+        for (int i = 0; i < n; i++)
+            ww[i] = 1.0 / (double)n;
+        return 0;
+    }
+    if (d < 1 || d > SIMINF_ABC_MAX_PARS || !distribution || !p1 || !p2 || !x || !xx ||
+        !sigma)
+        return SIMINF_ERR_INVALID_MODEL;
+    std::vector<int> kind(d);
+    std::vector<double> lg(d, 0.0);
+    for (int k = 0; k < d; k++) {
+        kind[k] = distribution[k];
+        if (kind[k] != SIMINF_PRIOR_GAMMA && kind[k] != SIMINF_PRIOR_LOGNORMAL &&
+            kind[k] != SIMINF_PRIOR_NORMAL && kind[k] != SIMINF_PRIOR_UNIFORM)
+            return SIMINF_ERR_UNKNOWN_DISTRIBUTION;
+        if (kind[k] == SIMINF_PRIOR_GAMMA)
+            lg[k] = std::lgamma(p1[k]);
+    }
+    std::vector<double> L((size_t)d * d);
+    double lognorm = 0.0;
+    if (siminf_cholesky(sigma, d, L.data(), &lognorm))
+        return SIMINF_ERR_ABC_WEIGHTS;
+
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    int rc = 0;
+    {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        DeviceBuffers B;
+        B.stream = st;
+        B.pooled = pool_allocation(dev);
+        const size_t nd = (size_t)n * d;
+        const double *x_d = B.upload(x, nd), *xx_d = B.upload(xx, nd);
+        const double *w_d = B.upload(w, (size_t)n);
+        const double *L_d = B.upload(L.data(), L.size());
+        const int *kind_d = B.upload(kind.data(), kind.size());
+        const double *p1_d = B.upload(p1, (size_t)d), *p2_d = B.upload(p2, (size_t)d);
+        const double *lg_d = B.upload(lg.data(), lg.size());
+        double *zc = B.alloc<double>(nd), *zp = B.alloc<double>(nd);
+        double *logw = B.alloc<double>((size_t)n);
+        int *flag = B.alloc<int>((size_t)n);
+        const int zero = 0;
+        int *err_d = B.upload(&zero, 1);
+        if (B.status != hipSuccess) {
+            rc = SIMINF_ERR_DEVICE;
+        } else {
+            const unsigned grid = (unsigned)((n + 255) / 256);
+            hipLaunchKernelGGL(k_abc_transform, dim3(grid), dim3(256), 0, st, xx_d, n, d, L_d,
+                               zc);
+            hipLaunchKernelGGL(k_abc_transform, dim3(grid), dim3(256), 0, st, x_d, n, d, L_d,
+                               zp);
+            hipLaunchKernelGGL(k_abc_logw, dim3(grid), dim3(256), 0, st, xx_d, x_d, zc, zp, w_d,
+                               n, d, kind_d, p1_d, p2_d, lg_d, lognorm, logw, flag);
+            hipLaunchKernelGGL(k_abc_normalize, dim3(1), dim3(256), 0, st, logw, flag, n,
+                               err_d);
+            int err = 0;
+            if (hipGetLastError() != hipSuccess) {
+                rc = SIMINF_ERR_DEVICE;
+            } else {
+                B.copy(&err, err_d, sizeof(int), hipMemcpyDeviceToHost);
+                B.copy(ww, logw, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+                rc = B.status != hipSuccess ? SIMINF_ERR_DEVICE : err;
+            }
+        }
+        B.release();
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
 }

@@ -77,7 +77,8 @@ class SimInfModel:
 
     def __init__(self, plugin, G, S, tspan, u0, ldata=None, gdata=None,
                  v0=None, E=None, N=None, events=None, replicates=1,
-                 U_sparse=None, V_sparse=None, compartments=None, v0_names=None):
+                 U_sparse=None, V_sparse=None, compartments=None, v0_names=None,
+                 ldata_names=None, gdata_names=None):
         self.plugin = plugin
         self.G = _csc(G)
         self.S = _csc(S)
@@ -123,6 +124,14 @@ class SimInfModel:
         self.Nld = self.ldata.shape[0]
         self.gdata = np.asarray(gdata if gdata is not None else [],
                                 dtype=np.float64).ravel()
+        # Row names of ldata and names of gdata (the parameters abc() can
+        # infer).
+        self.ldata_names = list(ldata_names) if ldata_names is not None else \
+            ["L%d" % (k + 1) for k in range(self.Nld)]
+        self.gdata_names = list(gdata_names) if gdata_names is not None else \
+            ["G%d" % (k + 1) for k in range(len(self.gdata))]
+        if len(self.ldata_names) != self.Nld or len(self.gdata_names) != len(self.gdata):
+            raise ValueError("one name per ldata row and per gdata value")
         self.events = make_events(E if E is not None else np.zeros((self.Nc, 0)),
                                   N, events)
         self.E = self.events["E"]

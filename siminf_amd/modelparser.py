@@ -600,7 +600,10 @@ def mparse(transitions, compartments, ldata=None, gdata=None, u0=None, v0=None,
                         ldata=ld, gdata=gvec, v0=v if v.shape[0] else None,
                         E=E if E is not None else np.zeros((len(compartments), 0)),
                         N=N, events=events, replicates=replicates,
-                        compartments=compartments, v0_names=v_names)
+                        compartments=compartments, v0_names=v_names,
+                        ldata_names=ld_names,
+                        gdata_names=[k if k is not None else "" for k in gdata]
+                        if gdata else None)
     model.C_code = parsed.C_code
     model.G_rownames = parsed.G_rownames
     return model

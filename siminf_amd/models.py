@@ -77,7 +77,7 @@ def SIR(u0, tspan, events=None, beta=None, gamma=None, replicates=1):
     u, _ = _tile_reps(u, None, replicates)
     return SimInfModel("SIR", G, S, tspan, u, ldata=ldata, E=select_matrix_SIR(),
                        events=events, replicates=replicates,
-                       compartments=("S", "I", "R"))
+                       compartments=("S", "I", "R"), ldata_names=("beta", "gamma"))
 
 
 def SEIR(u0, tspan, events=None, beta=None, epsilon=None, gamma=None,
@@ -91,7 +91,8 @@ def SEIR(u0, tspan, events=None, beta=None, epsilon=None, gamma=None,
     return SimInfModel("SEIR", G, S, tspan, u, ldata=ldata,
                        E=select_matrix_SEIR(), events=events,
                        replicates=replicates,
-                       compartments=("S", "E", "I", "R"))
+                       compartments=("S", "E", "I", "R"),
+                       ldata_names=("beta", "epsilon", "gamma"))
 
 
 def SIS(u0, tspan, events=None, beta=None, gamma=None, replicates=1):
@@ -103,7 +104,7 @@ def SIS(u0, tspan, events=None, beta=None, gamma=None, replicates=1):
     u, _ = _tile_reps(u, None, replicates)
     return SimInfModel("SIS", G, S, tspan, u, ldata=ldata, E=select_matrix_SIS(),
                        events=events, replicates=replicates,
-                       compartments=("S", "I"))
+                       compartments=("S", "I"), ldata_names=("beta", "gamma"))
 
 
 def SISe(u0, tspan, events=None, phi=0.0, upsilon=None, gamma=None, alpha=None,
@@ -122,7 +123,10 @@ def SISe(u0, tspan, events=None, phi=0.0, upsilon=None, gamma=None, alpha=None,
     return SimInfModel("SISe", G, S, tspan, u, ldata=ldata, gdata=gdata, v0=v0,
                        E=select_matrix_SIS(), events=events,
                        replicates=replicates,
-                       compartments=("S", "I"), v0_names=("phi",))
+                       compartments=("S", "I"), v0_names=("phi",),
+                       ldata_names=("end_t1", "end_t2", "end_t3", "end_t4"),
+                       gdata_names=("upsilon", "gamma", "alpha", "beta_t1", "beta_t2",
+                                    "beta_t3", "beta_t4", "epsilon"))
 
 
 def SISe3(u0, tspan, events=None, phi=0.0, upsilon_1=None, upsilon_2=None,
@@ -148,7 +152,11 @@ def SISe3(u0, tspan, events=None, phi=0.0, upsilon_1=None, upsilon_2=None,
                        E=select_matrix_SISe3(), N=shift_matrix_SISe3(),
                        events=events, replicates=replicates,
                        compartments=("S_1", "I_1", "S_2", "I_2", "S_3", "I_3"),
-                       v0_names=("phi",))
+                       v0_names=("phi",),
+                       ldata_names=("end_t1", "end_t2", "end_t3", "end_t4"),
+                       gdata_names=("upsilon_1", "upsilon_2", "upsilon_3", "gamma_1",
+                                    "gamma_2", "gamma_3", "alpha", "beta_t1", "beta_t2",
+                                    "beta_t3", "beta_t4", "epsilon"))
 
 
 def SIR_mparse(u0, tspan, beta, gamma, events=None, E=None, N=None,
@@ -166,4 +174,4 @@ def SIR_mparse(u0, tspan, beta, gamma, events=None, E=None, N=None,
     return SimInfModel("SIR_mparse", G, S, tspan, u, gdata=[beta, gamma],
                        E=E if E is not None else np.zeros((3, 0)), N=N,
                        events=events, replicates=replicates,
-                       compartments=("S", "I", "R"))
+                       compartments=("S", "I", "R"), gdata_names=("beta", "gamma"))

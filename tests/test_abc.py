@@ -1,0 +1,136 @@
+"""ABC-SMC host logic and the checker's weights, on the CPU.
+
+The checker's weights (oracle_abc_weights) are pinned against a direct
+scipy restatement of SimInf_abc_weights (SimInf_abc.c:283-406: prior
+log-densities, the Gaussian kernel sum over the previous generation, the
+maximum started at 0, normalisation); prior parsing, tolerance, distance
+and acceptance checks against the reference's messages; proposals
+against their defining properties."""
+import numpy as np
+import pytest
+from scipy import stats
+
+import siminf_amd as si
+from siminf_amd import abc_smc as A
+
+ERR_INVALID_WEIGHT, ERR_UNKNOWN_DIST, ERR_ABC_WEIGHTS = -102, -106, -107
+
+
+def _reference_weights(priors, x, xx, w, sigma):
+    n = xx.shape[0]
+    ww = np.zeros(n)
+    for i in range(n):
+        lp = 0.0
+        for k, p in enumerate(priors):
+            if p.kind == "g":
+                lp += stats.gamma.logpdf(xx[i, k], p.p1, scale=1 / p.p2)
+            elif p.kind == "l":
+                lp += stats.lognorm.logpdf(xx[i, k], p.p2, scale=np.exp(x[i, k]))
+            elif p.kind == "n":
+                lp += stats.norm.logpdf(xx[i, k], x[i, k], p.p2)
+            else:
+                lp += stats.uniform.logpdf(xx[i, k], p.p1, p.p2 - p.p1)
+        s = sum(w[j] * stats.multivariate_normal.pdf(xx[i], x[j], sigma) for j in range(n))
+        ww[i] = lp - np.log(s)
+    ww = np.exp(ww - max(0.0, ww.max()))
+    return ww / ww.sum()
+
+
+def _case(seed, n, priors):
+    rng = np.random.default_rng(seed)
+    pr = A.parse_priors(priors)
+    d = len(pr)
+    x = np.column_stack([rng.uniform(0.2, 0.8, n) for _ in range(d)])
+    xx = x + rng.normal(0, 0.05, (n, d))
+    w = rng.random(n)
+    w /= w.sum()
+    sigma = 2 * np.atleast_2d(np.cov(x, rowvar=False))
+    return pr, x, xx, w, sigma
+
+
+@pytest.mark.parametrize("priors", [
+    ["beta ~ uniform(0, 1)"],
+    ["beta ~ uniform(0, 1)", "gamma ~ gamma(2, 3)"],
+    ["a ~ normal(0.5, 0.2)", "b ~ lognormal(0, 1)", "c ~ uniform(-1, 2)"],
+])
+def test_checker_weights_against_scipy(oracle, priors):
+    pr, x, xx, w, sigma = _case(len(priors), 40, priors)
+    err, ww = oracle.abc_weights("".join(p.kind for p in pr), [p.p1 for p in pr],
+                                 [p.p2 for p in pr], x, xx, w, sigma)
+    assert err == 0
+    assert np.allclose(ww, _reference_weights(pr, x, xx, w, sigma), rtol=1e-10, atol=0)
+    assert abs(ww.sum() - 1.0) < 1e-12
+
+
+def test_checker_weights_errors(oracle):
+    pr, x, xx, w, sigma = _case(1, 10, ["beta ~ uniform(0, 1)"])
+    assert oracle.abc_weights("z", [0.0], [1.0], x, xx, w, sigma)[0] == ERR_UNKNOWN_DIST
+    bad = xx.copy()
+    bad[3, 0] = 1.5                                     # outside the uniform prior
+    assert oracle.abc_weights("u", [0.0], [1.0], x, bad, w, sigma)[0] == ERR_INVALID_WEIGHT
+    assert oracle.abc_weights("u", [0.0], [1.0], x, xx, w, [[0.0]])[0] == ERR_ABC_WEIGHTS
+    err, ww = oracle.abc_weights("u", [0.0], [1.0], None, xx, None, None)
+    assert err == 0 and np.allclose(ww, 0.1)
+
+
+def test_parse_priors_messages():
+    p = A.parse_priors(["beta ~ uniform(0.05, 0.4)", "gamma~gamma(2,4)"])
+    assert [(q.parameter, q.kind, q.p1, q.p2) for q in p] == [
+        ("beta", "u", 0.05, 0.4), ("gamma", "g", 2.0, 4.0)]
+    for bad, msg in [(5, "'priors' must be a formula or a list with formula items."),
+                     ("beta ~ weibull(1, 2)", "Unknown distribution: 'weibull'."),
+                     ("beta ~ uniform(1, 0)", "Invalid distribution: uniform bounds in wrong order."),
+                     ("beta ~ gamma(0, 1)", "Invalid distribution: gamma hyperparameters must be > 0."),
+                     ("beta ~ normal(0, -1)", "Invalid distribution: normal variance must be >= 0."),
+                     ("beta ~ uniform(0)", "Invalid formula specification for uniform distribution."),
+                     ("beta ~ uniform(a, 1)", "Invalid formula specification for 'priors'."),
+                     (["b ~ uniform(0, 1)", "b ~ uniform(0, 2)"],
+                      "'priors' must have non-duplicated parameter names.")]:
+        with pytest.raises(A.ABCError) as e:
+            A.parse_priors(bad)
+        assert str(e.value) == msg, bad
+
+
+def test_tolerance_distance_accept():
+    assert A.init_tolerance([3, 2, 1]).shape == (1, 3)
+    for bad, msg in [([1, 2], "'tolerance' must be a decreasing vector."),
+                     ([1, -1], "'tolerance' must have non-negative values.")]:
+        with pytest.raises(A.ABCError, match=msg):
+            A.init_tolerance(bad)
+    with pytest.raises(A.ABCError, match="must be non-negative"):
+        A.check_distance([1.0, -2.0], 2)
+    with pytest.raises(A.ABCError, match="Invalid dimension"):
+        A.check_distance([1.0, 2.0], 3)
+    d = A.check_distance([[0.1, 0.5], [0.3, 0.1]], 2)
+    assert A.accept(d, np.array([0.2, 0.6])).tolist() == [True, False]
+    with pytest.raises(A.ABCError, match="Mismatch"):
+        A.accept(d, np.array([0.2]))
+
+
+def test_proposals_properties():
+    rng = np.random.default_rng(4)
+    pr = A.parse_priors(["beta ~ uniform(0, 1)", "gamma ~ gamma(2, 4)"])
+    xx, anc = A.proposals(pr, 20000, None, None, None, rng)
+    assert np.all(anc == -1)
+    assert stats.kstest(xx[:, 0], "uniform").pvalue > 1e-3
+    assert stats.kstest(xx[:, 1], "gamma", args=(2, 0, 0.25)).pvalue > 1e-3
+    x = np.column_stack([rng.uniform(0.9, 1.0, 50), rng.gamma(2, 0.25, 50)])
+    w = np.zeros(50)
+    w[[3, 7]] = [0.25, 0.75]
+    xx, anc = A.proposals(pr, 4000, x, w, 2 * np.cov(x, rowvar=False), rng)
+    assert set(anc.tolist()) == {3, 7}
+    assert abs(np.mean(anc == 7) - 0.75) < 0.03
+    assert np.all((xx[:, 0] >= 0) & (xx[:, 0] <= 1)) and np.all(xx[:, 1] >= 0)
+
+
+def test_match_priors_and_replicate_first_node():
+    m = si.SIR(u0={"S": [99, 50], "I": [5, 1], "R": [0, 0]}, tspan=np.arange(1, 11.0),
+               beta=0.2, gamma=0.1)
+    assert A.match_priors(m, A.parse_priors(["gamma ~ uniform(0, 1)"])) == ("ldata", [1])
+    mp = si.SIR_mparse(u0={"S": [99], "I": [5], "R": [0]}, tspan=np.arange(1, 11.0),
+                       beta=0.2, gamma=0.1)
+    assert A.match_priors(mp, A.parse_priors(["beta ~ uniform(0, 1)"])) == ("gdata", [0])
+    with pytest.raises(A.ABCError, match="either in 'gdata' or 'ldata'"):
+        A.match_priors(m, A.parse_priors(["delta ~ uniform(0, 1)"]))
+    r = A.replicate_first_node(m, 4)
+    assert r.Nn == 4 and r.u0[:, 3].tolist() == [99, 5, 0] and r.ldata.shape == (2, 4)

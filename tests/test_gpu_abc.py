@@ -1,0 +1,76 @@
+"""ABC-SMC on the GPU: the device weights equal the checker's bit for bit,
+and the whole loop (one node per proposal, the simulations and the weights
+on the device) recovers a known infection rate."""
+import numpy as np
+import pytest
+
+import siminf_amd as si
+from siminf_amd import abc_smc as A
+from tests.test_abc import _case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,priors", [
+    (40, ["beta ~ uniform(0, 1)"]),
+    (300, ["beta ~ uniform(0, 1)", "gamma ~ gamma(2, 3)"]),
+    (3000, ["a ~ normal(0.5, 0.2)", "b ~ lognormal(0, 1)", "c ~ uniform(-1, 2)"]),
+])
+def test_device_weights_equal_checker(oracle, n, priors):
+    print("case: abc weights n=%d d=%d" % (n, len(priors)), flush=True)
+    pr, x, xx, w, sigma = _case(n, n, priors)
+    dist = "".join(p.kind for p in pr)
+    p1, p2 = [p.p1 for p in pr], [p.p2 for p in pr]
+    lib = si._native.load()
+    err, ww = A.call_weights(lib.SimInf_hip_abc_weights, dist, p1, p2, x, xx, w, sigma)
+    rerr, rww = oracle.abc_weights(dist, p1, p2, x, xx, w, sigma)
+    assert err == rerr == 0
+    np.testing.assert_array_equal(ww, rww)
+    bad = xx.copy()
+    bad[n // 2, 0] = 7.0 if dist[0] == "u" else bad[n // 2, 0]
+    if dist[0] == "u":
+        err, _ = A.call_weights(lib.SimInf_hip_abc_weights, dist, p1, p2, x, bad, w, sigma)
+        assert err == oracle.abc_weights(dist, p1, p2, x, bad, w, sigma)[0] == -102
+
+
+def _final_recovered(model):
+    """Recovered individuals per node at the last time point."""
+    return model.U[2::3, -1].astype(float)
+
+
+def test_abc_ldata_recovers_beta():
+    truth = si.SIR(u0={"S": [990], "I": [10], "R": [0]}, tspan=np.arange(1, 61.0),
+                   beta=0.3, gamma=0.1)
+    si.run(truth, seed=11)
+    observed = _final_recovered(truth)[0]
+    print("case: abc ldata, observed R=%d" % observed, flush=True)
+
+    def distance(result, generation, data):
+        return np.abs(_final_recovered(result) - data) / 1000.0
+
+    model = si.SIR(u0={"S": [990], "I": [10], "R": [0]}, tspan=np.arange(1, 61.0),
+                   beta=0.2, gamma=0.1)
+    fit = si.abc(model, ["beta ~ uniform(0.05, 0.8)"], n_particles=200,
+                 distance=distance, tolerance=[0.3, 0.1, 0.03], data=observed, seed=5)
+    assert len(fit.x) == 3
+    for g in range(3):
+        assert fit.x[g].shape == (200, 1)
+        assert np.all(fit.distance[g] <= fit.tolerance[g])
+        assert abs(fit.weight[g].sum() - 1.0) < 1e-12 and fit.ess[g] > 10
+    beta = fit.x[-1][:, 0]
+    mean = float(np.sum(fit.weight[-1] * beta))
+    assert 0.2 < mean < 0.45 and np.std(beta) < np.std(fit.x[0][:, 0])
+
+
+def test_abc_gdata_one_run_per_proposal():
+    model = si.SIR_mparse(u0={"S": [99], "I": [5], "R": [0]}, tspan=np.arange(1, 31.0),
+                          beta=0.16, gamma=0.077)
+    print("case: abc gdata", flush=True)
+
+    def distance(result, generation, data):
+        return [abs(float(result.U[1, -1]) - 10.0) / 100.0]
+
+    fit = si.abc(model, ["beta ~ uniform(0.05, 0.5)"], n_particles=12, distance=distance,
+                 tolerance=[0.5, 0.2], seed=2)
+    assert [x.shape for x in fit.x] == [(12, 1), (12, 1)]
+    assert all(n >= 12 for n in fit.nprop)
