@@ -937,6 +937,60 @@ __device__ int apply_e1(const EvTab &tab, int type, int n, int select, int shift
  * slot src, the destination in slot dst; upd receives the update flags of
  * both slots.  dest is the destination node, checked against the node
  * count before any slot is touched. */
+/* Shift field of an external transfer that the host found to need no
+ * shift and to select one row, or two rows of equal weight, of E, with a
+ * count n > 0, a valid destination other than the source. */
+#define SIMINF_E2_FAST (-2)
+
+/* apply_e2 for those events: sample_select's branches for one or two
+ * rows (take all, a single row, a single non-empty row, one conditional
+ * hypergeometric draw for the first row with the second taking the rest)
+ * over counts read straight from the state array, without the per-
+ * compartment loops of the general case; the same draws, the same
+ * results and errors. */
+template <int NC>
+__device__ __forceinline__ int apply_e2_fast(const EvTab &tab, int n, int select, int *uw,
+                                             int *upd, int src, int dst,
+                                             siminf_stream *rng)
+{
+    const uint32_t mask = tab.Emask[select];
+    const int ra = mask_first(mask);
+    const uint32_t rest = mask & (mask - 1u);
+    const bool two = rest != 0u;
+    const int rb = two ? mask_first(rest) : ra;
+    int *sa = uw + src * NC + ra, *sb = uw + src * NC + rb;
+    int *da = uw + dst * NC + ra, *db = uw + dst * NC + rb;
+    const int ua = *sa, ub = two ? *sb : 0;
+    const int nind = ua + ub;
+    if (n > nind)
+        return SIMINF_ERR_SAMPLE_SELECT;
+    int xa;
+    if (n == nind)
+        xa = ua;
+    else if (!two || ub <= 0)
+        xa = n;
+    else if (ua <= 0)
+        xa = 0;
+    else
+        xa = siminf_hypergeometric(rng, ua, ub, n);
+    const int xb = n - xa;
+    const int va = *da + xa, wa = ua - xa;
+    *da = va;
+    *sa = wa;
+    bool neg = (va < 0) | (wa < 0);
+    if (two) {
+        const int vb = *db + xb, wb = ub - xb;
+        *db = vb;
+        *sb = wb;
+        neg = neg | (vb < 0) | (wb < 0);
+    }
+    if (neg)
+        return SIMINF_ERR_NEGATIVE_STATE;
+    upd[dst] = 1;
+    upd[src] = 1;
+    return 0;
+}
+
 template <int NC>
 __device__ int apply_e2(const EvTab &tab, int n, int select, int shift,
                         double prop, int dest, int Nn, int *uw, int *upd,

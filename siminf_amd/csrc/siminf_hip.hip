@@ -607,9 +607,27 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     }
     std::vector<int4> e2_rec(std::max<size_t>(e2_idx.size(), 1) * 2, int4{0, 0, 0, 0});
     std::vector<double> e2_prop(std::max<size_t>(e2_idx.size(), 1), 0.0);
+    /* Events the fast path takes (SIMINF_E2_FAST in the shift field):
+     * a count, no shift, another herd as destination, and an E column of
+     * one row or of two rows with equal weights. */
+    auto e2_fast = [&](int k) {
+        const int sel = ev_select[k];
+        if (a->n[k] <= 0 || ev_shift[k] >= 0 || ev_dest[k] < 0 || ev_dest[k] >= Nn ||
+            ev_dest[k] == ev_node[k] || sel < 0 || sel >= Nselect)
+            return false;
+        const uint32_t m = Emask[sel];
+        const int rows = __builtin_popcount(m);
+        if (rows == 1)
+            return true;
+        if (rows != 2)
+            return false;
+        const int r0 = __builtin_ctz(m), r1 = __builtin_ctz(m & (m - 1u));
+        return Ew[(size_t)sel * Nc + r0] == Ew[(size_t)sel * Nc + r1];
+    };
     for (size_t i = 0; i < e2_idx.size(); i++) {
         const int k = e2_idx[i];
-        e2_rec[2 * i] = int4{k, a->n[k], ev_select[k], ev_shift[k]};
+        e2_rec[2 * i] = int4{k, a->n[k], ev_select[k],
+                             e2_fast(k) ? SIMINF_E2_FAST : ev_shift[k]};
         e2_rec[2 * i + 1] = int4{e2_lsrc[k], e2_ldst[k], ev_node[k], ev_dest[k]};
         e2_prop[i] = a->proportion[k];
     }

@@ -401,9 +401,12 @@ __device__ __forceinline__ int e2_levels(const KParams &p, const EvTab &tab,
                 rng.pre = pre + (long long)i * SIMINF_E2_PRE;
                 rng.npre = SIMINF_E2_PRE;
             }
-            const int err = apply_e2<NC>(tab, r0.y, r0.z, r0.w, prop[i], r1.w,
-                                         p.Nn, su, sf, LDS ? r1.x : r1.z,
-                                         LDS ? r1.y : r1.w, &rng);
+            const int err =
+                r0.w == SIMINF_E2_FAST
+                    ? apply_e2_fast<NC>(tab, r0.y, r0.z, su, sf, LDS ? r1.x : r1.z,
+                                        LDS ? r1.y : r1.w, &rng)
+                    : apply_e2<NC>(tab, r0.y, r0.z, r0.w, prop[i], r1.w, p.Nn, su, sf,
+                                   LDS ? r1.x : r1.z, LDS ? r1.y : r1.w, &rng);
             if (err) {
                 record_error(p, rep, day, PH_E2, p.ev_rank[r0.x], err);
                 myerr = 1;
