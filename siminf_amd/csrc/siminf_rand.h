@@ -475,29 +475,41 @@ SIMINF_FN int siminf_hypergeometric(siminf_stream *s, int n1, int n2, int t)
     if (t > total)
         t = total;
     draws = (2 * t <= total) ? t : total - t;
-    if (draws <= SIMINF_PRE_MAX && s->j + (uint64_t)draws <= s->npre) {
+    if (draws > 0 && draws <= SIMINF_PRE_MAX && s->j + (uint64_t)draws <= s->npre) {
         /* Every uniform the urn can need was computed ahead: the same
-         * draws as below, from values loaded together up front, in a
-         * fully unrolled loop without the stream's branches; the counts
-         * are kept as doubles (exact: integers < 2^31). */
+         * draws as below, without the stream's branches.  Until the urn
+         * runs out of good items, draw i sees left = total - i, so every
+         * product u_i * left is known up front, and with good an integer
+         * u_i * left < good is floor(u_i * left) < good: the draws then
+         * chain through integer additions only.  Once no good item is
+         * left, no product is below 0, so k stops growing exactly where
+         * the draw-by-draw urn stops drawing. */
         const double *pre = s->pre + s->j;
+        const int full = s->j + (uint64_t)SIMINF_PRE_MAX <= s->npre;
         double v[SIMINF_PRE_MAX];
-        double dgood = (double)good, dleft = (double)left;
-        int used = 0, done = 0;
+        int fl[SIMINF_PRE_MAX];
+        int used = draws;
+        /* Unconditional loads, issued together (entries past draws are
+         * read from a valid slot and not used). */
+        if (full) {
+            SIMINF_UNROLL
+            for (int i = 0; i < SIMINF_PRE_MAX; i++)
+                v[i] = pre[i];
+        } else {
+            SIMINF_UNROLL
+            for (int i = 0; i < SIMINF_PRE_MAX; i++)
+                v[i] = pre[i < draws ? i : 0];
+        }
         SIMINF_UNROLL
         for (int i = 0; i < SIMINF_PRE_MAX; i++)
-            v[i] = i < draws ? pre[i] : 0.0;
+            fl[i] = i < draws ? (int)(v[i] * (double)(left - i)) : 0;
         SIMINF_UNROLL
         for (int i = 0; i < SIMINF_PRE_MAX; i++) {
-            if (i < draws && !done) {
-                const int take = v[i] * dleft < dgood;
-                used = i + 1;
+            if (i < draws) {
+                const int take = fl[i] + k < good;
                 k += take;
-                dgood -= (double)take;
-                if (take && dgood == 0.0)
-                    done = 1;
-                else
-                    dleft -= 1.0;
+                if (take && k == good && used == draws)
+                    used = i + 1;
             }
         }
         s->j += (uint64_t)used;
