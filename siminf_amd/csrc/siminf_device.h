@@ -227,6 +227,11 @@ struct NodeState {
     double rate[M::NT];
     double a0;
     siminf_stream rng;
+    /* The SSA's next uniform pair (Philox block nb of rng), computed one
+     * attempt ahead so that Philox runs beside the current attempt's
+     * log and division instead of before them. */
+    double nu0, nu1;
+    uint64_t nb;
 };
 
 /* What a lane reads while it fires transitions: its node's ldata and the
@@ -395,9 +400,20 @@ __device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
     for (;;) {
         if (s.a0 <= 0.0)
             break;
+        /* siminf_uniform2 for a node stream (never computed ahead), with
+         * the pair of the next attempt started here. */
         double u_tau, u_tr;
-        siminf_uniform2(&s.rng, &u_tau, &u_tr);
-        const double tau = -siminf_log(u_tau) / s.a0;
+        const uint64_t b = (s.rng.j + 1u) >> 1;
+        if (b == s.nb) {
+            u_tau = s.nu0;
+            u_tr = s.nu1;
+        } else {
+            siminf_uniform_pair(&s.rng, b, &u_tau, &u_tr);
+        }
+        s.rng.j = 2u * b + 2u;
+        siminf_uniform_pair(&s.rng, b + 1u, &s.nu0, &s.nu1);
+        s.nb = b + 1u;
+        const double tau = -siminf_log_u01(u_tau) / s.a0;
         if ((tau + t) >= next)
             break;
         t += tau;

@@ -112,6 +112,7 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
             });
         }
         s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+        s.nb = ~0ull;
 
         /* Initial rates (ssm.c:96-117). */
         s.a0 = 0.0;
@@ -585,6 +586,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
             });
         }
         s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+        s.nb = ~0ull;
         s.a0 = 0.0;
         int err = 0;
         static_for<NT>([&](auto J) {

@@ -197,35 +197,12 @@ SIMINF_FN double siminf_from_bits(uint64_t u)
     return c.d;
 }
 
-/*
- * Natural logarithm.  x = 2^e * m with m in [sqrt(2)/2, sqrt(2)); with
- * f = m - 1 (exact) and s = f / (2 + f),
- *     log(m) = 2 atanh(s) = 2s (1 + s^2/3 + s^4/5 + ...),
- * |s| <= 0.1716, and the series is summed to s^24 (truncation below
- * 2^-60 relative).  ln2 is split so that e*LN2_HI is exact.
- */
-SIMINF_FN double siminf_log(double x)
+/* The arithmetic of siminf_log for the bits b of a positive normal
+ * finite number, plus e to its exponent. */
+SIMINF_FN double siminf_log_core(uint64_t b, int e)
 {
     const double LN2_HI = 0x1.62e42fef00000p-1;
     const double LN2_LO = 0x1.473de6af278edp-34;
-    uint64_t b = siminf_bits(x);
-    int e;
-
-    if (x != x)
-        return x;                          /* NaN */
-    if (x < 0.0)
-        return (x - x) / (x - x);          /* NaN */
-    if (x == 0.0)
-        return -1.0 / (x * x);             /* -inf */
-    if (b >= 0x7FF0000000000000ull)
-        return x;                          /* +inf */
-
-    e = 0;
-    if (b < 0x0010000000000000ull) {       /* subnormal: scale by 2^54 */
-        x = x * 0x1p54;
-        b = siminf_bits(x);
-        e = -54;
-    }
     e += (int)(b >> 52) - 1023;
     b &= 0x000FFFFFFFFFFFFFull;
     if (b >= 0x6A09E667F3BCDull) {         /* m >= sqrt(2): use m/2 */
@@ -259,6 +236,44 @@ SIMINF_FN double siminf_log(double x)
             return de * LN2_HI + (two_s + (two_s * r + de * LN2_LO));
         }
     }
+}
+
+/* siminf_log of a uniform in (0, 1) (always normal): the same arithmetic
+ * without the special cases, so that the SSA's waiting time is
+ * straight-line code. */
+SIMINF_FN double siminf_log_u01(double x)
+{
+    return siminf_log_core(siminf_bits(x), 0);
+}
+
+/*
+ * Natural logarithm.  x = 2^e * m with m in [sqrt(2)/2, sqrt(2)); with
+ * f = m - 1 (exact) and s = f / (2 + f),
+ *     log(m) = 2 atanh(s) = 2s (1 + s^2/3 + s^4/5 + ...),
+ * |s| <= 0.1716, and the series is summed to s^24 (truncation below
+ * 2^-60 relative).  ln2 is split so that e*LN2_HI is exact.
+ */
+SIMINF_FN double siminf_log(double x)
+{
+    uint64_t b = siminf_bits(x);
+    int e;
+
+    if (x != x)
+        return x;                          /* NaN */
+    if (x < 0.0)
+        return (x - x) / (x - x);          /* NaN */
+    if (x == 0.0)
+        return -1.0 / (x * x);             /* -inf */
+    if (b >= 0x7FF0000000000000ull)
+        return x;                          /* +inf */
+
+    e = 0;
+    if (b < 0x0010000000000000ull) {       /* subnormal: scale by 2^54 */
+        x = x * 0x1p54;
+        b = siminf_bits(x);
+        e = -54;
+    }
+    return siminf_log_core(b, e);
 }
 
 /* Square root by Newton's iteration from an exponent-halving start
