@@ -16,6 +16,7 @@ import ctypes
 import hashlib
 import os
 import subprocess
+import threading
 
 from . import build as _build
 
@@ -116,9 +117,20 @@ def _digest(text, flags):
     return h.hexdigest()[:24]
 
 
+# Compiles and loads are serialised: the serving loop's worker threads may
+# ask for the same plugin at once (they share the process id the temporary
+# file names carry).
+_lock = threading.RLock()
+
+
 def compile_device(parsed, name="mparse"):
     """Path of the device plugin for a parsed model, compiling it on first
     use (cached under siminf_amd/_plugins/ by a digest of the source)."""
+    with _lock:
+        return _compile_device(parsed, name)
+
+
+def _compile_device(parsed, name):
     src = device_source(parsed, name)
     flags = list(_build.FLAGS)
     key = _digest(src, flags)
@@ -139,6 +151,11 @@ def compile_device(parsed, name="mparse"):
 
 def compile_host(parsed):
     """Host (C) plugin of the same functions, for CPU callers."""
+    with _lock:
+        return _compile_host(parsed)
+
+
+def _compile_host(parsed):
     src = host_source(parsed)
     flags = ["-O2", "-fPIC", "-shared", "-ffp-contract=off"]
     key = _digest(src, flags)
@@ -167,11 +184,12 @@ def clear_cache():
 
 def load_device(so):
     """SimInf_hip_model* of a compiled device plugin."""
-    if so not in _loaded:
-        lib = ctypes.CDLL(so)
-        lib.siminf_plugin_model.restype = ctypes.c_void_p
-        _loaded[so] = (lib, lib.siminf_plugin_model())
-    return _loaded[so][1]
+    with _lock:
+        if so not in _loaded:
+            lib = ctypes.CDLL(so)
+            lib.siminf_plugin_model.restype = ctypes.c_void_p
+            _loaded[so] = (lib, lib.siminf_plugin_model())
+        return _loaded[so][1]
 
 
 class GeneratedPlugin:
