@@ -6,6 +6,10 @@ configuration).  A step is one complete simulation of that model with all
 inputs and the trajectory resident in HBM (SimInf_hip_plan_run); the seed
 changes every step.
 
+Before the timed steps the plan's autotuner picks the node kernels' lanes
+per wavefront (results do not depend on it; the coupled kernel of the
+default workload has no such choice).
+
 Prints one JSON line: node-days/s (value), transitions/s, p50/p99 step
 latency, the roofline of the solver kernel (algorithmic bytes over the
 kernel time measured with HIP events on the plan's stream) and a CPU
@@ -229,6 +233,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sise3_herds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default lanes-per-wavefront rule of the node kernels")
     ap.add_argument("--requests", type=int, default=24,
                     help="pfilter_stream: requests in the timed stream")
     ap.add_argument("--particles", type=int, default=2000)
@@ -288,6 +294,13 @@ def main():
 
     for i in range(args.warmup):
         step(100 + i)
+    # The node kernels' launch shape (lanes per wavefront) from the plan's
+    # autotuner, before the timed steps; coupled kernels have none (-1).
+    best = ctypes.c_int(-1)
+    if not args.no_autotune:
+        err = lib.SimInf_hip_plan_autotune(plan, 200, 2, ctypes.byref(best), None)
+        if err:
+            raise SystemExit("autotune failed: %d" % err)
     del split[:]
     lat, kms, trans = [], [], 0
     node_days = 0
@@ -338,6 +351,7 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "kernel_ms_mean": statistics.mean(kms),
+        "lanes_per_wave_shift": best.value,
         "kernel_ms_split": {"solver": solver_ms, "fill": fill_ms,
                             "fill_node_columns": fill_cols},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -221,6 +221,21 @@ int SimInf_hip_plan_stats(const SimInf_hip_plan *plan,
 int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *plan, double *solver_ms,
                                  double *fill_ms, long long *fill_columns);
 
+/* Lanes per wavefront of the node kernels' active pass, 2^shift
+ * (shift 0..6), or -1 for the default rule (about two wavefronts per
+ * SIMD).  Results do not depend on it: every lane keeps its own stream
+ * (no reference counterpart). */
+int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *plan, int shift);
+
+/* Autotuner of the node kernels' launch shape (no reference counterpart):
+ * runs the plan reps times (seeds seed, seed+1, ...) for each lanes-per-
+ * wavefront shift 0..6, keeps the one with the least mean solver time and
+ * returns it (-1 for plans whose kernel has no such choice); ms, if not
+ * NULL, gets the 7 mean times.  The last run's trajectory is the last
+ * candidate's. */
+int SimInf_hip_plan_autotune(SimInf_hip_plan *plan, unsigned long int seed, int reps,
+                             int *best_shift, double *ms);
+
 void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
 
 /* Bootstrap particle filter with the state of every particle kept in HBM

@@ -86,6 +86,8 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_prevalence",
     "SimInf_hip_plan_trajectory",
     "SimInf_hip_abc_weights",
+    "SimInf_hip_plan_set_lanes_per_wave",
+    "SimInf_hip_plan_autotune",
     "SimInf_hip_plan_destroy",
 )
 
@@ -147,6 +149,10 @@ def load(path=None):
     lib.SimInf_hip_abc_weights.restype = ctypes.c_int
     lib.SimInf_hip_abc_weights.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int,
                                            ctypes.c_int, dp, dp, dp, dp, dp]
+    lib.SimInf_hip_plan_set_lanes_per_wave.restype = ctypes.c_int
+    lib.SimInf_hip_plan_set_lanes_per_wave.argtypes = [vp, ctypes.c_int]
+    lib.SimInf_hip_plan_autotune.restype = ctypes.c_int
+    lib.SimInf_hip_plan_autotune.argtypes = [vp, ctypes.c_ulong, ctypes.c_int, ip, dp]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
     if path == LIB_PATH:

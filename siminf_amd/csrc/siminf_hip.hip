@@ -303,6 +303,10 @@ struct SimInf_hip_plan {
     int ndays;
     uint64_t host_err_key;     /* errors found while validating events */
 
+    /* Lanes per wavefront of the active pass as a power of two, or -1
+     * for the rule of about two wavefronts per SIMD. */
+    int lpw_fixed = -1;
+
     /* Last run. */
     bool has_run = false;       /* a run completed without error */
     unsigned long long transitions = 0;
@@ -933,6 +937,8 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
             int shift = 0;
             while (shift < 6 && ((long long)n_active >> shift) > 2LL * P->simds)
                 shift++;
+            if (P->lpw_fixed >= 0)
+                shift = P->lpw_fixed;
             p.mode = NODES_ACTIVE;
             p.n_active = n_active;
             p.lpw_shift = shift;
@@ -1054,6 +1060,51 @@ extern "C" int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *P,
         *fill_ms = P->fill_ms;
     if (fill_columns)
         *fill_columns = (long long)P->fill_cols;
+    return 0;
+}
+
+extern "C" int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *P, int shift)
+{
+    if (!P || shift < -1 || shift > 6)
+        return SIMINF_ERR_INVALID_MODEL;
+    P->lpw_fixed = shift;
+    return 0;
+}
+
+extern "C" int SimInf_hip_plan_autotune(SimInf_hip_plan *P, unsigned long int seed, int reps,
+                                        int *best_shift, double *ms)
+{
+    if (!P || reps < 1)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES) {
+        /* Nothing to tune: the coupled kernels have one lane per node. */
+        if (best_shift)
+            *best_shift = -1;
+        return 0;
+    }
+    int best = -1;
+    double best_ms = INFINITY;
+    for (int shift = 0; shift <= 6; shift++) {
+        P->lpw_fixed = shift;
+        double sum = 0.0;
+        for (int r = 0; r < reps; r++) {
+            const int err = SimInf_hip_plan_run(P, seed + (unsigned long)r);
+            if (err) {
+                P->lpw_fixed = -1;
+                return err;
+            }
+            sum += P->solver_ms;
+        }
+        if (ms)
+            ms[shift] = sum / reps;
+        if (sum / reps < best_ms) {
+            best_ms = sum / reps;
+            best = shift;
+        }
+    }
+    P->lpw_fixed = best;
+    if (best_shift)
+        *best_shift = best;
     return 0;
 }
 

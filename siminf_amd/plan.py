@@ -61,6 +61,20 @@ class Plan:
                                        ctypes.byref(ms))
         return tr.value, nd.value, ms.value
 
+    def set_lanes_per_wave(self, shift):
+        """2^shift lanes per wavefront in the node kernels' active pass (-1:
+        the default rule)."""
+        self._check(self.lib.SimInf_hip_plan_set_lanes_per_wave(self._handle, int(shift)))
+
+    def autotune(self, seed=1, reps=2):
+        """Try every lanes-per-wavefront shift, keep the fastest; returns
+        (best shift, mean solver ms per shift)."""
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * 7)()
+        self._check(self.lib.SimInf_hip_plan_autotune(self._handle, int(seed), int(reps),
+                                                      ctypes.byref(best), ms))
+        return best.value, list(ms)
+
     def download(self):
         """Copy the last run's U/V into the model (as run() leaves them)."""
         b = self._bundle
