@@ -1314,8 +1314,18 @@ __global__ __launch_bounds__(64) void k_pf_weights(const int *U, int Nn, int Nc,
 __device__ __forceinline__ double pf_serial_sum(const double *x, int n)
 {
     double s = 0.0;
-#pragma unroll 8
-    for (int i = 0; i < n; i++)
+    int i = 0;
+    /* Sixteen loads issued together, then the dependent adds. */
+    for (; i + 16 <= n; i += 16) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            v[k] = x[i + k];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            s = s + v[k];
+    }
+    for (; i < n; i++)
         s = s + x[i];
     return s;
 }
@@ -1388,8 +1398,26 @@ __global__ __launch_bounds__(256) void k_pf_resample(double *w, double *c_global
         /* ESS, the validity pass and the running sums; then the u_i. */
         double s2 = 0.0, cs = 0.0;
         int r = 0;
-#pragma unroll 8
-        for (int i = 0; i < n; i++) {
+        int i = 0;
+        for (; i + 16 <= n; i += 16) {
+            double v[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                v[k] = c[i + k];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const double x = v[k];
+                s2 = s2 + x * x;
+                if (!(x >= 0.0) || x == INFINITY)
+                    r = -1;
+                cs = cs + x;
+                v[k] = cs;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                c[i + k] = v[k];
+        }
+        for (; i < n; i++) {
             const double x = c[i];
             s2 = s2 + x * x;
             if (!(x >= 0.0) || x == INFINITY)
