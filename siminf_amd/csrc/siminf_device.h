@@ -389,12 +389,70 @@ __device__ __forceinline__ int fire_any(NodeState<M> &s, const CT &cx, int tr,
     return err;
 }
 
+/* One SSA firing at time t: the transition u_tr picks by a linear scan
+ * of the cumulative rates (ssm.c:160-189) is applied.  Returns false when
+ * the node stops: a0 fell to zero, or the step failed (err). */
+template <class M, class CT>
+__device__ __forceinline__ bool ssa_fire(NodeState<M> &s, const CT &cx, double t,
+                                         double u_tr, unsigned long long &fired,
+                                         int &err)
+{
+    constexpr int NT = M::NT;
+    /* The rate of the selected transition is carried along (no run-time
+     * indexing). */
+    const double rnd = u_tr * s.a0;
+    int tr = 0;
+    double rt = s.rate[0];
+    double cum = s.rate[0];
+    bool searching = rnd > cum;
+    static_for<NT>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if (k > 0 && searching) {
+            tr = k;
+            rt = s.rate[k];
+            cum += s.rate[k];
+            searching = rnd > cum;
+        }
+    });
+    /* Floating point fix (ssm.c:170-189): past the end means the last
+     * transition, which the scan already holds; a zero rate walks back to
+     * the nearest non-zero one below it. */
+    if (rt == 0.0) {
+        static_for<NT>([&](auto K) {
+            constexpr int k = NT - 1 - decltype(K)::value;
+            if (k > 0 && tr == k && s.rate[k] == 0.0) {
+                tr = k - 1;
+                rt = s.rate[k - 1];
+            }
+        });
+        if (rt == 0.0) {
+            s.a0 = 0.0;
+            return false;
+        }
+    }
+
+    fired++;
+    if constexpr (CT::canon && NT <= 8) {
+        err = fire_any<M>(s, cx, tr, t);
+    } else {
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if (tr == k) {
+                const int e = fire<M, k>(s, cx, t);
+                if (e)
+                    err = e;
+            }
+        });
+    }
+    /* The node stops at its first failing step. */
+    return err == 0;
+}
+
 template <class M, class CT>
 __device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
                                        double tt, double next,
                                        unsigned long long &fired)
 {
-    constexpr int NT = M::NT;
     int err = 0;
     double t = tt;
     for (;;) {
@@ -417,57 +475,118 @@ __device__ __forceinline__ int ssa_day(NodeState<M> &s, const CT &cx,
         if ((tau + t) >= next)
             break;
         t += tau;
-
-        /* Linear scan of the cumulative rates; the rate of the selected
-         * transition is carried along (no run-time indexing). */
-        const double rnd = u_tr * s.a0;
-        int tr = 0;
-        double rt = s.rate[0];
-        double cum = s.rate[0];
-        bool searching = rnd > cum;
-        static_for<NT>([&](auto K) {
-            constexpr int k = decltype(K)::value;
-            if (k > 0 && searching) {
-                tr = k;
-                rt = s.rate[k];
-                cum += s.rate[k];
-                searching = rnd > cum;
-            }
-        });
-        /* Floating point fix (ssm.c:170-189): past the end means the last
-         * transition, which the scan already holds; a zero rate walks
-         * back to the nearest non-zero one below it. */
-        if (rt == 0.0) {
-            static_for<NT>([&](auto K) {
-                constexpr int k = NT - 1 - decltype(K)::value;
-                if (k > 0 && tr == k && s.rate[k] == 0.0) {
-                    tr = k - 1;
-                    rt = s.rate[k - 1];
-                }
-            });
-            if (rt == 0.0) {
-                s.a0 = 0.0;
-                break;
-            }
-        }
-
-        fired++;
-        if constexpr (CT::canon && NT <= 8) {
-            err = fire_any<M>(s, cx, tr, t);
-        } else {
-            static_for<NT>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                if (tr == k) {
-                    const int e = fire<M, k>(s, cx, t);
-                    if (e)
-                        err = e;
-                }
-            });
-        }
-        /* The node stops at its first failing step. */
-        if (err)
+        if (!ssa_fire<M>(s, cx, t, u_tr, fired, err))
             break;
     }
+    return err;
+}
+
+/* Lane l's value v from lane src of the wavefront. */
+__device__ __forceinline__ uint32_t lane_pull(int src, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+
+__device__ __forceinline__ double lane_pull(int src, double v)
+{
+    const uint64_t b = siminf_bits(v);
+    const uint32_t lo = lane_pull(src, (uint32_t)b);
+    const uint32_t hi = lane_pull(src, (uint32_t)(b >> 32));
+    return siminf_from_bits(((uint64_t)hi << 32) | lo);
+}
+
+/* Position of the (x+1)-th set bit of m (x < popcount(m)). */
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int x)
+{
+    int p = 0;
+    uint32_t w = (uint32_t)m;
+    int c = __popc(w);
+    if (x >= c) {
+        x -= c;
+        p = 32;
+        w = (uint32_t)(m >> 32);
+    }
+    c = __popc(w & 0xFFFFu);
+    if (x >= c) { x -= c; p += 16; w >>= 16; }
+    c = __popc(w & 0xFFu);
+    if (x >= c) { x -= c; p += 8; w >>= 8; }
+    c = __popc(w & 0xFu);
+    if (x >= c) { x -= c; p += 4; w >>= 4; }
+    c = __popc(w & 0x3u);
+    if (x >= c) { x -= c; p += 2; w >>= 2; }
+    return p + (x >= (int)(w & 1u) ? 1 : 0);
+}
+
+/* ssa_day for a whole wavefront: every lane enters, live says whether it
+ * holds a node.  The attempts' Philox blocks and waiting-time logarithms
+ * are computed by all 64 lanes together in refills: with A lanes still in
+ * their day, lane l computes attempt k = l / A of the running lane of
+ * rank l % A, so each running lane holds up to 64 / A attempts after one
+ * refill and fetches them from the lanes that computed them
+ * (ds_bpermute).  A refill runs when some running lane has used its
+ * share.  A wavefront's day lasts as long as its busiest lane's, and most
+ * of that lane's attempts come when few lanes are left, so the Philox
+ * block and the logarithm, which are most of an attempt's instructions,
+ * leave the busiest lane's chain.  Stream positions and every value are
+ * those of ssa_day. */
+template <class M, class CT>
+__device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
+                                            double tt, double next,
+                                            unsigned long long &fired,
+                                            bool live)
+{
+    int err = 0;
+    double t = tt;
+    bool run = live && s.a0 > 0.0;
+    const uint64_t b0 = live ? (s.rng.j + 1u) >> 1 : 0u;
+    uint64_t b = b0;               /* this lane's next block */
+    double hE = 0.0, hU = 0.0;     /* the attempt this lane computed */
+    int A = 1, rank = 0, pos = 0, have = 0;
+    const int lane = (int)__lane_id();
+    for (;;) {
+        const uint64_t running = __ballot(run);
+        if (running == 0)
+            break;
+        if (__ballot(run && pos >= have) != 0) {
+            A = __popcll(running);
+            const int c = nth_set_bit(running, lane % A);
+            const int k = lane / A;
+            siminf_stream st;
+            st.key0 = lane_pull(c, s.rng.key0);
+            st.key1 = lane_pull(c, s.rng.key1);
+            st.id0 = lane_pull(c, s.rng.id0);
+            st.id1 = lane_pull(c, s.rng.id1);
+            const uint64_t bc = ((uint64_t)lane_pull(c, (uint32_t)(b >> 32)) << 32) |
+                                lane_pull(c, (uint32_t)b);
+            double u0, u1;
+            siminf_uniform_pair(&st, bc + (uint64_t)k, &u0, &u1);
+            hE = -siminf_log_u01(u0);
+            hU = u1;
+            rank = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(running >> 32),
+                __builtin_amdgcn_mbcnt_lo((uint32_t)running, 0u));
+            have = (64 - rank + A - 1) / A;
+            pos = 0;
+        }
+        /* Every lane takes part in the pulls; only running lanes use
+         * them. */
+        const int src = run ? pos * A + rank : lane;
+        const double E = lane_pull(src, hE);
+        const double u_tr = lane_pull(src, hU);
+        if (run) {
+            b++;
+            pos++;
+            const double tau = E / s.a0;
+            if ((tau + t) >= next) {
+                run = false;
+            } else {
+                t += tau;
+                run = ssa_fire<M>(s, cx, t, u_tr, fired, err);
+            }
+        }
+    }
+    if (live && b != b0)
+        s.rng.j = 2u * b;
     return err;
 }
 

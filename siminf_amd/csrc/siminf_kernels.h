@@ -531,6 +531,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     NodeState<M> s;
     Ctx<M, CANON> cx;
     E1Cursor ec;
+    s.a0 = 0.0;
 
     const EvTab tab = global_tables(p);
 
@@ -609,11 +610,11 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         return;
 
     for (int day = 0;; day++) {
-        /* A: SSA and E1 events. */
+        /* A: SSA (the whole wavefront takes part) and E1 events. */
+        int err = ssa_day_wave<M>(s, cx, tt, next, fired, active);
         if (active) {
             bool upd = false;
             int bad = 0;
-            int err = ssa_day<M>(s, cx, tt, next, fired);
             if (err) {
                 record_error(p, rep, day, PH_SSA, node, err);
                 myerr = 1;
