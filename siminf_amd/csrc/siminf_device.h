@@ -107,7 +107,9 @@ struct KParams {
      *   where the slots index the herds of the day's touched list
      *   e2_tn[e2_tn_ptr[d] .. e2_tn_ptr[d+1]) (dst slot -1 for an invalid
      *   destination).  select and shift are zero-based, shift -1 for
-     *   none. */
+     *   none; an event the fast path takes has shift SIMINF_E2_FAST and
+     *   its E column's rows packed in place of the select (first row |
+     *   second row << 8 | (two rows) << 16). */
     const int *ev_rank;
     const int *e1_ptr;
     const int4 *e1_rec;
@@ -129,7 +131,7 @@ struct KParams {
      * e2_cap_t touched herds, e2_cap_ev events and e2_cap_lv levels
      * (other days run from global memory). */
     int e2_cap_t, e2_cap_ev, e2_cap_lv;
-    int lds_u, lds_upd, lds_rec, lds_prop, lds_lvl, lds_pre;
+    int lds_u, lds_upd, lds_tn, lds_rec, lds_prop, lds_lvl, lds_pre;
     /* E and N for the E2 work, staged once per kernel (zero caps when
      * they do not fit). */
     int lds_Ew, lds_Emask, lds_N;
@@ -1084,15 +1086,11 @@ __device__ int apply_e1(const EvTab &tab, int type, int n, int select, int shift
  * compartment loops of the general case; the same draws, the same
  * results and errors. */
 template <int NC>
-__device__ __forceinline__ int apply_e2_fast(const EvTab &tab, int n, int select, int *uw,
-                                             int *upd, int src, int dst,
-                                             siminf_stream *rng)
+__device__ __forceinline__ int apply_e2_fast(int n, int rows, int *uw, int *upd,
+                                             int src, int dst, siminf_stream *rng)
 {
-    const uint32_t mask = tab.Emask[select];
-    const int ra = mask_first(mask);
-    const uint32_t rest = mask & (mask - 1u);
-    const bool two = rest != 0u;
-    const int rb = two ? mask_first(rest) : ra;
+    const int ra = rows & 0xFF, rb = (rows >> 8) & 0xFF;
+    const bool two = (rows >> 16) != 0;
     int *sa = uw + src * NC + ra, *sb = uw + src * NC + rb;
     int *da = uw + dst * NC + ra, *db = uw + dst * NC + rb;
     const int ua = *sa, ub = two ? *sb : 0;
@@ -1115,6 +1113,83 @@ __device__ __forceinline__ int apply_e2_fast(const EvTab &tab, int n, int select
     bool neg = (va < 0) | (wa < 0);
     if (two) {
         const int vb = *db + xb, wb = ub - xb;
+        *db = vb;
+        *sb = wb;
+        neg = neg | (vb < 0) | (wb < 0);
+    }
+    if (neg)
+        return SIMINF_ERR_NEGATIVE_STATE;
+    upd[dst] = 1;
+    upd[src] = 1;
+    return 0;
+}
+
+/* apply_e2_fast for an event whose stream's first SIMINF_PRE_MAX uniforms
+ * are in pre.  Both herds' counts and the uniforms are loaded together
+ * and the urn is computed before any decision (its result is used only
+ * where siminf_hypergeometric would be called), as in that function's
+ * computed-ahead branch: with g_i = ua - floor(u_i * (total - i)), draw i
+ * takes a good item exactly when k < g_i, so after the independent g_i
+ * the chain is one compare and one add per draw.  The event's stream is
+ * used by nothing else, so its position is not kept; an urn of more
+ * draws than were computed ahead takes the stream. */
+template <int NC>
+__device__ __forceinline__ int apply_e2_fast_pre(int n, int rows, int *uw, int *upd,
+                                                 int src, int dst, const double *pre,
+                                                 siminf_stream *rng)
+{
+    const int ra = rows & 0xFF, rb = (rows >> 8) & 0xFF;
+    const bool two = (rows >> 16) != 0;
+    int *sa = uw + src * NC + ra, *sb = uw + src * NC + rb;
+    int *da = uw + dst * NC + ra, *db = uw + dst * NC + rb;
+    const int ua = *sa, ub0 = *sb, dva = *da, dvb = *db;
+    double v[SIMINF_PRE_MAX];
+    SIMINF_UNROLL
+    for (int i = 0; i < SIMINF_PRE_MAX; i++)
+        v[i] = pre[i];
+    /* Keep the loads here, ahead of the branches below (the compiler
+     * would otherwise sink the uniforms' loads into the urn's branch, a
+     * second round trip after the counts'). */
+    SIMINF_UNROLL
+    for (int i = 0; i < SIMINF_PRE_MAX; i++)
+        asm volatile("" ::"v"(v[i]));
+    const int ub = two ? ub0 : 0;
+    const int total = ua + ub;
+    const bool low = 2 * n <= total;
+    const int draws = low ? n : total - n;
+    const double dt = (double)total;
+    int g[SIMINF_PRE_MAX];
+    SIMINF_UNROLL
+    for (int i = 0; i < SIMINF_PRE_MAX; i++)
+        g[i] = i < draws ? ua - (int)(v[i] * (dt - (double)i)) : -1;
+    /* k + (k < g) as med3(k, k + 1, g). */
+    int k = 0;
+    SIMINF_UNROLL
+    for (int i = 0; i < SIMINF_PRE_MAX; i++)
+        k = min(k + 1, max(k, g[i]));
+    if (n > total)
+        return SIMINF_ERR_SAMPLE_SELECT;
+    int xa;
+    if (n == total) {
+        xa = ua;
+    } else if (!two || ub <= 0) {
+        xa = n;
+    } else if (ua <= 0) {
+        xa = 0;
+    } else if (draws <= SIMINF_PRE_MAX) {
+        xa = low ? k : ua - k;
+    } else {
+        rng->pre = pre;
+        rng->npre = SIMINF_PRE_MAX;
+        xa = siminf_hypergeometric(rng, ua, ub, n);
+    }
+    const int xb = n - xa;
+    const int va = dva + xa, wa = ua - xa;
+    *da = va;
+    *sa = wa;
+    bool neg = (va < 0) | (wa < 0);
+    if (two) {
+        const int vb = dvb + xb, wb = ub - xb;
         *db = vb;
         *sb = wb;
         neg = neg | (vb < 0) | (wb < 0);

@@ -366,14 +366,14 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
         tab = align16(tab + (size_t)std::max(Nshift, 1) * Nc * sizeof(int));
     if (tab > budget / 4) {
         p.e2_cap_t = p.e2_cap_ev = p.e2_cap_lv = 0;
-        p.lds_u = p.lds_upd = p.lds_rec = p.lds_prop = p.lds_lvl = p.lds_pre = 0;
+        p.lds_u = p.lds_upd = p.lds_tn = p.lds_rec = p.lds_prop = p.lds_lvl = p.lds_pre = 0;
         return 16;
     }
     budget -= tab;
-    const size_t per_t = (size_t)(Nc + 1) * sizeof(int);
+    const size_t per_t = (size_t)(Nc + 2) * sizeof(int);
     const size_t per_ev = 2 * sizeof(int4) + (1 + SIMINF_E2_PRE) * sizeof(double);
     auto need = [&](size_t t, size_t ev, size_t lv) {
-        return align16(t * Nc * sizeof(int)) + align16(t * sizeof(int)) +
+        return align16(t * Nc * sizeof(int)) + 2 * align16(t * sizeof(int)) +
                ev * 2 * sizeof(int4) + align16(ev * sizeof(double)) +
                align16((lv + 1) * sizeof(int)) +
                ev * SIMINF_E2_PRE * sizeof(double);
@@ -393,6 +393,8 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
     p.lds_u = (int)off;
     off = align16(off + cap_t * Nc * sizeof(int));
     p.lds_upd = (int)off;
+    off = align16(off + cap_t * sizeof(int));
+    p.lds_tn = (int)off;
     off = align16(off + cap_t * sizeof(int));
     p.lds_rec = (int)off;
     off = align16(off + cap_ev * 2 * sizeof(int4));
@@ -628,10 +630,19 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         const int r0 = __builtin_ctz(m), r1 = __builtin_ctz(m & (m - 1u));
         return Ew[(size_t)sel * Nc + r0] == Ew[(size_t)sel * Nc + r1];
     };
+    /* A fast event carries its E column's rows in place of the select:
+     * first row | second row << 8 | (two rows) << 16. */
+    auto e2_rows = [&](int k) {
+        const uint32_t m = Emask[ev_select[k]];
+        const int r0 = __builtin_ctz(m);
+        const uint32_t rest = m & (m - 1u);
+        return rest ? (r0 | (__builtin_ctz(rest) << 8) | (1 << 16)) : (r0 | (r0 << 8));
+    };
     for (size_t i = 0; i < e2_idx.size(); i++) {
         const int k = e2_idx[i];
-        e2_rec[2 * i] = int4{k, a->n[k], ev_select[k],
-                             e2_fast(k) ? SIMINF_E2_FAST : ev_shift[k]};
+        const bool fast = e2_fast(k);
+        e2_rec[2 * i] = int4{k, a->n[k], fast ? e2_rows(k) : ev_select[k],
+                             fast ? SIMINF_E2_FAST : ev_shift[k]};
         e2_rec[2 * i + 1] = int4{e2_lsrc[k], e2_ldst[k], ev_node[k], ev_dest[k]};
         e2_prop[i] = a->proportion[k];
     }
@@ -799,7 +810,8 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
             cus = 1;
         size_t budget = std::min<size_t>((size_t)lds_max, 128 * 1024);
         if (P->kernel == K_COUPLED_GRID) {
-            const int blocks = (Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK;
+            /* The herds' workgroups and the one that runs E2. */
+            const int blocks = (Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1;
             const int per_cu = (blocks + cus - 1) / cus;
             const size_t share = (size_t)lds_cu / (size_t)per_cu;
             budget = std::min<size_t>(budget, share > 2048 ? share - 1024 : 1024);
@@ -809,7 +821,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                                     budget);
         if (P->kernel == K_COUPLED_GRID) {
             P->block = SIMINF_GRID_BLOCK;
-            P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK);
+            P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1);
             int max_blocks = 0;
             if (model->coupled(p, P->canon, true, P->grid, P->block, P->lds,
                                nullptr, &max_blocks) != hipSuccess ||

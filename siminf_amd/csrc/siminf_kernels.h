@@ -404,7 +404,7 @@ __device__ __forceinline__ int e2_levels(const KParams &p, const EvTab &tab,
             }
             const int err =
                 r0.w == SIMINF_E2_FAST
-                    ? apply_e2_fast<NC>(tab, r0.y, r0.z, su, sf, LDS ? r1.x : r1.z,
+                    ? apply_e2_fast<NC>(r0.y, r0.z, su, sf, LDS ? r1.x : r1.z,
                                         LDS ? r1.y : r1.w, &rng)
                     : apply_e2<NC>(tab, r0.y, r0.z, r0.w, prop[i], r1.w, p.Nn, su, sf,
                                    LDS ? r1.x : r1.z, LDS ? r1.y : r1.w, &rng);
@@ -418,63 +418,132 @@ __device__ __forceinline__ int e2_levels(const KParams &p, const EvTab &tab,
     return myerr;
 }
 
-/* One day's external transfers of replicate rep, run by one workgroup.
- * The day's event records, level bounds and the herds they touch are
- * staged in LDS with coalesced loads, the conflict-free levels run there
- * with a workgroup barrier between levels, and the herds are scattered
- * back with their update flags.  Events in one level touch disjoint
- * herds, so no atomics are needed; within a herd the events keep list
- * order.  A day larger than the LDS buffers runs the same levels on the
- * global state and records.  Returns 1 if this thread recorded an
- * error. */
+/* The levels of a staged day whose events all take the fast path, run by
+ * one wavefront (lane = event within a level; a level wider than the
+ * wavefront takes more rounds).  A wavefront's LDS accesses complete in
+ * issue order, so a level reads what the one before it wrote without a
+ * barrier.  The level bounds sit in two registers (read with
+ * v_readlane), and the next level's records are loaded while the current
+ * level runs, so the chain from level to level is one round of LDS loads
+ * (both herds' counts and the event's uniforms), the urn and the stores.
+ * Needs nlv < 128.  Returns 1 if this lane recorded an error. */
 template <int NC>
-__device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int rep, int day,
-                      int *uw, int *updw, unsigned char *lds)
+__device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day, int *su,
+                                              int *sf, const int4 *rec, const int *lvl,
+                                              int nlv, int i0, const double *pre)
 {
+    const int lane = (int)threadIdx.x;
+    int myerr = 0;
+    const int bnd0 = lane <= nlv ? lvl[lane] - i0 : 0;
+    const int bnd1 = 64 + lane <= nlv ? lvl[64 + lane] - i0 : 0;
+    auto bound = [&](int l) -> int {
+        return l < 64 ? __builtin_amdgcn_readlane(bnd0, l)
+                      : __builtin_amdgcn_readlane(bnd1, l - 64);
+    };
+    auto process = [&](int i, int ev, int n, int rows, int src, int dst) {
+        siminf_stream rng = siminf_stream_make(
+            p.seed, SIMINF_DOMAIN_E2, (uint64_t)rep * (uint64_t)p.len + (uint64_t)ev);
+        const int err = apply_e2_fast_pre<NC>(n, rows, su, sf, src, dst,
+                                              pre + (long long)i * SIMINF_E2_PRE, &rng);
+        if (err) {
+            record_error(p, rep, day, PH_E2, p.ev_rank[ev], err);
+            myerr = 1;
+        }
+    };
+    int a = bound(0), b = bound(1);
+    bool has = lane < b - a;
+    int4 c0 = int4{0, 0, 0, 0}, c1 = int4{0, 0, 0, 0};
+    if (has) {
+        c0 = rec[2 * (a + lane)];
+        c1 = rec[2 * (a + lane) + 1];
+    }
+    for (int l = 0; l < nlv; l++) {
+        /* This level's record, loaded during the level before, is taken
+         * into registers before the next level's loads are issued: the
+         * wait for it is then not a wait for those. */
+        const int ev = c0.x, n = c0.y, rows = c0.z, src = c1.x, dst = c1.y;
+        asm volatile("" ::"v"(ev), "v"(n), "v"(rows), "v"(src), "v"(dst) : "memory");
+        const bool more = l + 1 < nlv;
+        const int an = more ? bound(l + 1) : 0, bn = more ? bound(l + 2) : 0;
+        const bool hasn = lane < bn - an;
+        c0 = int4{0, 0, 0, 0};
+        c1 = int4{0, 0, 0, 0};
+        if (hasn) {
+            c0 = rec[2 * (an + lane)];
+            c1 = rec[2 * (an + lane) + 1];
+        }
+        if (has)
+            process(a + lane, ev, n, rows, src, dst);
+        for (int i = a + lane + 64; i < b; i += 64) {
+            const int4 r0 = rec[2 * i], r1 = rec[2 * i + 1];
+            process(i, r0.x, r0.y, r0.z, r1.x, r1.y);
+        }
+        has = hasn;
+        a = an;
+        b = bn;
+    }
+    return myerr;
+}
+
+/* A day's external transfers as staged for the workgroup that runs them:
+ * the day's levels and touched herds, and where the levels run. */
+struct E2Day {
+    int l0, nlv, i0, nev, t0, nt;
+    int mode; /* E2_NONE, E2_GLOBAL, E2_LDS (any events), E2_LDS_FAST */
+};
+enum { E2_NONE = 0, E2_GLOBAL = 1, E2_LDS = 2, E2_LDS_FAST = 3 };
+
+/* The part of a day's E2 staging that does not depend on the herds'
+ * state: the day's event records, proportions, level bounds and touched
+ * herds copied to LDS with coalesced loads, and the first uniforms of
+ * every event's stream computed in parallel (so that the levels, which
+ * run one after another, find them in LDS instead of each running Philox
+ * on the critical path).  A day larger than the LDS buffers runs from
+ * global memory.  Called by the whole workgroup; in a grid launch the
+ * workgroup that runs E2 does this while the others run the day's SSA. */
+template <int NC>
+__device__ __forceinline__ E2Day e2_stage(const KParams &p, int rep, int day,
+                                          unsigned char *lds)
+{
+    E2Day d{0, 0, 0, 0, 0, 0, E2_NONE};
     /* The plan's day (tables) and the run's day (error order). */
     const int pd = day + p.day0;
     if (pd >= p.ndays || pd <= p.e2_floor)
-        return 0;
-    const int l0 = p.e2_day[pd], nlv = p.e2_day[pd + 1] - l0;
-    if (nlv == 0)
-        return 0;
-    const int i0 = p.e2_lvl[l0], nev = p.e2_lvl[l0 + nlv] - i0;
-    const int t0 = p.e2_tn_ptr[pd], nt = p.e2_tn_ptr[pd + 1] - t0;
-    const int4 *rec = p.e2_rec + 2 * (long long)i0;
-    const double *prop = p.e2_prop + i0;
-    const int *lvl = p.e2_lvl + l0;
-    if (!(nt <= p.e2_cap_t && nev <= p.e2_cap_ev && nlv <= p.e2_cap_lv))
-        return e2_levels<NC, false>(p, tab, rep, day, uw, updw, rec, prop, lvl,
-                                    nlv, i0, nullptr);
-
+        return d;
+    d.l0 = p.e2_day[pd];
+    d.nlv = p.e2_day[pd + 1] - d.l0;
+    if (d.nlv == 0)
+        return d;
+    d.i0 = p.e2_lvl[d.l0];
+    d.nev = p.e2_lvl[d.l0 + d.nlv] - d.i0;
+    d.t0 = p.e2_tn_ptr[pd];
+    d.nt = p.e2_tn_ptr[pd + 1] - d.t0;
+    if (!(d.nt <= p.e2_cap_t && d.nev <= p.e2_cap_ev && d.nlv <= p.e2_cap_lv)) {
+        d.mode = E2_GLOBAL;
+        return d;
+    }
+    const int4 *rec = p.e2_rec + 2 * (long long)d.i0;
+    const double *prop = p.e2_prop + d.i0;
+    const int *lvl = p.e2_lvl + d.l0;
     const int tid = (int)threadIdx.x, nth = (int)blockDim.x;
-    const EvTab ltab{(const uint32_t *)(lds + p.lds_Emask),
-                     (const double *)(lds + p.lds_Ew),
-                     p.Nmat ? (const int *)(lds + p.lds_N) : nullptr};
-    int *lu = (int *)(lds + p.lds_u);
-    int *lupd = (int *)(lds + p.lds_upd);
     int4 *lrec = (int4 *)(lds + p.lds_rec);
     double *lprop = (double *)(lds + p.lds_prop);
     int *llvl = (int *)(lds + p.lds_lvl);
+    int *ltn = (int *)(lds + p.lds_tn);
     double *lpre = (double *)(lds + p.lds_pre);
-    for (int i = tid; i < nt; i += nth) {
-        const int node = p.e2_tn[t0 + i];
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            lu[i * NC + c] = uw[node * NC + c];
-        });
-        lupd[i] = 0;
+    int slow = 0;
+    for (int i = tid; i < 2 * d.nev; i += nth) {
+        const int4 r = rec[i];
+        lrec[i] = r;
+        slow |= (i & 1) == 0 && r.w != SIMINF_E2_FAST;
     }
-    for (int i = tid; i < 2 * nev; i += nth)
-        lrec[i] = rec[i];
-    for (int i = tid; i < nev; i += nth)
+    for (int i = tid; i < d.nev; i += nth)
         lprop[i] = prop[i];
-    for (int i = tid; i <= nlv; i += nth)
+    for (int i = tid; i <= d.nlv; i += nth)
         llvl[i] = lvl[i];
-    /* The first uniforms of every event's stream, computed in parallel
-     * here so that the levels, which run one after another, find them in
-     * LDS instead of each running Philox on the critical path. */
-    for (int q = tid; q < nev * (SIMINF_E2_PRE / 2); q += nth) {
+    for (int i = tid; i < d.nt; i += nth)
+        ltn[i] = p.e2_tn[d.t0 + i];
+    for (int q = tid; q < d.nev * (SIMINF_E2_PRE / 2); q += nth) {
         const int i = q / (SIMINF_E2_PRE / 2), b = q % (SIMINF_E2_PRE / 2);
         const siminf_stream st = siminf_stream_make(
             p.seed, SIMINF_DOMAIN_E2,
@@ -484,11 +553,62 @@ __device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int re
         lpre[i * SIMINF_E2_PRE + 2 * b] = u0;
         lpre[i * SIMINF_E2_PRE + 2 * b + 1] = u1;
     }
+    d.mode = (__syncthreads_or(slow) == 0 && d.nlv < 128) ? E2_LDS_FAST : E2_LDS;
+    return d;
+}
+
+/* One day's external transfers of replicate rep, run by one workgroup
+ * after e2_stage: the touched herds are loaded into LDS, the
+ * conflict-free levels run there (one wavefront without barriers when
+ * every event takes the fast path, else a workgroup barrier between
+ * levels; no atomics: events in one level touch disjoint herds, and
+ * within a herd the events keep list order), and the herds are
+ * scattered back with their update flags.  Returns 1 if this thread
+ * recorded an error. */
+template <int NC>
+__device__ __forceinline__ int e2_run(const KParams &p, const EvTab &tab, int rep, int day,
+                                      int *uw, int *updw, unsigned char *lds,
+                                      const E2Day &d)
+{
+    if (d.mode == E2_NONE)
+        return 0;
+    if (d.mode == E2_GLOBAL)
+        return e2_levels<NC, false>(p, tab, rep, day, uw, updw,
+                                    p.e2_rec + 2 * (long long)d.i0, p.e2_prop + d.i0,
+                                    p.e2_lvl + d.l0, d.nlv, d.i0, nullptr);
+
+    const int tid = (int)threadIdx.x, nth = (int)blockDim.x;
+    const EvTab ltab{(const uint32_t *)(lds + p.lds_Emask),
+                     (const double *)(lds + p.lds_Ew),
+                     p.Nmat ? (const int *)(lds + p.lds_N) : nullptr};
+    int *lu = (int *)(lds + p.lds_u);
+    int *lupd = (int *)(lds + p.lds_upd);
+    const int4 *lrec = (const int4 *)(lds + p.lds_rec);
+    const double *lprop = (const double *)(lds + p.lds_prop);
+    const int *llvl = (const int *)(lds + p.lds_lvl);
+    const int *ltn = (const int *)(lds + p.lds_tn);
+    const double *lpre = (const double *)(lds + p.lds_pre);
+    for (int i = tid; i < d.nt; i += nth) {
+        const int node = ltn[i];
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            lu[i * NC + c] = uw[node * NC + c];
+        });
+        lupd[i] = 0;
+    }
     __syncthreads();
-    const int myerr = e2_levels<NC, true>(p, ltab, rep, day, lu, lupd, lrec,
-                                          lprop, llvl, nlv, i0, lpre);
-    for (int i = tid; i < nt; i += nth) {
-        const int node = p.e2_tn[t0 + i];
+    int myerr;
+    if (d.mode == E2_LDS_FAST) {
+        myerr = tid < 64 ? e2_levels_fast<NC>(p, rep, day, lu, lupd, lrec, llvl, d.nlv,
+                                              d.i0, lpre)
+                         : 0;
+        __syncthreads();
+    } else {
+        myerr = e2_levels<NC, true>(p, ltab, rep, day, lu, lupd, lrec, lprop, llvl,
+                                    d.nlv, d.i0, lpre);
+    }
+    for (int i = tid; i < d.nt; i += nth) {
+        const int node = ltn[i];
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
             uw[node * NC + c] = lu[i * NC + c];
@@ -500,12 +620,14 @@ __device__ __forceinline__ int e2_day(const KParams &p, const EvTab &tab, int re
 }
 
 /* Day loop with one lane per node and the node's state resident in VGPRs
- * across days.  GRID: one replicate spread over all workgroups of a
- * cooperative launch, joined by grid barriers; E2 runs in workgroup 0.
- * Otherwise: one workgroup per replicate, joined by workgroup barriers.
+ * across days.  GRID: one replicate spread over the workgroups of a
+ * cooperative launch, joined by grid barriers; one more workgroup, with
+ * no herds, runs E2 and stages each day's event data while the others
+ * run the day's SSA.  Otherwise: one workgroup per replicate, joined by
+ * workgroup barriers.
  * Per day:  A  SSA + E1 of the lane's node, its u published to global
- *              memory;                                   barrier
- *           B  the day's E2 events (e2_day);              barrier
+ *              memory (E2 workgroup: e2_stage);         barrier
+ *           B  the day's E2 events (e2_run);              barrier
  *           C  reload u, post time step, refresh, writeback.
  * (SimInf_solver_ssm.c:134-362, SimInf_solver_mssm.c:158-383.) */
 template <class M, bool CANON, bool GRID, int BLOCK>
@@ -535,8 +657,13 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 
     const EvTab tab = global_tables(p);
 
+    /* The workgroup that runs the E2 levels: in a grid launch the last,
+     * which holds no herds. */
+    const bool e2wg = !GRID || blockIdx.x == gridDim.x - 1;
+    E2Day e2d{0, 0, 0, 0, 0, 0, E2_NONE};
+
     /* E and N into LDS for the workgroup that runs the E2 levels. */
-    if (!GRID || blockIdx.x == 0) {
+    if (e2wg) {
         if (p.e2_cap_ev > 0) {
             uint32_t *lEmask = (uint32_t *)(lds + p.lds_Emask);
             double *lEw = (double *)(lds + p.lds_Ew);
@@ -632,13 +759,18 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 uw[node * NC + c] = s.u[c];
             });
         }
+        if (GRID && e2wg)
+            e2d = e2_stage<NC>(p, rep, day, lds);
         barrier();
         if (failed(day, PH_E1))
             break;
 
         /* B: external transfers in one workgroup. */
-        if (!GRID || blockIdx.x == 0)
-            myerr |= e2_day<NC>(p, tab, rep, day, uw, updw, lds);
+        if (e2wg) {
+            if (!GRID)
+                e2d = e2_stage<NC>(p, rep, day, lds);
+            myerr |= e2_run<NC>(p, tab, rep, day, uw, updw, lds, e2d);
+        }
         barrier();
         if (failed(day, PH_E2))
             break;
