@@ -72,9 +72,10 @@ def pmc_traffic(workload, dominant):
 
 def make_workload(name):
     from siminf_amd import workloads as W
-    if name == "sise3_herds":
+    if name.startswith("sise3_herds"):
+        solver = "aem" if name.endswith("aem") else "ssm"
         return W.sise3_herds(), {"workload": "SISe3 1600 herds x 1460 days, events",
-                                 "solver": "ssm"}
+                                 "solver": solver}
     if name == "mparse_sir":
         return W.mparse_sir_readme(), {"workload": "mparse SIR 1000 nodes x 150 days",
                                        "solver": "ssm"}

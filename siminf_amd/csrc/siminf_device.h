@@ -150,6 +150,9 @@ struct KParams {
     int *u_work;                     /* Nc x (Nn*Nrep) */
     int *upd_work;                   /* Nn*Nrep */
     int rep0;
+    /* The coupled kernel runs the all-events method instead of the SSA
+     * (one replicate, grid launch). */
+    int coupled_aem;
 
     /* Node kernels with a dense trajectory: a node whose state can no
      * longer change records the first column it has not written

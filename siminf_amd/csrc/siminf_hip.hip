@@ -668,12 +668,9 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
 
     /* Kernel choice. */
     if (a->solver == 1) {
-        if (n_e2 > 0) {
-            /* aem with external transfers is not implemented on the GPU. */
-            delete P;
-            return SIMINF_ERR_UNKNOWN_SOLVER;
-        }
-        P->kernel = K_AEM_NODES;
+        /* aem runs one replicate; with external transfers the nodes are
+         * coupled and it runs in the grid-wide coupled kernel. */
+        P->kernel = n_e2 > 0 ? K_COUPLED_GRID : K_AEM_NODES;
     } else if (n_e2 == 0) {
         P->kernel = K_SSM_NODES;
     } else {
@@ -713,6 +710,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     p.len = len;
     p.Nshift = Nshift;
     p.lanes = lanes;
+    p.coupled_aem = a->solver == 1 && P->kernel == K_COUPLED_GRID;
     p.tt0 = tt0;
     p.next0 = std::floor(tt0) + 1.0;
     p.ndays = ndays;

@@ -625,12 +625,13 @@ __device__ __forceinline__ int e2_run(const KParams &p, const EvTab &tab, int re
  * no herds, runs E2 and stages each day's event data while the others
  * run the day's SSA.  Otherwise: one workgroup per replicate, joined by
  * workgroup barriers.
- * Per day:  A  SSA + E1 of the lane's node, its u published to global
- *              memory (E2 workgroup: e2_stage);         barrier
+ * Per day:  A  SSA (or, with AEM, the all-events method) + E1 of the
+ *              lane's node, its u published to global memory (E2
+ *              workgroup: e2_stage);                        barrier
  *           B  the day's E2 events (e2_run);              barrier
  *           C  reload u, post time step, refresh, writeback.
  * (SimInf_solver_ssm.c:134-362, SimInf_solver_mssm.c:158-383.) */
-template <class M, bool CANON, bool GRID, int BLOCK>
+template <class M, bool CANON, bool GRID, int BLOCK, bool AEM = false>
 __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
@@ -650,10 +651,12 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     double ts = p.tspan[0];
     int myerr = 0;
 
-    NodeState<M> s;
+    std::conditional_t<AEM, AemState<M>, NodeState<M>> s;
     Ctx<M, CANON> cx;
     E1Cursor ec;
-    s.a0 = 0.0;
+    const uint64_t chan_base = (uint64_t)g * (uint64_t)NT;
+    if constexpr (!AEM)
+        s.a0 = 0.0;
 
     const EvTab tab = global_tables(p);
 
@@ -714,17 +717,36 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
             });
         }
         s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
-        s.nb = ~0ull;
-        s.a0 = 0.0;
         int err = 0;
-        static_for<NT>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
-            s.rate[j] = r;
-            s.a0 += r;
-            if (!rate_ok(r))
-                err = SIMINF_ERR_INVALID_RATE;
-        });
+        if constexpr (AEM) {
+            /* Initial rates and reaction times (aem.c:117-153). */
+            static_for<NT>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+                s.rate[j] = r;
+                if (!rate_ok(r))
+                    err = SIMINF_ERR_INVALID_RATE;
+                const siminf_stream c = siminf_stream_make(
+                    p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
+                double tj = -siminf_log(siminf_uniform_at(&c, 0)) / r + tt;
+                if (tj <= 0.0)
+                    tj = INFINITY;
+                s.time[j] = tj;
+                s.inf[j] = 0.0;
+                s.j[j] = 1;
+            });
+        } else {
+            s.nb = ~0ull;
+            s.a0 = 0.0;
+            static_for<NT>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+                s.rate[j] = r;
+                s.a0 += r;
+                if (!rate_ok(r))
+                    err = SIMINF_ERR_INVALID_RATE;
+            });
+        }
         updw[node] = 0;
         ec.start(p, node);
         if (err) {
@@ -737,8 +759,15 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         return;
 
     for (int day = 0;; day++) {
-        /* A: SSA (the whole wavefront takes part) and E1 events. */
-        int err = ssa_day_wave<M>(s, cx, tt, next, fired, active);
+        /* A: SSA (the whole wavefront takes part) or aem, and E1
+         * events. */
+        int err = 0;
+        if constexpr (AEM) {
+            if (active)
+                err = aem_day<M>(s, cx, next, chan_base, fired);
+        } else {
+            err = ssa_day_wave<M>(s, cx, tt, next, fired, active);
+        }
         if (active) {
             bool upd = false;
             int bad = 0;
@@ -788,7 +817,22 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 record_error(p, rep, day, PH_PTS, node, rc);
                 myerr = 1;
             } else if (rc > 0 || upd) {
-                const int err = refresh_rates<M>(s, cx, tt);
+                int err = 0;
+                if constexpr (AEM) {
+                    /* Through calcTimes at the node's time, which is the
+                     * next unit of time (aem.c:329-384). */
+                    static_for<NT>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        const double old = s.rate[j];
+                        const double r = M::template rate<j>(s.u, s.vnew, cx.ld, cx.gd, tt);
+                        s.rate[j] = r;
+                        if (!rate_ok(r))
+                            err = SIMINF_ERR_INVALID_RATE;
+                        calc_times<M, j>(s, cx.seed, chan_base, next, old, r);
+                    });
+                } else {
+                    err = refresh_rates<M>(s, cx, tt);
+                }
                 if (err) {
                     record_error(p, rep, day, PH_PTS, node, err);
                     myerr = 1;
@@ -940,6 +984,13 @@ static const void *coupled_fn()
                 : (const void *)k_coupled<M, CANON, false, SIMINF_REP_BLOCK>;
 }
 
+/* The all-events method runs one replicate only: grid launches. */
+template <class M, bool CANON>
+static const void *coupled_aem_fn()
+{
+    return (const void *)k_coupled<M, CANON, true, SIMINF_GRID_BLOCK, true>;
+}
+
 /* Launch the coupled kernel.  With grid, a cooperative launch of `grid`
  * workgroups that must all be resident; max_blocks (out, optional)
  * reports how many can be, for the caller's planning. */
@@ -948,10 +999,14 @@ static hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
                                  unsigned grid, unsigned block, size_t lds,
                                  hipStream_t st, int *max_blocks)
 {
-    const void *fn = grid_mode ? (canon ? coupled_fn<M, true, true>()
-                                        : coupled_fn<M, false, true>())
-                               : (canon ? coupled_fn<M, true, false>()
-                                        : coupled_fn<M, false, false>());
+    if (p.coupled_aem && !grid_mode)
+        return hipErrorInvalidValue;
+    const void *fn = p.coupled_aem ? (canon ? coupled_aem_fn<M, true>()
+                                            : coupled_aem_fn<M, false>())
+                     : grid_mode ? (canon ? coupled_fn<M, true, true>()
+                                          : coupled_fn<M, false, true>())
+                                 : (canon ? coupled_fn<M, true, false>()
+                                          : coupled_fn<M, false, false>());
     hipError_t e;
     if (lds > 0) {
         e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -66,7 +66,8 @@ def _sise3(nodes, days, events, replicates=1):
                     beta_t4=0.125, epsilon=0.0)
 
 
-def test_hub_herd_transfer_chains(oracle):
+@pytest.mark.parametrize("solver", ["ssm", "aem"])
+def test_hub_herd_transfer_chains(oracle, solver):
     """Herd 1 sends to and receives from the others 40 times a day: the
     day's external transfers form chains of about 40 levels."""
     rng = np.random.default_rng(4)
@@ -79,7 +80,7 @@ def test_hub_herd_transfer_chains(oracle):
             else:
                 rows.append((3, t, other, 1, 1, 0.0, 4, 0))
     m = _sise3(30, 25, _events(rows))
-    _check(m, oracle, seed=5)
+    _check(m, oracle, solver=solver, seed=5)
 
 
 def test_many_replicates_coupled(oracle):

@@ -108,6 +108,24 @@ def test_sise3_e2_events(oracle):
     _check(m, oracle, seed=6)
 
 
+@pytest.mark.parametrize("seed", [6, 11])
+def test_sise3_e2_events_aem(oracle, seed):
+    """aem with external transfers: the all-events method in the coupled
+    kernel (E1, E2, PTS refresh through calcTimes at the next day)."""
+    m = W.sise3_herds(Nn=300, days=200, seed=5)
+    ref, _ = _check(m, oracle, solver="aem", seed=seed)
+    assert ref.err == 0 and ref.transitions > 0
+    assert ref.aem_ties == 0
+
+
+def test_sise3_config2_full_aem(oracle):
+    """Config 2's workload under aem."""
+    m = W.sise3_herds()
+    ref, _ = _check(m, oracle, solver="aem", seed=5)
+    assert ref.transitions > 0
+    assert ref.aem_ties == 0
+
+
 def test_sise3_e2_events_replicates(oracle):
     m = W.sise3_herds(Nn=120, days=100, seed=8, replicates=6)
     _check(m, oracle, seed=7)
