@@ -295,8 +295,9 @@ def main():
 
     for i in range(args.warmup):
         step(100 + i)
-    # The node kernels' launch shape (lanes per wavefront) from the plan's
-    # autotuner, before the timed steps; coupled kernels have none (-1).
+    # The launch shape from the plan's autotuner, before the timed steps:
+    # lanes per wavefront for the node kernels, the number of herds on a
+    # wavefront of their own for a grid-wide coupled kernel.
     best = ctypes.c_int(-1)
     if not args.no_autotune:
         err = lib.SimInf_hip_plan_autotune(plan, 200, 2, ctypes.byref(best), None)
@@ -352,7 +353,9 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "kernel_ms_mean": statistics.mean(kms),
-        "lanes_per_wave_shift": best.value,
+        "autotuned": ({"solo_herds": best.value} if model.events is not None and
+                      (np.asarray(model.events["event"]) == 3).any()
+                      else {"lanes_per_wave_shift": best.value}),
         "kernel_ms_split": {"solver": solver_ms, "fill": fill_ms,
                             "fill_node_columns": fill_cols},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

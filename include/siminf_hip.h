@@ -227,12 +227,23 @@ int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *plan, double *solver_ms,
  * (no reference counterpart). */
 int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *plan, int shift);
 
-/* Autotuner of the node kernels' launch shape (no reference counterpart):
- * runs the plan reps times (seeds seed, seed+1, ...) for each lanes-per-
- * wavefront shift 0..6, keeps the one with the least mean solver time and
- * returns it (-1 for plans whose kernel has no such choice); ms, if not
- * NULL, gets the 7 mean times.  The last run's trajectory is the last
- * candidate's. */
+/* For a plan whose coupled kernel spans the grid (one replicate with
+ * external transfers): give each of the n herds `nodes` (zero-based,
+ * distinct, n <= 16) a wavefront of its own, whose other lanes only help
+ * it draw its SSA attempts; n = 0 packs all herds 64 to a wavefront.
+ * Results do not depend on it (no reference counterpart).  Returns
+ * "Invalid model." for other plans or bad nodes. */
+int SimInf_hip_plan_set_solo_herds(SimInf_hip_plan *plan, const int *nodes, int n);
+
+/* Autotuner of the launch shape (no reference counterpart): runs the plan
+ * reps times (seeds seed, seed+1, ...) for each candidate, keeps the one
+ * with the least mean solver time and returns it in *best_shift; ms, if
+ * not NULL, gets the 7 candidates' mean times.  Node kernels: the
+ * lanes-per-wavefront shifts 0..6.  Grid-wide coupled kernels: 0, 1, 2,
+ * 3, 4, 8 or 16 herds, those with the most transitions in a first run,
+ * each on a wavefront of its own (SimInf_hip_plan_set_solo_herds; an
+ * infeasible candidate's time is +Inf).  Other plans: -1, nothing run.
+ * The last run's trajectory is the last candidate's. */
 int SimInf_hip_plan_autotune(SimInf_hip_plan *plan, unsigned long int seed, int reps,
                              int *best_shift, double *ms);
 

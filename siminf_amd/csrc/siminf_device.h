@@ -153,6 +153,11 @@ struct KParams {
     /* The coupled kernel runs the all-events method instead of the SSA
      * (one replicate, grid launch). */
     int coupled_aem;
+    /* Grid launch: the node each lane runs (-1: none), or null for lane
+     * = node; and, if not null, each node's transition count of the last
+     * run (Nn). */
+    const int *lane_node;
+    unsigned long long *node_fired;
 
     /* Node kernels with a dense trajectory: a node whose state can no
      * longer change records the first column it has not written
@@ -396,32 +401,37 @@ __device__ __forceinline__ int fire_any(NodeState<M> &s, const CT &cx, int tr,
 
 /* One SSA firing at time t: the transition u_tr picks by a linear scan
  * of the cumulative rates (ssm.c:160-189) is applied.  Returns false when
- * the node stops: a0 fell to zero, or the step failed (err). */
+ * the node stops: a0 fell to zero, or the step failed (err).  few (the
+ * same on every lane of the wavefront) says that at most two lanes are
+ * firing: each then runs only its transition's code, where otherwise all
+ * lanes run the one code path of fire_any. */
 template <class M, class CT>
 __device__ __forceinline__ bool ssa_fire(NodeState<M> &s, const CT &cx, double t,
                                          double u_tr, unsigned long long &fired,
-                                         int &err)
+                                         int &err, bool few = false)
 {
     constexpr int NT = M::NT;
-    /* The rate of the selected transition is carried along (no run-time
-     * indexing). */
+    /* The scan's stopping point without branches: rates are non-negative,
+     * so the cumulative sums do not decrease and the transitions the
+     * scan passes are those with rnd above their sum (the additions are
+     * the scan's, in its order).  Past the end means the last one. */
     const double rnd = u_tr * s.a0;
     int tr = 0;
-    double rt = s.rate[0];
     double cum = s.rate[0];
-    bool searching = rnd > cum;
+    static_for<NT - 1>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        tr += rnd > cum ? 1 : 0;
+        if constexpr (k + 1 < NT)
+            cum += s.rate[k + 1];
+    });
+    double rt = s.rate[0];
     static_for<NT>([&](auto K) {
         constexpr int k = decltype(K)::value;
-        if (k > 0 && searching) {
-            tr = k;
-            rt = s.rate[k];
-            cum += s.rate[k];
-            searching = rnd > cum;
-        }
+        if constexpr (k > 0)
+            rt = tr == k ? s.rate[k] : rt;
     });
-    /* Floating point fix (ssm.c:170-189): past the end means the last
-     * transition, which the scan already holds; a zero rate walks back to
-     * the nearest non-zero one below it. */
+    /* Floating point fix (ssm.c:170-189): a zero rate walks back to the
+     * nearest non-zero one below it. */
     if (rt == 0.0) {
         static_for<NT>([&](auto K) {
             constexpr int k = NT - 1 - decltype(K)::value;
@@ -437,9 +447,7 @@ __device__ __forceinline__ bool ssa_fire(NodeState<M> &s, const CT &cx, double t
     }
 
     fired++;
-    if constexpr (CT::canon && NT <= 8) {
-        err = fire_any<M>(s, cx, tr, t);
-    } else {
+    auto each = [&]() {
         static_for<NT>([&](auto K) {
             constexpr int k = decltype(K)::value;
             if (tr == k) {
@@ -448,6 +456,14 @@ __device__ __forceinline__ bool ssa_fire(NodeState<M> &s, const CT &cx, double t
                     err = e;
             }
         });
+    };
+    if constexpr (CT::canon && NT <= 8) {
+        if (few)
+            each();
+        else
+            err = fire_any<M>(s, cx, tr, t);
+    } else {
+        each();
     }
     /* The node stops at its first failing step. */
     return err == 0;
@@ -573,6 +589,7 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
             have = (64 - rank + A - 1) / A;
             pos = 0;
         }
+        const bool few = __popcll(running) <= 2;
         /* Every lane takes part in the pulls; only running lanes use
          * them. */
         const int src = run ? pos * A + rank : lane;
@@ -586,7 +603,7 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
                 run = false;
             } else {
                 t += tau;
-                run = ssa_fire<M>(s, cx, t, u_tr, fired, err);
+                run = ssa_fire<M>(s, cx, t, u_tr, fired, err, few);
             }
         }
     }

@@ -280,6 +280,12 @@ struct DeviceBuffers {
 
 } // namespace
 
+/* Herds a grid-wide coupled plan can give a wavefront of their own. */
+#define SIMINF_SOLO_MAX 16
+
+struct SimInf_hip_plan;
+static int plan_set_solo(SimInf_hip_plan *P, const int *nodes, int n);
+
 struct SimInf_hip_plan {
     const SimInf_hip_model *model;
     Kernel kernel;
@@ -306,6 +312,10 @@ struct SimInf_hip_plan {
     /* Lanes per wavefront of the active pass as a power of two, or -1
      * for the rule of about two wavefronts per SIMD. */
     int lpw_fixed = -1;
+    /* Grid-wide coupled plans: herds given a wavefront of their own
+     * (the rest packed 64 to a workgroup), and the workgroups a grid
+     * launch may hold. */
+    int solo = 0, grid_max = 0;
 
     /* Last run. */
     bool has_run = false;       /* a run completed without error */
@@ -828,6 +838,11 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                 SimInf_hip_plan_destroy(P);
                 return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
             }
+            /* Room for SIMINF_SOLO_MAX herds on wavefronts of their own,
+             * within what one co-resident grid holds. */
+            P->grid_max = std::min<int>(max_blocks, (int)P->grid + SIMINF_SOLO_MAX);
+            p.lane_node = B.alloc<int>((size_t)P->grid_max * SIMINF_GRID_BLOCK);
+            p.node_fired = B.alloc<unsigned long long>((size_t)Nn);
         } else {
             P->block = (unsigned)(((Nn + 63) / 64) * 64);
             P->grid = (unsigned)Nrep;
@@ -848,6 +863,13 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     if (B.status != hipSuccess) {
         SimInf_hip_plan_destroy(P);
         return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+    }
+    if (P->kernel == K_COUPLED_GRID) {
+        const int err = plan_set_solo(P, nullptr, 0);
+        if (err) {
+            SimInf_hip_plan_destroy(P);
+            return err;
+        }
     }
     {
         int cus = 256;
@@ -1073,6 +1095,49 @@ extern "C" int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *P,
     return 0;
 }
 
+/* The lane -> node map of a grid-wide coupled plan: the n given herds
+ * each alone on a wavefront (its other lanes only help it draw its
+ * attempts), then the other herds in node order, 64 to a workgroup, then
+ * the workgroup that runs E2. */
+static int plan_set_solo(SimInf_hip_plan *P, const int *nodes, int n)
+{
+    if (P->kernel != K_COUPLED_GRID)
+        return n == 0 ? 0 : SIMINF_ERR_INVALID_MODEL;
+    const int Nn = P->Nn;
+    if (n < 0 || n > SIMINF_SOLO_MAX || n > Nn || (n > 0 && !nodes))
+        return SIMINF_ERR_INVALID_MODEL;
+    std::vector<char> solo((size_t)Nn, 0);
+    for (int i = 0; i < n; i++) {
+        if (nodes[i] < 0 || nodes[i] >= Nn || solo[nodes[i]])
+            return SIMINF_ERR_INVALID_MODEL;
+        solo[nodes[i]] = 1;
+    }
+    const int grid = n + (Nn - n + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1;
+    if (grid > P->grid_max)
+        return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+    std::vector<int> map((size_t)grid * SIMINF_GRID_BLOCK, -1);
+    for (int i = 0; i < n; i++)
+        map[(size_t)i * SIMINF_GRID_BLOCK] = nodes[i];
+    size_t L = (size_t)n * SIMINF_GRID_BLOCK;
+    for (int node = 0; node < Nn; node++)
+        if (!solo[node])
+            map[L++] = node;
+    if (hipMemcpyAsync((void *)P->p.lane_node, map.data(), map.size() * sizeof(int),
+                       hipMemcpyHostToDevice, P->stream) != hipSuccess ||
+        hipStreamSynchronize(P->stream) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    P->grid = (unsigned)grid;
+    P->solo = n;
+    return 0;
+}
+
+extern "C" int SimInf_hip_plan_set_solo_herds(SimInf_hip_plan *P, const int *nodes, int n)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    return plan_set_solo(P, nodes, n);
+}
+
 extern "C" int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *P, int shift)
 {
     if (!P || shift < -1 || shift > 6)
@@ -1086,8 +1151,55 @@ extern "C" int SimInf_hip_plan_autotune(SimInf_hip_plan *P, unsigned long int se
 {
     if (!P || reps < 1)
         return SIMINF_ERR_INVALID_MODEL;
+    if (P->kernel == K_COUPLED_GRID) {
+        /* The herds with the most transitions in a first run, 0, 1, 2, 3,
+         * 4, 8 or 16 of them, each on a wavefront of its own. */
+        int err = plan_set_solo(P, nullptr, 0);
+        if (!err)
+            err = SimInf_hip_plan_run(P, seed);
+        if (err)
+            return err;
+        std::vector<unsigned long long> nf((size_t)P->Nn);
+        if (hipMemcpyAsync(nf.data(), P->p.node_fired, nf.size() * sizeof(nf[0]),
+                           hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
+            hipStreamSynchronize(P->stream) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        std::vector<int> order((size_t)P->Nn);
+        for (int i = 0; i < P->Nn; i++)
+            order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return nf[a] > nf[b]; });
+        static const int cand[7] = {0, 1, 2, 3, 4, 8, 16};
+        int best = 0;
+        double best_ms = INFINITY;
+        for (int c = 0; c < 7; c++) {
+            if (ms)
+                ms[c] = INFINITY;
+            if (cand[c] > P->Nn || plan_set_solo(P, order.data(), cand[c]) != 0)
+                continue;
+            double sum = 0.0;
+            for (int r = 0; r < reps; r++) {
+                err = SimInf_hip_plan_run(P, seed + (unsigned long)r);
+                if (err)
+                    return err;
+                sum += P->solver_ms;
+            }
+            if (ms)
+                ms[c] = sum / reps;
+            if (sum / reps < best_ms) {
+                best_ms = sum / reps;
+                best = cand[c];
+            }
+        }
+        err = plan_set_solo(P, order.data(), best);
+        if (err)
+            return err;
+        if (best_shift)
+            *best_shift = best;
+        return 0;
+    }
     if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES) {
-        /* Nothing to tune: the coupled kernels have one lane per node. */
+        /* Nothing to tune: one workgroup per replicate. */
         if (best_shift)
             *best_shift = -1;
         return 0;

@@ -638,9 +638,10 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     extern __shared__ int4 lds4[];
     unsigned char *lds = (unsigned char *)lds4;
     const int rep = GRID ? p.rep0 : (int)blockIdx.x;
-    const int node = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
-                          : (int)threadIdx.x;
-    const bool active = node < p.Nn;
+    const int lane_g = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
+                            : (int)threadIdx.x;
+    const int node = GRID && p.lane_node ? p.lane_node[lane_g] : lane_g;
+    const bool active = node >= 0 && node < p.Nn;
     const long long base = (long long)rep * p.Nn;
     const long long g = base + node;
     int *uw = p.u_work + base * NC;
@@ -860,6 +861,8 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         if (U_it >= p.tlen)
             break;
     }
+    if (GRID && active && p.node_fired)
+        p.node_fired[node] = fired;
     add_transitions(p, fired);
 }
 

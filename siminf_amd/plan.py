@@ -66,9 +66,19 @@ class Plan:
         the default rule)."""
         self._check(self.lib.SimInf_hip_plan_set_lanes_per_wave(self._handle, int(shift)))
 
+    def set_solo_herds(self, nodes):
+        """Give each of these herds (zero-based, at most 16) a wavefront of
+        its own in a grid-wide coupled plan; [] packs all herds."""
+        nodes = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32).ravel())
+        self._check(self.lib.SimInf_hip_plan_set_solo_herds(
+            self._handle, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(nodes)))
+
     def autotune(self, seed=1, reps=2):
-        """Try every lanes-per-wavefront shift, keep the fastest; returns
-        (best shift, mean solver ms per shift)."""
+        """Try every candidate launch shape, keep the fastest; returns
+        (best, mean solver ms per candidate): the lanes-per-wavefront shift
+        0..6 for node kernels, the number of busiest herds (0, 1, 2, 3, 4,
+        8, 16) on wavefronts of their own for grid-wide coupled kernels,
+        -1 when there is nothing to tune."""
         best = ctypes.c_int()
         ms = (ctypes.c_double * 7)()
         self._check(self.lib.SimInf_hip_plan_autotune(self._handle, int(seed), int(reps),

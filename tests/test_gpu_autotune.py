@@ -27,9 +27,33 @@ def test_lanes_per_wave_do_not_change_results():
         np.testing.assert_array_equal(plan.download().U, ref)
 
 
-def test_autotune_coupled_plan_has_nothing_to_tune():
-    m = W.sise3_herds(Nn=100, days=20, seed=2)
-    print("case: SISe3 autotune", flush=True)
+def test_solo_herds_do_not_change_results(oracle):
+    """A grid-wide coupled plan (one replicate with external transfers):
+    herds moved to wavefronts of their own give the same trajectory, and
+    the autotuner's choice too."""
+    m = W.sise3_herds(Nn=300, days=60, seed=2)
+    print("case: SISe3 300 herds x 60 days, solo herds", flush=True)
+    ref = oracle.run(m, seed=4)
+    assert ref.err == 0
+    with si.Plan(m) as plan:
+        for nodes in ([], [7], [299, 0, 150], list(range(16))):
+            plan.set_solo_herds(nodes)
+            plan.run(4)
+            np.testing.assert_array_equal(plan.download().U, ref.U)
+        best, ms = plan.autotune(seed=4, reps=1)
+        assert best in (0, 1, 2, 3, 4, 8, 16)
+        assert ms[(0, 1, 2, 3, 4, 8, 16).index(best)] == min(ms)
+        plan.run(4)
+        np.testing.assert_array_equal(plan.download().U, ref.U)
+        with pytest.raises(si.SimInfError):
+            plan.set_solo_herds([3, 3])
+        with pytest.raises(si.SimInfError):
+            plan.set_solo_herds(list(range(17)))
+
+
+def test_autotune_replicated_coupled_plan_has_nothing_to_tune():
+    m = W.sise3_herds(Nn=100, days=20, seed=2, replicates=3)
+    print("case: SISe3 replicates autotune", flush=True)
     with si.Plan(m) as plan:
         best, _ = plan.autotune(reps=1)
         assert best == -1
