@@ -1178,15 +1178,25 @@ __device__ __forceinline__ int apply_e2_fast_pre(int n, int rows, int *uw, int *
     const bool low = 2 * n <= total;
     const int draws = low ? n : total - n;
     const double dt = (double)total;
-    int g[SIMINF_PRE_MAX];
-    SIMINF_UNROLL
-    for (int i = 0; i < SIMINF_PRE_MAX; i++)
-        g[i] = i < draws ? ua - (int)(v[i] * (dt - (double)i)) : -1;
-    /* k + (k < g) as med3(k, k + 1, g). */
+    /* k + (k < g) as med3(k, k + 1, g), four draws at a time; draws past
+     * every lane's count change nothing and are skipped. */
     int k = 0;
-    SIMINF_UNROLL
-    for (int i = 0; i < SIMINF_PRE_MAX; i++)
-        k = min(k + 1, max(k, g[i]));
+    auto four = [&](auto I0) {
+        constexpr int i0 = decltype(I0)::value;
+        int g[4];
+        static_for<4>([&](auto J) {
+            constexpr int i = i0 + decltype(J)::value;
+            g[i - i0] = i < draws ? ua - (int)(v[i] * (dt - (double)i)) : -1;
+        });
+        static_for<4>([&](auto J) { k = min(k + 1, max(k, g[decltype(J)::value])); });
+    };
+    static_assert(SIMINF_PRE_MAX == 12, "the urn runs in three groups of four draws");
+    four(std::integral_constant<int, 0>{});
+    if (__ballot(draws > 4) != 0) {
+        four(std::integral_constant<int, 4>{});
+        if (__ballot(draws > 8) != 0)
+            four(std::integral_constant<int, 8>{});
+    }
     if (n > total)
         return SIMINF_ERR_SAMPLE_SELECT;
     int xa;
