@@ -99,3 +99,25 @@ def test_solver_level_errors():
         si.run(m, solver="aem", seed=1)
     with pytest.raises(si.SimInfError, match="Invalid 'solver' value."):
         si.run(m, solver="direct", seed=1)
+
+
+def test_grid_coupled_replicates(oracle):
+    """More herds than one workgroup holds with replicates: one
+    cooperative launch per replicate (mssm), each with the E2 workgroup."""
+    from siminf_amd import workloads as W
+    m = W.sise3_herds(Nn=600, days=40, seed=12, replicates=2)
+    _check(m, oracle, seed=13)
+
+
+def test_grid_coupled_particle_filter(oracle):
+    """A particle filter over 600 herds with external transfers runs each
+    particle as a grid launch."""
+    from siminf_amd import workloads as W
+    from tests.test_gpu_pfilter import _check as pf_check
+    h = W.sise3_herds(Nn=600, days=30, seed=14)
+    rng = np.random.default_rng(15)
+    times = np.arange(5, 31, 5.0)
+    obs = rng.poisson(40.0, size=(len(times), 1))
+    err, _ = pf_check(oracle, h, 1, times, obs, 6, seed=16, p=0.5, dist="poisson",
+                      offset=1.0)
+    assert err == 0
