@@ -303,6 +303,13 @@ def main():
         err = lib.SimInf_hip_plan_autotune(plan, 200, 2, ctypes.byref(best), None)
         if err:
             raise SystemExit("autotune failed: %d" % err)
+    autotuned = {"lanes_per_wave_shift": best.value}
+    if lib.SimInf_hip_plan_get_herd_map(plan, None, None) == 0:
+        n_solo, pack = ctypes.c_int(), ctypes.c_int()
+        lib.SimInf_hip_plan_get_herd_map(plan, ctypes.byref(n_solo), ctypes.byref(pack))
+        if (best.value >= 0 and model.events is not None and
+                (np.asarray(model.events["event"]) == 3).any()):
+            autotuned = {"solo_herds": n_solo.value, "herds_per_wavefront": pack.value}
     del split[:]
     lat, kms, trans = [], [], 0
     node_days = 0
@@ -353,9 +360,7 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "kernel_ms_mean": statistics.mean(kms),
-        "autotuned": ({"solo_herds": best.value} if model.events is not None and
-                      (np.asarray(model.events["event"]) == 3).any()
-                      else {"lanes_per_wave_shift": best.value}),
+        "autotuned": autotuned,
         "kernel_ms_split": {"solver": solver_ms, "fill": fill_ms,
                             "fill_node_columns": fill_cols},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

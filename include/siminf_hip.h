@@ -228,22 +228,33 @@ int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *plan, double *solver_ms,
 int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *plan, int shift);
 
 /* For a plan whose coupled kernel spans the grid (one replicate with
- * external transfers): give each of the n herds `nodes` (zero-based,
- * distinct, n <= 16) a wavefront of its own, whose other lanes only help
- * it draw its SSA attempts; n = 0 packs all herds 64 to a wavefront.
- * Results do not depend on it (no reference counterpart).  Returns
- * "Invalid model." for other plans or bad nodes. */
-int SimInf_hip_plan_set_solo_herds(SimInf_hip_plan *plan, const int *nodes, int n);
+ * external transfers, or more herds than one workgroup holds): give each
+ * of the n herds `nodes` (zero-based, distinct, n <= 16) a wavefront of
+ * its own, and pack the other herds herds_per_wavefront (8..64) to a
+ * wavefront; a wavefront's lanes without a herd help its herds draw their
+ * SSA attempts.  (nodes, 0, 64) is the default map.  Results do not
+ * depend on it (no reference counterpart).  Returns "Invalid model." for
+ * other plans or bad arguments. */
+int SimInf_hip_plan_set_herd_map(SimInf_hip_plan *plan, const int *nodes, int n,
+                                 int herds_per_wavefront);
+
+/* The current map's solo herd count and herds per wavefront (0 and 64
+ * for plans without a map). */
+int SimInf_hip_plan_get_herd_map(const SimInf_hip_plan *plan, int *n_solo,
+                                 int *herds_per_wavefront);
 
 /* Autotuner of the launch shape (no reference counterpart): runs the plan
  * reps times (seeds seed, seed+1, ...) for each candidate, keeps the one
  * with the least mean solver time and returns it in *best_shift; ms, if
- * not NULL, gets the 7 candidates' mean times.  Node kernels: the
- * lanes-per-wavefront shifts 0..6.  Grid-wide coupled kernels: 0, 1, 2,
- * 3, 4, 8 or 16 herds, those with the most transitions in a first run,
- * each on a wavefront of its own (SimInf_hip_plan_set_solo_herds; an
- * infeasible candidate's time is +Inf).  Other plans: -1, nothing run.
- * The last run's trajectory is the last candidate's. */
+ * not NULL, gets the candidates' mean times (at most 8 entries).  Node
+ * kernels: the lanes-per-wavefront shifts 0..6 (7 entries).  Grid-wide
+ * coupled kernels (8 entries): herds ranked by their transitions in a
+ * first run; with the 4 busiest on wavefronts of their own, the others
+ * 64, 32, 16 or 8 to a wavefront; then, with the best of those, 0, 2, 8
+ * or 16 herds on their own (an infeasible candidate's time is +Inf);
+ * *best_shift gets the kept solo count (SimInf_hip_plan_get_herd_map
+ * gives both).  Other plans: -1, nothing run.  The last run's trajectory
+ * is the last candidate's. */
 int SimInf_hip_plan_autotune(SimInf_hip_plan *plan, unsigned long int seed, int reps,
                              int *best_shift, double *ms);
 

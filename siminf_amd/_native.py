@@ -87,7 +87,8 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_trajectory",
     "SimInf_hip_abc_weights",
     "SimInf_hip_plan_set_lanes_per_wave",
-    "SimInf_hip_plan_set_solo_herds",
+    "SimInf_hip_plan_set_herd_map",
+    "SimInf_hip_plan_get_herd_map",
     "SimInf_hip_plan_autotune",
     "SimInf_hip_plan_destroy",
 )
@@ -152,8 +153,12 @@ def load(path=None):
                                            ctypes.c_int, dp, dp, dp, dp, dp]
     lib.SimInf_hip_plan_set_lanes_per_wave.restype = ctypes.c_int
     lib.SimInf_hip_plan_set_lanes_per_wave.argtypes = [vp, ctypes.c_int]
-    lib.SimInf_hip_plan_set_solo_herds.restype = ctypes.c_int
-    lib.SimInf_hip_plan_set_solo_herds.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    lib.SimInf_hip_plan_set_herd_map.restype = ctypes.c_int
+    lib.SimInf_hip_plan_set_herd_map.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                                 ctypes.c_int]
+    lib.SimInf_hip_plan_get_herd_map.restype = ctypes.c_int
+    lib.SimInf_hip_plan_get_herd_map.argtypes = [vp, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]
     lib.SimInf_hip_plan_autotune.restype = ctypes.c_int
     lib.SimInf_hip_plan_autotune.argtypes = [vp, ctypes.c_ulong, ctypes.c_int, ip, dp]
     lib.SimInf_hip_plan_destroy.restype = None

@@ -118,6 +118,12 @@ struct KParams {
     const int4 *e2_rec;
     const double *e2_prop;
     const int *e2_tn_ptr, *e2_tn;
+    /* Per plan day: 1 when the day's E2 work fits the LDS buffers, every
+     * event takes the fast path and there are under 128 levels; such a
+     * day's second record words hold the byte offsets, in the staged herd
+     * state, of the source's and the destination's counts in the event's
+     * rows (src row a, src row b, dst row a, dst row b). */
+    const int *e2_day_fast;
     int ndays;
     int Nselect;
     /* A run over part of the plan's days (the intervals of a particle
@@ -1144,30 +1150,32 @@ __device__ __forceinline__ int apply_e2_fast(int n, int rows, int *uw, int *upd,
     return 0;
 }
 
-/* apply_e2_fast for an event whose stream's first SIMINF_PRE_MAX uniforms
- * are in pre.  Both herds' counts and the uniforms are loaded together
- * and the urn is computed before any decision (its result is used only
- * where siminf_hypergeometric would be called), as in that function's
- * computed-ahead branch: with g_i = ua - floor(u_i * (total - i)), draw i
- * takes a good item exactly when k < g_i, so after the independent g_i
- * the chain is one compare and one add per draw.  The event's stream is
- * used by nothing else, so its position is not kept; an urn of more
- * draws than were computed ahead takes the stream. */
-template <int NC>
-__device__ __forceinline__ int apply_e2_fast_pre(int n, int rows, int *uw, int *upd,
-                                                 int src, int dst, const double *pre,
-                                                 siminf_stream *rng)
+/* apply_e2_fast for an event of a day prepared by the host (offsets of the
+ * four counts in the staged state lu, in bytes) whose stream's first
+ * SIMINF_PRE_MAX uniforms are in pre.  The counts and the uniforms are
+ * loaded together and the urn is computed before any decision (its result
+ * is used only where siminf_hypergeometric would be called), as in that
+ * function's computed-ahead branch: with g_i = ua - floor(u_i * (total -
+ * i)), draw i takes a good item exactly when k < g_i, so after the
+ * independent g_i the chain is one compare and one add per draw.  The
+ * choice among the cases is made with selects, and a failing event writes
+ * back the counts it read, which leaves the state as skipping it would.
+ * The event's stream is used by nothing else, so its position is not
+ * kept; an urn of more draws than were computed ahead takes the stream.
+ * Returns the error code or 0. */
+__device__ __forceinline__ int apply_e2_fast_pre(int n, int rows, unsigned char *lu,
+                                                 int4 off, const double *pre,
+                                                 unsigned long long seed, uint64_t stream_id)
 {
-    const int ra = rows & 0xFF, rb = (rows >> 8) & 0xFF;
     const bool two = (rows >> 16) != 0;
-    int *sa = uw + src * NC + ra, *sb = uw + src * NC + rb;
-    int *da = uw + dst * NC + ra, *db = uw + dst * NC + rb;
+    int *sa = (int *)(lu + off.x), *sb = (int *)(lu + off.y);
+    int *da = (int *)(lu + off.z), *db = (int *)(lu + off.w);
     const int ua = *sa, ub0 = *sb, dva = *da, dvb = *db;
     double v[SIMINF_PRE_MAX];
     SIMINF_UNROLL
     for (int i = 0; i < SIMINF_PRE_MAX; i++)
         v[i] = pre[i];
-    /* Keep the loads here, ahead of the branches below (the compiler
+    /* Keep the loads here, ahead of the decisions below (the compiler
      * would otherwise sink the uniforms' loads into the urn's branch, a
      * second round trip after the counts'). */
     SIMINF_UNROLL
@@ -1197,38 +1205,26 @@ __device__ __forceinline__ int apply_e2_fast_pre(int n, int rows, int *uw, int *
         if (__ballot(draws > 8) != 0)
             four(std::integral_constant<int, 8>{});
     }
-    if (n > total)
-        return SIMINF_ERR_SAMPLE_SELECT;
-    int xa;
-    if (n == total) {
-        xa = ua;
-    } else if (!two || ub <= 0) {
-        xa = n;
-    } else if (ua <= 0) {
-        xa = 0;
-    } else if (draws <= SIMINF_PRE_MAX) {
-        xa = low ? k : ua - k;
-    } else {
-        rng->pre = pre;
-        rng->npre = SIMINF_PRE_MAX;
-        xa = siminf_hypergeometric(rng, ua, ub, n);
+    const bool urn = n < total && two && ub > 0 && ua > 0;
+    int xa = n == total ? ua : (!two || ub <= 0) ? n : ua <= 0 ? 0 : low ? k : ua - k;
+    if (urn && draws > SIMINF_PRE_MAX) {
+        siminf_stream rng = siminf_stream_make(seed, SIMINF_DOMAIN_E2, stream_id);
+        rng.pre = pre;
+        rng.npre = SIMINF_PRE_MAX;
+        xa = siminf_hypergeometric(&rng, ua, ub, n);
     }
+    const bool ok = n <= total;
     const int xb = n - xa;
-    const int va = dva + xa, wa = ua - xa;
+    const int va = ok ? dva + xa : dva, wa = ok ? ua - xa : ua;
+    const int vb = ok ? dvb + xb : dvb, wb = ok ? ub - xb : ub0;
     *da = va;
     *sa = wa;
-    bool neg = (va < 0) | (wa < 0);
     if (two) {
-        const int vb = dvb + xb, wb = ub - xb;
         *db = vb;
         *sb = wb;
-        neg = neg | (vb < 0) | (wb < 0);
     }
-    if (neg)
-        return SIMINF_ERR_NEGATIVE_STATE;
-    upd[dst] = 1;
-    upd[src] = 1;
-    return 0;
+    const bool neg = (va < 0) | (wa < 0) | (two & ((vb < 0) | (wb < 0)));
+    return !ok ? SIMINF_ERR_SAMPLE_SELECT : neg ? SIMINF_ERR_NEGATIVE_STATE : 0;
 }
 
 template <int NC>

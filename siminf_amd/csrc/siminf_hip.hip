@@ -282,9 +282,11 @@ struct DeviceBuffers {
 
 /* Herds a grid-wide coupled plan can give a wavefront of their own. */
 #define SIMINF_SOLO_MAX 16
+/* The fewest herds a grid-wide coupled plan packs on one wavefront. */
+#define SIMINF_PACK_MIN 8
 
 struct SimInf_hip_plan;
-static int plan_set_solo(SimInf_hip_plan *P, const int *nodes, int n);
+static int plan_set_herd_map(SimInf_hip_plan *P, const int *nodes, int n, int pack);
 
 struct SimInf_hip_plan {
     const SimInf_hip_model *model;
@@ -312,10 +314,10 @@ struct SimInf_hip_plan {
     /* Lanes per wavefront of the active pass as a power of two, or -1
      * for the rule of about two wavefronts per SIMD. */
     int lpw_fixed = -1;
-    /* Grid-wide coupled plans: herds given a wavefront of their own
-     * (the rest packed 64 to a workgroup), and the workgroups a grid
-     * launch may hold. */
-    int solo = 0, grid_max = 0;
+    /* Grid-wide coupled plans: herds given a wavefront of their own, the
+     * other herds per wavefront, and the workgroups a grid launch may
+     * hold. */
+    int solo = 0, pack = SIMINF_GRID_BLOCK, grid_max = 0;
 
     /* Last run. */
     bool has_run = false;       /* a run completed without error */
@@ -827,6 +829,40 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         P->lds = coupled_lds_layout(p, Nc, Nselect, Nshift, a->N != nullptr,
                                     max_touched, max_day_events, max_day_levels,
                                     budget);
+        /* Days whose E2 work fits the LDS buffers, with every event on the
+         * fast path and under 128 levels, run on one wavefront; their
+         * records' second words become the byte offsets of the four counts
+         * an event reads in the staged herd state. */
+        {
+            std::vector<int> day_fast(e2_day.size(), 0);
+            bool any = false;
+            for (size_t d = 0; d + 1 < e2_day.size(); d++) {
+                const int l0 = e2_day[d], nlv = e2_day[d + 1] - l0;
+                if (nlv == 0 || nlv >= 128)
+                    continue;
+                const int i0 = e2_lvl[l0], nev = e2_lvl[l0 + nlv] - i0;
+                const int nt = e2_tn_ptr[d + 1] - e2_tn_ptr[d];
+                if (!(nt <= p.e2_cap_t && nev <= p.e2_cap_ev && nlv <= p.e2_cap_lv))
+                    continue;
+                bool fast = true;
+                for (int i = i0; i < i0 + nev && fast; i++)
+                    fast = e2_rec[2 * i].w == SIMINF_E2_FAST;
+                if (!fast)
+                    continue;
+                for (int i = i0; i < i0 + nev; i++) {
+                    const int rows = e2_rec[2 * i].z, ra = rows & 0xFF, rb = (rows >> 8) & 0xFF;
+                    const int sl = e2_rec[2 * i + 1].x, dl = e2_rec[2 * i + 1].y;
+                    e2_rec[2 * i + 1] = int4{(sl * Nc + ra) * 4, (sl * Nc + rb) * 4,
+                                             (dl * Nc + ra) * 4, (dl * Nc + rb) * 4};
+                }
+                day_fast[d] = 1;
+                any = true;
+            }
+            if (any)
+                B.copy((void *)p.e2_rec, e2_rec.data(), e2_rec.size() * sizeof(int4),
+                       hipMemcpyHostToDevice);
+            p.e2_day_fast = B.upload(day_fast.data(), day_fast.size());
+        }
         if (P->kernel == K_COUPLED_GRID) {
             P->block = SIMINF_GRID_BLOCK;
             P->grid = (unsigned)((Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1);
@@ -838,9 +874,11 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                 SimInf_hip_plan_destroy(P);
                 return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
             }
-            /* Room for SIMINF_SOLO_MAX herds on wavefronts of their own,
-             * within what one co-resident grid holds. */
-            P->grid_max = std::min<int>(max_blocks, (int)P->grid + SIMINF_SOLO_MAX);
+            /* Room for SIMINF_SOLO_MAX herds on wavefronts of their own
+             * and the others packed as few as SIMINF_PACK_MIN to a
+             * wavefront, within what one co-resident grid holds. */
+            P->grid_max = std::min<int>(
+                max_blocks, (Nn + SIMINF_PACK_MIN - 1) / SIMINF_PACK_MIN + SIMINF_SOLO_MAX + 1);
             p.lane_node = B.alloc<int>((size_t)P->grid_max * SIMINF_GRID_BLOCK);
             p.node_fired = B.alloc<unsigned long long>((size_t)Nn);
         } else {
@@ -865,7 +903,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
     }
     if (P->kernel == K_COUPLED_GRID) {
-        const int err = plan_set_solo(P, nullptr, 0);
+        const int err = plan_set_herd_map(P, nullptr, 0, SIMINF_GRID_BLOCK);
         if (err) {
             SimInf_hip_plan_destroy(P);
             return err;
@@ -1097,14 +1135,15 @@ extern "C" int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *P,
 
 /* The lane -> node map of a grid-wide coupled plan: the n given herds
  * each alone on a wavefront (its other lanes only help it draw its
- * attempts), then the other herds in node order, 64 to a workgroup, then
- * the workgroup that runs E2. */
-static int plan_set_solo(SimInf_hip_plan *P, const int *nodes, int n)
+ * attempts), then the other herds in node order, pack to a wavefront (the
+ * wavefront's remaining lanes help), then the workgroup that runs E2. */
+static int plan_set_herd_map(SimInf_hip_plan *P, const int *nodes, int n, int pack)
 {
     if (P->kernel != K_COUPLED_GRID)
-        return n == 0 ? 0 : SIMINF_ERR_INVALID_MODEL;
+        return n == 0 && pack == SIMINF_GRID_BLOCK ? 0 : SIMINF_ERR_INVALID_MODEL;
     const int Nn = P->Nn;
-    if (n < 0 || n > SIMINF_SOLO_MAX || n > Nn || (n > 0 && !nodes))
+    if (n < 0 || n > SIMINF_SOLO_MAX || n > Nn || (n > 0 && !nodes) ||
+        pack < SIMINF_PACK_MIN || pack > SIMINF_GRID_BLOCK)
         return SIMINF_ERR_INVALID_MODEL;
     std::vector<char> solo((size_t)Nn, 0);
     for (int i = 0; i < n; i++) {
@@ -1112,30 +1151,47 @@ static int plan_set_solo(SimInf_hip_plan *P, const int *nodes, int n)
             return SIMINF_ERR_INVALID_MODEL;
         solo[nodes[i]] = 1;
     }
-    const int grid = n + (Nn - n + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1;
+    const int grid = n + (Nn - n + pack - 1) / pack + 1;
     if (grid > P->grid_max)
         return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
     std::vector<int> map((size_t)grid * SIMINF_GRID_BLOCK, -1);
     for (int i = 0; i < n; i++)
         map[(size_t)i * SIMINF_GRID_BLOCK] = nodes[i];
-    size_t L = (size_t)n * SIMINF_GRID_BLOCK;
-    for (int node = 0; node < Nn; node++)
-        if (!solo[node])
-            map[L++] = node;
+    size_t k = 0;
+    for (int node = 0; node < Nn; node++) {
+        if (solo[node])
+            continue;
+        map[(size_t)(n + k / pack) * SIMINF_GRID_BLOCK + k % pack] = node;
+        k++;
+    }
     if (hipMemcpyAsync((void *)P->p.lane_node, map.data(), map.size() * sizeof(int),
                        hipMemcpyHostToDevice, P->stream) != hipSuccess ||
         hipStreamSynchronize(P->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
     P->grid = (unsigned)grid;
     P->solo = n;
+    P->pack = pack;
     return 0;
 }
 
-extern "C" int SimInf_hip_plan_set_solo_herds(SimInf_hip_plan *P, const int *nodes, int n)
+extern "C" int SimInf_hip_plan_set_herd_map(SimInf_hip_plan *P, const int *nodes, int n,
+                                            int herds_per_wavefront)
 {
     if (!P)
         return SIMINF_ERR_INVALID_MODEL;
-    return plan_set_solo(P, nodes, n);
+    return plan_set_herd_map(P, nodes, n, herds_per_wavefront);
+}
+
+extern "C" int SimInf_hip_plan_get_herd_map(const SimInf_hip_plan *P, int *n_solo,
+                                            int *herds_per_wavefront)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (n_solo)
+        *n_solo = P->solo;
+    if (herds_per_wavefront)
+        *herds_per_wavefront = P->pack;
+    return 0;
 }
 
 extern "C" int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *P, int shift)
@@ -1152,9 +1208,11 @@ extern "C" int SimInf_hip_plan_autotune(SimInf_hip_plan *P, unsigned long int se
     if (!P || reps < 1)
         return SIMINF_ERR_INVALID_MODEL;
     if (P->kernel == K_COUPLED_GRID) {
-        /* The herds with the most transitions in a first run, 0, 1, 2, 3,
-         * 4, 8 or 16 of them, each on a wavefront of its own. */
-        int err = plan_set_solo(P, nullptr, 0);
+        /* Herds ranked by their transitions in a first run; then, with
+         * the 4 busiest on wavefronts of their own, the others 64, 32, 16
+         * or 8 to a wavefront; then, with the best of those, 0, 2, 8 or
+         * 16 herds on their own. */
+        int err = plan_set_herd_map(P, nullptr, 0, SIMINF_GRID_BLOCK);
         if (!err)
             err = SimInf_hip_plan_run(P, seed);
         if (err)
@@ -1169,33 +1227,41 @@ extern "C" int SimInf_hip_plan_autotune(SimInf_hip_plan *P, unsigned long int se
             order[i] = i;
         std::stable_sort(order.begin(), order.end(),
                          [&](int a, int b) { return nf[a] > nf[b]; });
-        static const int cand[7] = {0, 1, 2, 3, 4, 8, 16};
-        int best = 0;
+        int best_solo = 0, best_pack = SIMINF_GRID_BLOCK;
         double best_ms = INFINITY;
-        for (int c = 0; c < 7; c++) {
+        auto trial = [&](int c, int solo, int pack) -> int {
             if (ms)
                 ms[c] = INFINITY;
-            if (cand[c] > P->Nn || plan_set_solo(P, order.data(), cand[c]) != 0)
-                continue;
+            if (solo > P->Nn || plan_set_herd_map(P, order.data(), solo, pack) != 0)
+                return 0;
             double sum = 0.0;
             for (int r = 0; r < reps; r++) {
-                err = SimInf_hip_plan_run(P, seed + (unsigned long)r);
-                if (err)
-                    return err;
+                const int e = SimInf_hip_plan_run(P, seed + (unsigned long)r);
+                if (e)
+                    return e;
                 sum += P->solver_ms;
             }
             if (ms)
                 ms[c] = sum / reps;
             if (sum / reps < best_ms) {
                 best_ms = sum / reps;
-                best = cand[c];
+                best_solo = solo;
+                best_pack = pack;
             }
-        }
-        err = plan_set_solo(P, order.data(), best);
+            return 0;
+        };
+        static const int packs[4] = {64, 32, 16, 8}, solos[4] = {0, 2, 8, 16};
+        for (int c = 0; c < 4 && !err; c++)
+            err = trial(c, std::min(4, P->Nn), packs[c]);
+        const int pack = best_pack;
+        for (int c = 0; c < 4 && !err; c++)
+            err = trial(4 + c, solos[c], pack);
+        if (!err)
+            err = plan_set_herd_map(P, order.data(), best_solo, best_pack);
         if (err)
             return err;
         if (best_shift)
-            *best_shift = best;
+            *best_shift = best_solo;
         return 0;
     }
     if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES) {

@@ -418,19 +418,23 @@ __device__ __forceinline__ int e2_levels(const KParams &p, const EvTab &tab,
     return myerr;
 }
 
-/* The levels of a staged day whose events all take the fast path, run by
- * one wavefront (lane = event within a level; a level wider than the
- * wavefront takes more rounds).  A wavefront's LDS accesses complete in
- * issue order, so a level reads what the one before it wrote without a
- * barrier.  The level bounds sit in two registers (read with
- * v_readlane), and the next level's records are loaded while the current
- * level runs, so the chain from level to level is one round of LDS loads
- * (both herds' counts and the event's uniforms), the urn and the stores.
- * Needs nlv < 128.  Returns 1 if this lane recorded an error. */
+/* The levels of a staged day prepared for the fast path (every event a
+ * count, no shift, one E row or two equal-weight rows; the records' second
+ * words hold the four counts' byte offsets in lu), run by one wavefront
+ * (lane = event within a level; a level wider than the wavefront takes
+ * more rounds).  A wavefront's LDS accesses complete in issue order, so a
+ * level reads what the one before it wrote without a barrier.  The level
+ * bounds sit in two registers (read with v_readlane), and the next
+ * level's records are loaded while the current level runs, so the chain
+ * from level to level is one round of LDS loads (both herds' counts and
+ * the event's uniforms), the urn and the stores.  The touched herds are
+ * all marked updated by the caller.  Needs nlv < 128.  Returns 1 if this
+ * lane recorded an error. */
 template <int NC>
-__device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day, int *su,
-                                              int *sf, const int4 *rec, const int *lvl,
-                                              int nlv, int i0, const double *pre)
+__device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day,
+                                              unsigned char *lu, const int4 *rec,
+                                              const int *lvl, int nlv, int i0,
+                                              const double *pre)
 {
     const int lane = (int)threadIdx.x;
     int myerr = 0;
@@ -440,11 +444,10 @@ __device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day
         return l < 64 ? __builtin_amdgcn_readlane(bnd0, l)
                       : __builtin_amdgcn_readlane(bnd1, l - 64);
     };
-    auto process = [&](int i, int ev, int n, int rows, int src, int dst) {
-        siminf_stream rng = siminf_stream_make(
-            p.seed, SIMINF_DOMAIN_E2, (uint64_t)rep * (uint64_t)p.len + (uint64_t)ev);
-        const int err = apply_e2_fast_pre<NC>(n, rows, su, sf, src, dst,
-                                              pre + (long long)i * SIMINF_E2_PRE, &rng);
+    auto process = [&](int i, int ev, int n, int rows, int4 off) {
+        const int err = apply_e2_fast_pre(n, rows, lu, off,
+                                          pre + (long long)i * SIMINF_E2_PRE, p.seed,
+                                          (uint64_t)rep * (uint64_t)p.len + (uint64_t)ev);
         if (err) {
             record_error(p, rep, day, PH_E2, p.ev_rank[ev], err);
             myerr = 1;
@@ -461,8 +464,11 @@ __device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day
         /* This level's record, loaded during the level before, is taken
          * into registers before the next level's loads are issued: the
          * wait for it is then not a wait for those. */
-        const int ev = c0.x, n = c0.y, rows = c0.z, src = c1.x, dst = c1.y;
-        asm volatile("" ::"v"(ev), "v"(n), "v"(rows), "v"(src), "v"(dst) : "memory");
+        const int ev = c0.x, n = c0.y, rows = c0.z;
+        const int4 off = c1;
+        asm volatile("" ::"v"(ev), "v"(n), "v"(rows), "v"(off.x), "v"(off.y), "v"(off.z),
+                     "v"(off.w)
+                     : "memory");
         const bool more = l + 1 < nlv;
         const int an = more ? bound(l + 1) : 0, bn = more ? bound(l + 2) : 0;
         const bool hasn = lane < bn - an;
@@ -473,10 +479,10 @@ __device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day
             c1 = rec[2 * (an + lane) + 1];
         }
         if (has)
-            process(a + lane, ev, n, rows, src, dst);
+            process(a + lane, ev, n, rows, off);
         for (int i = a + lane + 64; i < b; i += 64) {
             const int4 r0 = rec[2 * i], r1 = rec[2 * i + 1];
-            process(i, r0.x, r0.y, r0.z, r1.x, r1.y);
+            process(i, r0.x, r0.y, r0.z, r1);
         }
         has = hasn;
         a = an;
@@ -531,12 +537,8 @@ __device__ __forceinline__ E2Day e2_stage(const KParams &p, int rep, int day,
     int *llvl = (int *)(lds + p.lds_lvl);
     int *ltn = (int *)(lds + p.lds_tn);
     double *lpre = (double *)(lds + p.lds_pre);
-    int slow = 0;
-    for (int i = tid; i < 2 * d.nev; i += nth) {
-        const int4 r = rec[i];
-        lrec[i] = r;
-        slow |= (i & 1) == 0 && r.w != SIMINF_E2_FAST;
-    }
+    for (int i = tid; i < 2 * d.nev; i += nth)
+        lrec[i] = rec[i];
     for (int i = tid; i < d.nev; i += nth)
         lprop[i] = prop[i];
     for (int i = tid; i <= d.nlv; i += nth)
@@ -553,7 +555,8 @@ __device__ __forceinline__ E2Day e2_stage(const KParams &p, int rep, int day,
         lpre[i * SIMINF_E2_PRE + 2 * b] = u0;
         lpre[i * SIMINF_E2_PRE + 2 * b + 1] = u1;
     }
-    d.mode = (__syncthreads_or(slow) == 0 && d.nlv < 128) ? E2_LDS_FAST : E2_LDS;
+    __syncthreads();
+    d.mode = p.e2_day_fast[pd] ? E2_LDS_FAST : E2_LDS;
     return d;
 }
 
@@ -599,8 +602,8 @@ __device__ __forceinline__ int e2_run(const KParams &p, const EvTab &tab, int re
     __syncthreads();
     int myerr;
     if (d.mode == E2_LDS_FAST) {
-        myerr = tid < 64 ? e2_levels_fast<NC>(p, rep, day, lu, lupd, lrec, llvl, d.nlv,
-                                              d.i0, lpre)
+        myerr = tid < 64 ? e2_levels_fast<NC>(p, rep, day, (unsigned char *)lu, lrec, llvl,
+                                              d.nlv, d.i0, lpre)
                          : 0;
         __syncthreads();
     } else {
@@ -613,7 +616,7 @@ __device__ __forceinline__ int e2_run(const KParams &p, const EvTab &tab, int re
             constexpr int c = decltype(C)::value;
             uw[node * NC + c] = lu[i * NC + c];
         });
-        if (lupd[i])
+        if (d.mode == E2_LDS_FAST || lupd[i])
             updw[node] = 1;
     }
     return myerr;

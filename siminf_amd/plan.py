@@ -66,24 +66,38 @@ class Plan:
         the default rule)."""
         self._check(self.lib.SimInf_hip_plan_set_lanes_per_wave(self._handle, int(shift)))
 
-    def set_solo_herds(self, nodes):
-        """Give each of these herds (zero-based, at most 16) a wavefront of
-        its own in a grid-wide coupled plan; [] packs all herds."""
-        nodes = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32).ravel())
-        self._check(self.lib.SimInf_hip_plan_set_solo_herds(
-            self._handle, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(nodes)))
+    def set_herd_map(self, solo=(), herds_per_wavefront=64):
+        """Grid-wide coupled plans: each herd in solo (zero-based, at most
+        16) on a wavefront of its own, the others herds_per_wavefront (8 to
+        64) to a wavefront."""
+        nodes = np.ascontiguousarray(np.asarray(solo, dtype=np.int32).ravel())
+        self._check(self.lib.SimInf_hip_plan_set_herd_map(
+            self._handle, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(nodes),
+            int(herds_per_wavefront)))
+
+    def herd_map(self):
+        """(herds on their own wavefront, herds per packed wavefront)."""
+        n, k = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.SimInf_hip_plan_get_herd_map(self._handle, ctypes.byref(n),
+                                                          ctypes.byref(k)))
+        return n.value, k.value
 
     def autotune(self, seed=1, reps=2):
         """Try every candidate launch shape, keep the fastest; returns
         (best, mean solver ms per candidate): the lanes-per-wavefront shift
-        0..6 for node kernels, the number of busiest herds (0, 1, 2, 3, 4,
-        8, 16) on wavefronts of their own for grid-wide coupled kernels,
-        -1 when there is nothing to tune."""
+        0..6 for node kernels (7 times); for grid-wide coupled kernels the
+        number of busiest herds kept on wavefronts of their own (8 times:
+        4 solo herds with the others 64, 32, 16, 8 to a wavefront, then 0,
+        2, 8, 16 solo herds with the best packing; herd_map() gives the
+        kept map); -1 when there is nothing to tune."""
         best = ctypes.c_int()
-        ms = (ctypes.c_double * 7)()
+        ms = (ctypes.c_double * 8)(*([float("nan")] * 8))
         self._check(self.lib.SimInf_hip_plan_autotune(self._handle, int(seed), int(reps),
                                                       ctypes.byref(best), ms))
-        return best.value, list(ms)
+        times = list(ms)
+        while times and times[-1] != times[-1]:     # entries the plan did not use
+            times.pop()
+        return best.value, times
 
     def download(self):
         """Copy the last run's U/V into the model (as run() leaves them)."""

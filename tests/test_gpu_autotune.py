@@ -27,28 +27,31 @@ def test_lanes_per_wave_do_not_change_results():
         np.testing.assert_array_equal(plan.download().U, ref)
 
 
-def test_solo_herds_do_not_change_results(oracle):
+def test_herd_maps_do_not_change_results(oracle):
     """A grid-wide coupled plan (one replicate with external transfers):
-    herds moved to wavefronts of their own give the same trajectory, and
-    the autotuner's choice too."""
+    herds moved to wavefronts of their own, or packed fewer to a
+    wavefront, give the same trajectory, and the autotuner's choice too."""
     m = W.sise3_herds(Nn=300, days=60, seed=2)
-    print("case: SISe3 300 herds x 60 days, solo herds", flush=True)
+    print("case: SISe3 300 herds x 60 days, herd maps", flush=True)
     ref = oracle.run(m, seed=4)
     assert ref.err == 0
     with si.Plan(m) as plan:
-        for nodes in ([], [7], [299, 0, 150], list(range(16))):
-            plan.set_solo_herds(nodes)
+        assert plan.herd_map() == (0, 64)
+        for solo, pack in (([], 64), ([7], 64), ([299, 0, 150], 16), (list(range(16)), 8),
+                           ([], 8), ([5, 6], 33)):
+            plan.set_herd_map(solo, pack)
+            assert plan.herd_map() == (len(solo), pack)
             plan.run(4)
             np.testing.assert_array_equal(plan.download().U, ref.U)
         best, ms = plan.autotune(seed=4, reps=1)
-        assert best in (0, 1, 2, 3, 4, 8, 16)
-        assert ms[(0, 1, 2, 3, 4, 8, 16).index(best)] == min(ms)
+        assert len(ms) == 8 and all(t > 0 for t in ms)
+        solo, pack = plan.herd_map()
+        assert best == solo and solo in (0, 2, 4, 8, 16) and pack in (64, 32, 16, 8)
         plan.run(4)
         np.testing.assert_array_equal(plan.download().U, ref.U)
-        with pytest.raises(si.SimInfError):
-            plan.set_solo_herds([3, 3])
-        with pytest.raises(si.SimInfError):
-            plan.set_solo_herds(list(range(17)))
+        for bad in (([3, 3], 64), (list(range(17)), 64), ([], 7), ([], 65), ([300], 64)):
+            with pytest.raises(si.SimInfError):
+                plan.set_herd_map(*bad)
 
 
 def test_autotune_replicated_coupled_plan_has_nothing_to_tune():
