@@ -385,7 +385,7 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
     const size_t per_t = (size_t)(Nc + 2) * sizeof(int);
     const size_t per_ev = 2 * sizeof(int4) + (1 + SIMINF_E2_PRE) * sizeof(double);
     auto need = [&](size_t t, size_t ev, size_t lv) {
-        return align16(t * Nc * sizeof(int)) + 2 * align16(t * sizeof(int)) +
+        return align16((t * Nc + 4) * sizeof(int)) + 2 * align16(t * sizeof(int)) +
                ev * 2 * sizeof(int4) + align16(ev * sizeof(double)) +
                align16((lv + 1) * sizeof(int)) +
                ev * SIMINF_E2_PRE * sizeof(double);
@@ -402,8 +402,10 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
     p.e2_cap_ev = (int)cap_ev;
     p.e2_cap_lv = (int)cap_lv;
     size_t off = tab;
+    /* The herds' counts, and four more ints that the one-wavefront level
+     * loop's lanes without an event read and write. */
     p.lds_u = (int)off;
-    off = align16(off + cap_t * Nc * sizeof(int));
+    off = align16(off + (cap_t * Nc + 4) * sizeof(int));
     p.lds_upd = (int)off;
     off = align16(off + cap_t * sizeof(int));
     p.lds_tn = (int)off;
@@ -612,13 +614,20 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
             e2_tn_ptr[d + 1] = (int)e2_tn.size();
             max_touched = std::max(max_touched, (int)e2_tn.size() - t0);
             const int i_start = (int)e2_idx.size();
+            /* A level wider than a wavefront is split into levels of at
+             * most 64 events (its events are independent). */
+            int day_levels = 0;
             for (auto &lv : levels) {
-                e2_idx.insert(e2_idx.end(), lv.begin(), lv.end());
-                e2_lvl.push_back((int)e2_idx.size());
+                for (size_t j = 0; j < lv.size(); j += SIMINF_GRID_BLOCK) {
+                    const size_t e = std::min(lv.size(), j + SIMINF_GRID_BLOCK);
+                    e2_idx.insert(e2_idx.end(), lv.begin() + j, lv.begin() + e);
+                    e2_lvl.push_back((int)e2_idx.size());
+                    day_levels++;
+                }
             }
             e2_day[d + 1] = (int)e2_lvl.size() - 1;
             max_day_events = std::max(max_day_events, (int)e2_idx.size() - i_start);
-            max_day_levels = std::max(max_day_levels, (int)levels.size());
+            max_day_levels = std::max(max_day_levels, day_levels);
         }
         if (e2_tn.empty())
             e2_tn.push_back(0);

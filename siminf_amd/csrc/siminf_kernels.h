@@ -420,16 +420,18 @@ __device__ __forceinline__ int e2_levels(const KParams &p, const EvTab &tab,
 
 /* The levels of a staged day prepared for the fast path (every event a
  * count, no shift, one E row or two equal-weight rows; the records' second
- * words hold the four counts' byte offsets in lu), run by one wavefront
- * (lane = event within a level; a level wider than the wavefront takes
- * more rounds).  A wavefront's LDS accesses complete in issue order, so a
- * level reads what the one before it wrote without a barrier.  The level
- * bounds sit in two registers (read with v_readlane), and the next
- * level's records are loaded while the current level runs, so the chain
- * from level to level is one round of LDS loads (both herds' counts and
- * the event's uniforms), the urn and the stores.  The touched herds are
- * all marked updated by the caller.  Needs nlv < 128.  Returns 1 if this
- * lane recorded an error. */
+ * words hold the four counts' byte offsets in lu; levels of at most 64
+ * events), run by one wavefront with lane = event within a level.  Every
+ * lane runs every level: a lane without an event moves nothing between
+ * four spare ints after the herds' counts, so each level is the same
+ * straight sequence of instructions.  A wavefront's LDS accesses complete
+ * in issue order, so a level reads what the one before it wrote without
+ * a barrier.  The level bounds sit in two registers (read with
+ * v_readlane), and the next level's records are loaded while the current
+ * level runs, so the chain from level to level is one round of LDS loads
+ * (both herds' counts and the event's uniforms), the urn and the stores.
+ * The touched herds are all marked updated by the caller.  Needs
+ * nlv < 128.  Returns 1 if this lane recorded an error. */
 template <int NC>
 __device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day,
                                               unsigned char *lu, const int4 *rec,
@@ -444,49 +446,45 @@ __device__ __forceinline__ int e2_levels_fast(const KParams &p, int rep, int day
         return l < 64 ? __builtin_amdgcn_readlane(bnd0, l)
                       : __builtin_amdgcn_readlane(bnd1, l - 64);
     };
-    auto process = [&](int i, int ev, int n, int rows, int4 off) {
+    const int spare = p.e2_cap_t * NC * (int)sizeof(int);
+    const int4 idle0 = int4{0, 0, 0, 0}, idle1 = int4{spare, spare + 4, spare + 8, spare + 12};
+    /* Lane's record of the level [a, b): loaded from a valid position in
+     * any case, replaced by the idle one when the level has no event for
+     * the lane. */
+    auto load = [&](int a, int b, int4 &r0, int4 &r1, int &i) {
+        const bool has = lane < b - a;
+        i = has ? a + lane : a;
+        r0 = rec[2 * i];
+        r1 = rec[2 * i + 1];
+        if (!has) {
+            r0 = idle0;
+            r1 = idle1;
+        }
+    };
+    int a = bound(0), b = bound(1), i = 0;
+    int4 c0, c1;
+    load(a, b, c0, c1, i);
+    for (int l = 0; l < nlv; l++) {
+        /* This level's record, loaded during the level before, is taken
+         * into registers before the next level's loads are issued: the
+         * wait for it is then not a wait for those. */
+        const int ev = c0.x, n = c0.y, rows = c0.z, ic = i;
+        const int4 off = c1;
+        asm volatile("" ::"v"(ev), "v"(n), "v"(rows), "v"(off.x), "v"(off.y), "v"(off.z),
+                     "v"(off.w), "v"(ic)
+                     : "memory");
+        if (l + 1 < nlv) {
+            a = bound(l + 1);
+            b = bound(l + 2);
+            load(a, b, c0, c1, i);
+        }
         const int err = apply_e2_fast_pre(n, rows, lu, off,
-                                          pre + (long long)i * SIMINF_E2_PRE, p.seed,
+                                          pre + (long long)ic * SIMINF_E2_PRE, p.seed,
                                           (uint64_t)rep * (uint64_t)p.len + (uint64_t)ev);
         if (err) {
             record_error(p, rep, day, PH_E2, p.ev_rank[ev], err);
             myerr = 1;
         }
-    };
-    int a = bound(0), b = bound(1);
-    bool has = lane < b - a;
-    int4 c0 = int4{0, 0, 0, 0}, c1 = int4{0, 0, 0, 0};
-    if (has) {
-        c0 = rec[2 * (a + lane)];
-        c1 = rec[2 * (a + lane) + 1];
-    }
-    for (int l = 0; l < nlv; l++) {
-        /* This level's record, loaded during the level before, is taken
-         * into registers before the next level's loads are issued: the
-         * wait for it is then not a wait for those. */
-        const int ev = c0.x, n = c0.y, rows = c0.z;
-        const int4 off = c1;
-        asm volatile("" ::"v"(ev), "v"(n), "v"(rows), "v"(off.x), "v"(off.y), "v"(off.z),
-                     "v"(off.w)
-                     : "memory");
-        const bool more = l + 1 < nlv;
-        const int an = more ? bound(l + 1) : 0, bn = more ? bound(l + 2) : 0;
-        const bool hasn = lane < bn - an;
-        c0 = int4{0, 0, 0, 0};
-        c1 = int4{0, 0, 0, 0};
-        if (hasn) {
-            c0 = rec[2 * (an + lane)];
-            c1 = rec[2 * (an + lane) + 1];
-        }
-        if (has)
-            process(a + lane, ev, n, rows, off);
-        for (int i = a + lane + 64; i < b; i += 64) {
-            const int4 r0 = rec[2 * i], r1 = rec[2 * i + 1];
-            process(i, r0.x, r0.y, r0.z, r1);
-        }
-        has = hasn;
-        a = an;
-        b = bn;
     }
     return myerr;
 }
@@ -599,6 +597,8 @@ __device__ __forceinline__ int e2_run(const KParams &p, const EvTab &tab, int re
         });
         lupd[i] = 0;
     }
+    if (tid < 4)
+        lu[p.e2_cap_t * NC + tid] = 0; /* the idle lanes' spare counts */
     __syncthreads();
     int myerr;
     if (d.mode == E2_LDS_FAST) {
