@@ -164,6 +164,9 @@ struct KParams {
      * run (Nn). */
     const int *lane_node;
     unsigned long long *node_fired;
+    /* Grid launch: the barrier words (arrivals, last day whose E2 work is
+     * done), cleared before each launch. */
+    unsigned *gsync;
 
     /* Node kernels with a dense trajectory: a node whose state can no
      * longer change records the first column it has not written

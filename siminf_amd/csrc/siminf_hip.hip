@@ -890,6 +890,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                 max_blocks, (Nn + SIMINF_PACK_MIN - 1) / SIMINF_PACK_MIN + SIMINF_SOLO_MAX + 1);
             p.lane_node = B.alloc<int>((size_t)P->grid_max * SIMINF_GRID_BLOCK);
             p.node_fired = B.alloc<unsigned long long>((size_t)Nn);
+            p.gsync = B.alloc<unsigned>(2);
         } else {
             P->block = (unsigned)(((Nn + 63) / 64) * 64);
             P->grid = (unsigned)Nrep;

@@ -687,13 +687,27 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         __syncthreads();
     }
 
+    /* Grid launch: every workgroup is one wavefront.  p.gsync[0] counts
+     * arrivals (each workgroup once at the start, then each herd
+     * workgroup once a day after its SSA and E1); p.gsync[1] is the last
+     * day whose E2 work is done.  A wavefront publishes its writes
+     * device-wide before it arrives or signals, and observes the others'
+     * after its wait. */
+    auto gs_arrive = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(p.gsync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto gs_wait = [&](const unsigned *at, unsigned target) {
+        if (threadIdx.x == 0)
+            while (__hip_atomic_load(at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+                __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    };
     auto barrier = [&]() {
         if constexpr (GRID) {
-            /* Publish this lane's writes device-wide before the grid
-             * barrier and observe the others' after it. */
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            cooperative_groups::this_grid().sync();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            gs_arrive();
+            gs_wait(p.gsync, gridDim.x);
         } else {
             __syncthreads();
         }
@@ -792,19 +806,34 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 uw[node * NC + c] = s.u[c];
             });
         }
-        if (GRID && e2wg)
-            e2d = e2_stage<NC>(p, rep, day, lds);
-        barrier();
-        if (failed(day, PH_E1))
-            break;
-
-        /* B: external transfers in one workgroup. */
-        if (e2wg) {
-            if (!GRID)
+        if constexpr (GRID) {
+            /* The herd workgroups arrive and wait for the day's E2 work;
+             * the E2 workgroup stages the day during their SSA, waits for
+             * all of them, runs the levels (whatever errors the day
+             * recorded: nothing depends on its results then) and signals.
+             * Every workgroup decides on errors after the signal, which
+             * orders every error of the day's SSA, E1 and E2 before it. */
+            if (e2wg) {
                 e2d = e2_stage<NC>(p, rep, day, lds);
+                gs_wait(p.gsync, gridDim.x + (unsigned)(day + 1) * (gridDim.x - 1));
+                myerr |= e2_run<NC>(p, tab, rep, day, uw, updw, lds, e2d);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(p.gsync + 1, (unsigned)(day + 1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                gs_arrive();
+                gs_wait(p.gsync + 1, (unsigned)(day + 1));
+            }
+        } else {
+            __syncthreads();
+            if (failed(day, PH_E1))
+                break;
+            /* B: external transfers in the workgroup. */
+            e2d = e2_stage<NC>(p, rep, day, lds);
             myerr |= e2_run<NC>(p, tab, rep, day, uw, updw, lds, e2d);
+            __syncthreads();
         }
-        barrier();
         if (failed(day, PH_E2))
             break;
 
@@ -977,6 +1006,14 @@ static void launch_aem_nodes(const KParams &p, bool events, bool canon,
         hipLaunchKernelGGL((k_aem_nodes<M, false, false>), dim3(grid), dim3(block), 0, st, p);
 }
 
+/* Clears a grid launch's barrier words (a kernel, not a memset, so that
+ * the launch sequence holds kernels only). */
+__global__ void k_clear_u32(unsigned *w, int n)
+{
+    if ((int)threadIdx.x < n)
+        w[threadIdx.x] = 0u;
+}
+
 /* Block sizes of the coupled kernel: 64 lanes per workgroup when one
  * replicate spans the grid, up to 512 lanes (one per node) when a
  * workgroup holds a replicate. */
@@ -1031,6 +1068,9 @@ static hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
         return hipSuccess;
     }
     if (grid_mode) {
+        if (!p.gsync)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, p.gsync, 2);
         KParams q = p;
         void *args[] = {&q};
         return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args,
