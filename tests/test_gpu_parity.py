@@ -64,7 +64,7 @@ def _check(model, oracle, solver="ssm", seed=1, crn=False):
 
 
 def test_mparse_sir_readme_full(oracle):
-    """Config 1 at full size: 1000 nodes x 150 days."""
+    """The README's mparse SIR at full size: 1000 nodes x 150 days."""
     m = W.mparse_sir_readme()
     ref, b = _check(m, oracle, seed=123)
     assert ref.transitions > 100000
@@ -118,8 +118,9 @@ def test_sise3_e2_events_aem(oracle, seed):
     assert ref.aem_ties == 0
 
 
-def test_sise3_config2_full_aem(oracle):
-    """Config 2's workload under aem."""
+def test_sise3_1600_herds_full_aem(oracle):
+    """SISe3, 1600 herds x 1460 days with ~1.2M scheduled events, under
+    aem."""
     m = W.sise3_herds()
     ref, _ = _check(m, oracle, solver="aem", seed=5)
     assert ref.transitions > 0
@@ -131,8 +132,8 @@ def test_sise3_e2_events_replicates(oracle):
     _check(m, oracle, seed=7)
 
 
-def test_sise3_config2_full(oracle):
-    """Config 2 at full size: 1600 herds, 1460 days, ~1.2M events."""
+def test_sise3_1600_herds_full(oracle):
+    """SISe3 at full size: 1600 herds, 1460 days, ~1.2M events."""
     m = W.sise3_herds()
     ref, _ = _check(m, oracle, seed=5)
     assert ref.transitions > 0

@@ -351,8 +351,8 @@ def _sir_final_size_pmf(S0, I0, beta, gamma):
     return final
 
 
-def test_final_size_distribution_config1(oracle):
-    """Config-1 nodes (99, 5, 0), beta 0.16, gamma 0.077, run long enough
+def test_final_size_distribution_readme_nodes(oracle):
+    """The README model's nodes (99, 5, 0), beta 0.16, gamma 0.077, run long enough
     for every outbreak to end: the law of the final S against the exact
     final-size distribution of the chain."""
     Nn = 6000
