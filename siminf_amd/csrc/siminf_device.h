@@ -559,7 +559,7 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int x)
  * block and the logarithm, which are most of an attempt's instructions,
  * leave the busiest lane's chain.  Stream positions and every value are
  * those of ssa_day. */
-template <class M, class CT>
+template <class M, bool DENSE = false, class CT>
 __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
                                             double tt, double next,
                                             unsigned long long &fired,
@@ -577,10 +577,27 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
         const uint64_t running = __ballot(run);
         if (running == 0)
             break;
-        if (__ballot(run && pos >= have) != 0) {
+        bool own = false;
+        if (DENSE && running == ~0ull) {
+            /* Every lane running: each computes its own next attempt
+             * (compiled in for kernels whose wavefronts are often full). */
+            double u0, u1;
+            siminf_uniform_pair(&s.rng, b, &u0, &u1);
+            hE = -siminf_log_u01(u0);
+            hU = u1;
+            A = 64;
+            rank = lane;
+            have = 1;
+            pos = 0;
+            own = true;
+        } else if (__ballot(run && pos >= have) != 0) {
             A = __popcll(running);
-            const int c = nth_set_bit(running, lane % A);
-            const int k = lane / A;
+            /* l / A and l % A for l < 64 by a float reciprocal: (l + 0.5) / A
+             * is at least 0.5 / A away from an integer, far beyond the
+             * rounding of the product. */
+            const float rA = __builtin_amdgcn_rcpf((float)A);
+            const int k = (int)(((float)lane + 0.5f) * rA);
+            const int c = nth_set_bit(running, lane - k * A);
             siminf_stream st;
             st.key0 = lane_pull(c, s.rng.key0);
             st.key1 = lane_pull(c, s.rng.key1);
@@ -595,15 +612,16 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
             rank = (int)__builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(running >> 32),
                 __builtin_amdgcn_mbcnt_lo((uint32_t)running, 0u));
-            have = (64 - rank + A - 1) / A;
+            /* ceil((64 - rank) / A) = floor((63 - rank) / A) + 1 */
+            have = (int)(((float)(63 - rank) + 0.5f) * rA) + 1;
             pos = 0;
         }
         const bool few = __popcll(running) <= 2;
         /* Every lane takes part in the pulls; only running lanes use
          * them. */
         const int src = run ? pos * A + rank : lane;
-        const double E = lane_pull(src, hE);
-        const double u_tr = lane_pull(src, hU);
+        const double E = DENSE && own ? hE : lane_pull(src, hE);
+        const double u_tr = DENSE && own ? hU : lane_pull(src, hU);
         if (run) {
             b++;
             pos++;
@@ -612,7 +630,9 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
                 run = false;
             } else {
                 t += tau;
-                run = ssa_fire<M>(s, cx, t, u_tr, fired, err, few);
+                /* As in ssa_day, the node stops drawing once its total rate
+                 * is no longer positive. */
+                run = ssa_fire<M>(s, cx, t, u_tr, fired, err, few) && s.a0 > 0.0;
             }
         }
     }

@@ -88,18 +88,31 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
     const long long g = node_lane(p);
+    const bool live = g >= 0;
     unsigned long long fired = 0, frozen_cols = 0;
     bool push = false;
 
-    if (g >= 0) {
-        const int rep = (int)(g / p.Nn);
-        const int node = (int)(g - (long long)rep * p.Nn);
-        Ctx<M, CANON> cx;
-        cx.load(p, node);
-        NodeState<M> s;
-        int err = 0;
-        double tt = p.tt0, next = p.next0;
+    const int rep = live ? (int)(g / p.Nn) : 0;
+    const int node = live ? (int)(g - (long long)rep * p.Nn) : 0;
+    Ctx<M, CANON> cx;
+    NodeState<M> s;
+    s.a0 = 0.0;
+    int err = 0;
+    double tt = p.tt0, next = p.next0;
+    const EvTab tab = global_tables(p);
+    E1Cursor ec;
+    /* Can the node's state no longer change? */
+    auto frozen = [&]() -> bool {
+        if constexpr (pts_noop<M>::value)
+            return s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end);
+        else
+            return false;
+    };
+    /* The lane runs days (not the classify pass, no error at start). */
+    bool going = false;
 
+    if (live) {
+        cx.load(p, node);
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
             s.u[c] = p.u0[g * NC + c];
@@ -115,7 +128,6 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
         s.nb = ~0ull;
 
         /* Initial rates (ssm.c:96-117). */
-        s.a0 = 0.0;
         static_for<NT>([&](auto J) {
             constexpr int j = decltype(J)::value;
             const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
@@ -124,18 +136,8 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
             if (!rate_ok(r))
                 err = SIMINF_ERR_INVALID_RATE;
         });
-
-        const EvTab tab = global_tables(p);
-        E1Cursor ec;
         if constexpr (EVENTS)
             ec.start(p, node);
-        /* Can the node's state no longer change? */
-        auto frozen = [&]() -> bool {
-            if constexpr (pts_noop<M>::value)
-                return s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end);
-            else
-                return false;
-        };
 
         if (err) {
             record_error(p, rep, 0, PH_INIT, node, err);
@@ -147,63 +149,78 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
                 push = true;
             }
         } else {
-            int U_it = 0, day = 0;
-            double ts = p.tspan[0];
-            for (;;) {
-                bool upd = false;
-                int bad = 0;
-
-                if (p.mode == NODES_ACTIVE && frozen()) {
-                    store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
-                    break;
-                }
-
-                err = ssa_day<M>(s, cx, tt, next, fired);
-                if (err) {
-                    record_error(p, rep, day, PH_SSA, node, err);
-                    break;
-                }
-                if constexpr (EVENTS) {
-                    err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
-                    if (err) {
-                        record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
-                        break;
-                    }
-                }
-
-                /* Post time step and refresh (ssm.c:262-302). */
-                const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
-                if (rc < 0) {
-                    record_error(p, rep, day, PH_PTS, node, rc);
-                    break;
-                }
-                if (rc > 0 || upd) {
-                    err = refresh_rates<M>(s, cx, tt);
-                    if (err) {
-                        record_error(p, rep, day, PH_PTS, node, err);
-                        break;
-                    }
-                }
-
-                tt = next;
-                next += 1.0;
-                store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
-                if constexpr (ND > 0) {
-                    static_for<ND>([&](auto C) {
-                        constexpr int c = decltype(C)::value;
-                        const double x = s.v[c];
-                        s.v[c] = s.vnew[c];
-                        s.vnew[c] = x;
-                    });
-                }
-                if (U_it >= p.tlen)
-                    break;
-                day++;
-            }
+            going = true;
         }
-        if (p.mode == NODES_CLASSIFY && frozen_cols == 0)
-            p.frozen_from[g] = p.tlen;
     }
+
+    /* Day loop.  The wavefront runs it until its last lane is done: the
+     * SSA takes every lane (ssa_day_wave), and lanes without a node or
+     * past their last day only help the others draw their attempts. */
+    if (p.mode != NODES_CLASSIFY) {
+        int U_it = 0, day = 0;
+        double ts = p.tspan[0];
+        for (;;) {
+            if (__ballot(going) == 0)
+                break;
+            bool upd = false;
+            int bad = 0;
+
+            if (going && p.mode == NODES_ACTIVE && frozen()) {
+                store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                going = false;
+            }
+
+            err = ssa_day_wave<M, true>(s, cx, tt, next, fired, going);
+            if (!going)
+                continue;
+            if (err) {
+                record_error(p, rep, day, PH_SSA, node, err);
+                going = false;
+                continue;
+            }
+            if constexpr (EVENTS) {
+                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
+                if (err) {
+                    record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
+                    going = false;
+                    continue;
+                }
+            }
+
+            /* Post time step and refresh (ssm.c:262-302). */
+            const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+            if (rc < 0) {
+                record_error(p, rep, day, PH_PTS, node, rc);
+                going = false;
+                continue;
+            }
+            if (rc > 0 || upd) {
+                err = refresh_rates<M>(s, cx, tt);
+                if (err) {
+                    record_error(p, rep, day, PH_PTS, node, err);
+                    going = false;
+                    continue;
+                }
+            }
+
+            tt = next;
+            next += 1.0;
+            store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
+            if constexpr (ND > 0) {
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    const double x = s.v[c];
+                    s.v[c] = s.vnew[c];
+                    s.vnew[c] = x;
+                });
+            }
+            if (U_it >= p.tlen)
+                going = false;
+            day++;
+        }
+    }
+    if (live && p.mode == NODES_CLASSIFY && frozen_cols == 0)
+        p.frozen_from[g] = p.tlen;
     if (p.mode == NODES_CLASSIFY) {
         push_active(p, g, push);
         add_count(p.frozen_cols, frozen_cols);

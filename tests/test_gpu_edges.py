@@ -121,3 +121,24 @@ def test_grid_coupled_particle_filter(oracle):
     err, _ = pf_check(oracle, h, 1, times, obs, 6, seed=16, p=0.5, dist="poisson",
                       offset=1.0)
     assert err == 0
+
+
+@pytest.mark.parametrize("coupled", [False, True])
+@pytest.mark.parametrize("reps", [1, 4])
+def test_rates_falling_to_zero_with_enter_events(oracle, coupled, reps):
+    """Nodes whose last infected recover (total rate to zero, or a rounding
+    residue at or below it) stop drawing that day, as the reference's loop
+    does; later enter events then sample from the same stream positions.
+    Run through the node kernel and, with one external transfer, the
+    coupled kernel."""
+    rng = np.random.default_rng(3)
+    r0 = np.random.default_rng(5)
+    S, I = r0.integers(40, 200, 20), r0.integers(1, 8, 20)
+    rows = [(1, int(t), int(nd), 0, int(rng.integers(1, 4)), 0.0, 4, 0)
+            for t, nd in zip(rng.integers(1, 40, 60), rng.integers(1, 21, 60))]
+    if coupled:
+        rows.append((3, 39, 1, 2, 1, 0.0, 1, 0))
+    m = si.SIR_mparse(u0={"S": S, "I": I, "R": np.zeros(20, int)},
+                      tspan=np.arange(1, 41.0), beta=0.25, gamma=0.1, events=_events(rows),
+                      E=si.models.select_matrix_SIR(), replicates=reps)
+    _check(m, oracle, seed=21)
