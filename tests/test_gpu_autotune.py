@@ -54,6 +54,19 @@ def test_herd_maps_do_not_change_results(oracle):
                 plan.set_herd_map(*bad)
 
 
+def test_herd_maps_aem(oracle):
+    """The all-events method in the coupled kernel under other maps."""
+    m = W.sise3_herds(Nn=300, days=60, seed=2)
+    print("case: SISe3 300 herds x 60 days, aem, herd maps", flush=True)
+    ref = oracle.run(m, solver="aem", seed=6)
+    assert ref.err == 0 and ref.aem_ties == 0
+    with si.Plan(m, solver="aem") as plan:
+        for solo, pack in (([], 64), ([1, 2, 3, 250], 8), (list(range(16)), 32)):
+            plan.set_herd_map(solo, pack)
+            plan.run(6)
+            np.testing.assert_array_equal(plan.download().U, ref.U)
+
+
 def test_autotune_replicated_coupled_plan_has_nothing_to_tune():
     m = W.sise3_herds(Nn=100, days=20, seed=2, replicates=3)
     print("case: SISe3 replicates autotune", flush=True)
