@@ -1210,25 +1210,34 @@ __device__ __forceinline__ int apply_e2_fast_pre(int n, int rows, unsigned char 
     const int draws = low ? n : total - n;
     const double dt = (double)total;
     /* k + (k < g) as med3(k, k + 1, g), four draws at a time; draws past
-     * every lane's count change nothing and are skipped. */
-    int k = 0;
+     * every lane's count change nothing and are skipped.  The chain runs
+     * in single precision (v_med3_f32): k, k + 1 and g are integers below
+     * 2^24, exact there, so the result is the integer chain's. */
+    float kf = 0.0f;
     auto four = [&](auto I0) {
         constexpr int i0 = decltype(I0)::value;
-        int g[4];
+        float g[4];
         static_for<4>([&](auto J) {
             constexpr int i = i0 + decltype(J)::value;
-            g[i - i0] = i < draws ? ua - (int)(v[i] * (dt - (double)i)) : -1;
+            g[i - i0] = i < draws ? (float)(ua - (int)(v[i] * (dt - (double)i))) : -1.0f;
         });
-        static_for<4>([&](auto J) { k = min(k + 1, max(k, g[decltype(J)::value])); });
+        static_for<4>([&](auto J) {
+            kf = __builtin_amdgcn_fmed3f(kf, kf + 1.0f, g[decltype(J)::value]);
+        });
     };
     static_assert(SIMINF_PRE_MAX == 12, "the urn runs in three groups of four draws");
-    four(std::integral_constant<int, 0>{});
-    if (__ballot(draws > 4) != 0) {
-        four(std::integral_constant<int, 4>{});
-        if (__ballot(draws > 8) != 0)
-            four(std::integral_constant<int, 8>{});
-    }
+    /* Only an event that takes from both rows draws from the urn; the
+     * groups run when some lane of the wavefront needs them. */
     const bool urn = n < total && two && ub > 0 && ua > 0;
+    if (__ballot(urn) != 0) {
+        four(std::integral_constant<int, 0>{});
+        if (__ballot(urn && draws > 4) != 0) {
+            four(std::integral_constant<int, 4>{});
+            if (__ballot(urn && draws > 8) != 0)
+                four(std::integral_constant<int, 8>{});
+        }
+    }
+    const int k = (int)kf;
     int xa = n == total ? ua : (!two || ub <= 0) ? n : ua <= 0 ? 0 : low ? k : ua - k;
     if (urn && draws > SIMINF_PRE_MAX) {
         siminf_stream rng = siminf_stream_make(seed, SIMINF_DOMAIN_E2, stream_id);
