@@ -1180,7 +1180,7 @@ __device__ __forceinline__ int apply_e2_fast(int n, int rows, int *uw, int *upd,
  * is used only where siminf_hypergeometric would be called), as in that
  * function's computed-ahead branch: with g_i = ua - floor(u_i * (total -
  * i)), draw i takes a good item exactly when k < g_i, so after the
- * independent g_i the chain is one compare and one add per draw.  The
+ * independent g_i the chain is one add and one med3 per draw.  The
  * choice among the cases is made with selects, and a failing event writes
  * back the counts it read, which leaves the state as skipping it would.
  * The event's stream is used by nothing else, so its position is not

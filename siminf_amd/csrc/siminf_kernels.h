@@ -14,7 +14,6 @@
 
 #include "siminf_device.h"
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 namespace siminf {
