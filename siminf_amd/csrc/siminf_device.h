@@ -124,6 +124,10 @@ struct KParams {
      * state, of the source's and the destination's counts in the event's
      * rows (src row a, src row b, dst row a, dst row b). */
     const int *e2_day_fast;
+    /* Per node, the plan days on which external transfers touch it, in
+     * order: e2_hday[e2_hptr[i] .. e2_hptr[i+1]).  The coupled kernel's
+     * herds publish and reload their state only on those days. */
+    const int *e2_hptr, *e2_hday;
     int ndays;
     int Nselect;
     /* A run over part of the plan's days (the intervals of a particle

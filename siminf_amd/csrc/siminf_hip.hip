@@ -789,6 +789,22 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     p.e2_prop = B.upload(e2_prop.data(), e2_prop.size());
     p.e2_tn_ptr = B.upload(e2_tn_ptr.data(), e2_tn_ptr.size());
     p.e2_tn = B.upload(e2_tn.data(), e2_tn.size());
+    {
+        /* Each node's transfer days (the touched lists, by node). */
+        std::vector<int> hptr((size_t)Nn + 1, 0), hday;
+        for (size_t d = 0; d + 1 < e2_tn_ptr.size(); d++)
+            for (int i = e2_tn_ptr[d]; i < e2_tn_ptr[d + 1]; i++)
+                hptr[(size_t)e2_tn[i] + 1]++;
+        for (int i = 0; i < Nn; i++)
+            hptr[(size_t)i + 1] += hptr[i];
+        hday.resize(std::max<size_t>((size_t)hptr[Nn], 1));
+        std::vector<int> fill(hptr.begin(), hptr.end() - 1);
+        for (size_t d = 0; d + 1 < e2_tn_ptr.size(); d++)
+            for (int i = e2_tn_ptr[d]; i < e2_tn_ptr[d + 1]; i++)
+                hday[(size_t)fill[e2_tn[i]]++] = (int)d;
+        p.e2_hptr = B.upload(hptr.data(), hptr.size());
+        p.e2_hday = B.upload(hday.data(), hday.size());
+    }
     p.Nselect = Nselect;
 
     const size_t ncols = (size_t)Nrep * a->tlen;

@@ -674,6 +674,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     std::conditional_t<AEM, AemState<M>, NodeState<M>> s;
     Ctx<M, CANON> cx;
     E1Cursor ec;
+    int hcur = 0, hend = 0, hnext = INT_MAX;
     const uint64_t chan_base = (uint64_t)g * (uint64_t)NT;
     if constexpr (!AEM)
         s.a0 = 0.0;
@@ -783,6 +784,12 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         }
         updw[node] = 0;
         ec.start(p, node);
+        /* The first transfer day at or after the run's first plan day. */
+        hcur = p.e2_hptr[node];
+        hend = p.e2_hptr[node + 1];
+        while (hcur < hend && p.e2_hday[hcur] < p.day0)
+            hcur++;
+        hnext = hcur < hend ? p.e2_hday[hcur] : INT_MAX;
         if (err) {
             record_error(p, rep, 0, PH_INIT, node, err);
             myerr = 1;
@@ -802,25 +809,29 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         } else {
             err = ssa_day_wave<M>(s, cx, tt, next, fired, active);
         }
+        /* The plan day, and whether external transfers touch this herd
+         * today: only then is its state published and reloaded. */
+        const int pd = day + p.day0;
+        const bool touched = active && hnext == pd;
+        bool upd1 = false;
         if (active) {
-            bool upd = false;
             int bad = 0;
             if (err) {
                 record_error(p, rep, day, PH_SSA, node, err);
                 myerr = 1;
             } else {
-                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
+                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd1, bad);
                 if (err) {
                     record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
                     myerr = 1;
                 }
             }
-            if (upd)
-                updw[node] = 1;
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                uw[node * NC + c] = s.u[c];
-            });
+            if (touched) {
+                static_for<NC>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    uw[node * NC + c] = s.u[c];
+                });
+            }
         }
         if constexpr (GRID) {
             /* The herd workgroups arrive and wait for the day's E2 work;
@@ -855,12 +866,17 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 
         /* C: post time step, refresh, writeback. */
         if (active) {
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.u[c] = uw[node * NC + c];
-            });
-            const bool upd = updw[node] != 0;
-            updw[node] = 0;
+            bool upd = upd1;
+            if (touched) {
+                static_for<NC>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    s.u[c] = uw[node * NC + c];
+                });
+                upd = upd || updw[node] != 0;
+                updw[node] = 0;
+                hcur++;
+                hnext = hcur < hend ? p.e2_hday[hcur] : INT_MAX;
+            }
             const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
             if (rc < 0) {
                 record_error(p, rep, day, PH_PTS, node, rc);
