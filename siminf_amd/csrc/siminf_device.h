@@ -658,8 +658,29 @@ struct AemState {
     double time[M::NT];     /* next firing time of each transition */
     double inf[M::NT];      /* reactInf */
     unsigned long long j[M::NT];
+    /* Uniform cj[J] of channel J's stream, the other half of the Philox
+     * block of its last draw (~0 when none): a channel's draws run through
+     * consecutive positions, so every second one is already computed. */
+    double cu[M::NT];
+    unsigned long long cj[M::NT];
     siminf_stream rng;      /* node stream: E1 events */
 };
+
+/* Uniform at position j of channel J's stream c, from the channel's
+ * cached half block when it holds j; the other half of a computed block
+ * is cached. */
+template <class M, int J>
+__device__ __forceinline__ double aem_uniform(AemState<M> &s, const siminf_stream &c,
+                                              unsigned long long j)
+{
+    if (s.cj[J] == j)
+        return s.cu[J];
+    double u0, u1;
+    siminf_uniform_pair(&c, (uint64_t)(j >> 1), &u0, &u1);
+    s.cj[J] = j ^ 1ull;
+    s.cu[J] = (j & 1ull) ? u0 : u1;
+    return (j & 1ull) ? u1 : u0;
+}
 
 /* calcTimes (aem.c:61-86) for channel J. */
 template <class M, int J>
@@ -674,14 +695,14 @@ __device__ __forceinline__ void calc_times(AemState<M> &s,
     c.j = s.j[J];
     if (isinf(oldtime)) {
         if (s.inf[J] == 0.0) {
-            s.time[J] = -siminf_log(siminf_uniform_at(&c, c.j)) / new_rate + tt;
+            s.time[J] = -siminf_log(aem_uniform<M, J>(s, c, c.j)) / new_rate + tt;
             s.j[J] = c.j + 1;
         } else if (new_rate > 0.0) {
             s.time[J] = tt + (s.inf[J] / new_rate);
         }
     } else if (new_rate >= 2.2250738585072014e-308) {
         if (oldtime == tt) {
-            s.time[J] = -siminf_log(siminf_uniform_at(&c, c.j)) / new_rate + tt;
+            s.time[J] = -siminf_log(aem_uniform<M, J>(s, c, c.j)) / new_rate + tt;
             s.j[J] = c.j + 1;
         } else {
             s.time[J] = ((old_rate / new_rate) * (oldtime - tt)) + tt;

@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
                 err = SIMINF_ERR_INVALID_RATE;
             const siminf_stream c = siminf_stream_make(
                 p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
-            double tj = -siminf_log(siminf_uniform_at(&c, 0)) / r + tt;
+            s.cj[j] = ~0ull;
+            double tj = -siminf_log(aem_uniform<M, j>(s, c, 0)) / r + tt;
             if (tj <= 0.0)
                 tj = INFINITY;
             s.time[j] = tj;
@@ -763,7 +764,8 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                     err = SIMINF_ERR_INVALID_RATE;
                 const siminf_stream c = siminf_stream_make(
                     p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
-                double tj = -siminf_log(siminf_uniform_at(&c, 0)) / r + tt;
+                s.cj[j] = ~0ull;
+                double tj = -siminf_log(aem_uniform<M, j>(s, c, 0)) / r + tt;
                 if (tj <= 0.0)
                     tj = INFINITY;
                 s.time[j] = tj;
