@@ -10,6 +10,12 @@ Mirrors (what, not how):
   abc_gdata                  R/abc.R:562-632: one run per proposal
   replicate_first_node       R/abc.R:307-332
   abc_init_tolerance         R/abc.R:373-406
+  init_model / post_gen      R/abc.R:598-600, :647-650, :833-834: a
+                             gdata proposal's model passes through
+                             init_model before its run (ldata: not
+                             implemented, the reference's message);
+                             post_gen sees the result after each
+                             generation
   abc_distance / abc_accept  R/abc.R:517-560
   parse_priors               R/distributions.R:170-232 (gamma, lognormal,
                              normal, uniform priors)
@@ -302,7 +308,7 @@ def _generation_ldata(model, pars, priors, n_particles, fn, generation, eps, x, 
 
 
 def _generation_gdata(model, pars, priors, n_particles, fn, generation, eps, x, w, sigma,
-                      rng, data, solver, lib):
+                      rng, data, solver, lib, init_model=None):
     xs, ancs, ds = [], [], []
     nprop = 0
     while len(xs) < n_particles:
@@ -315,6 +321,8 @@ def _generation_gdata(model, pars, priors, n_particles, fn, generation, eps, x, 
                         gdata_names=model.gdata_names)
         for k, r in enumerate(pars):
             m.gdata[r] = xx[0, k]
+        if init_model is not None:
+            m = init_model(m)
         run_model(m, solver=solver, seed=int(rng.integers(0, 2**63 - 1)), lib=lib)
         d = check_distance(fn(m, generation, data), 1)
         nprop += 1
@@ -326,11 +334,17 @@ def _generation_gdata(model, pars, priors, n_particles, fn, generation, eps, x, 
 
 
 def abc(model, priors, n_particles, distance, tolerance, data=None, seed=None,
-        solver="ssm", lib=None):
+        solver="ssm", lib=None, post_gen=None, init_model=None):
     """ABC-SMC over the tolerance schedule (one column per generation).
     distance(result, generation, data) returns one row of non-negative
     distances per proposal: per node for ldata parameters (the run model
-    holds one node per proposal), one row for gdata parameters."""
+    holds one node per proposal), one row for gdata parameters.
+    init_model(model) -> model is applied to each gdata proposal's model
+    before its run; post_gen(result) is called after each generation."""
+    if init_model is not None and not callable(init_model):
+        raise ABCError("'init_model' must be 'NULL' or a 'function'.")
+    if post_gen is not None and not callable(post_gen):
+        raise ABCError("'post_gen' must be 'NULL' or a 'function'.")
     lib = lib or _native.load()
     if tolerance is None:
         raise ABCError("The adaptive tolerance is not available: give 'tolerance'.")
@@ -341,13 +355,16 @@ def abc(model, priors, n_particles, distance, tolerance, data=None, seed=None,
     target, pars = match_priors(model, priors)
     tol = init_tolerance(tolerance)
     rng = np.random.default_rng(seed)
+    if target == "ldata" and init_model is not None:
+        raise ABCError("'init_model' callback is not implemented for 'ldata'.")
     step = _generation_ldata if target == "ldata" else _generation_gdata
+    extra = {} if target == "ldata" else {"init_model": init_model}
     out = ABCResult([p.parameter for p in priors])
     x = w = None
     for g in range(tol.shape[1]):
         sigma = None if x is None else 2.0 * np.atleast_2d(np.cov(x, rowvar=False))
         xx, anc, d, nprop = step(model, pars, priors, n_particles, distance, g + 1,
-                                 tol[:, g], x, w, sigma, rng, data, solver, lib)
+                                 tol[:, g], x, w, sigma, rng, data, solver, lib, **extra)
         w = weights(priors, x, xx, anc, w, sigma, lib)
         x = xx
         out.x.append(xx)
@@ -356,6 +373,8 @@ def abc(model, priors, n_particles, distance, tolerance, data=None, seed=None,
         out.tolerance.append(tol[:, g])
         out.ess.append(1.0 / np.sum(w * w))
         out.nprop.append(nprop)
+        if post_gen is not None:
+            post_gen(out)
     return out
 
 

@@ -134,3 +134,20 @@ def test_match_priors_and_replicate_first_node():
         A.match_priors(m, A.parse_priors(["delta ~ uniform(0, 1)"]))
     r = A.replicate_first_node(m, 4)
     assert r.Nn == 4 and r.u0[:, 3].tolist() == [99, 5, 0] and r.ldata.shape == (2, 4)
+
+
+def test_callbacks_checked_before_any_run():
+    """init_model must be None or a function, and is not implemented for
+    ldata parameters (R/abc.R:106-108, :647-650); post_gen must be
+    callable."""
+    model = si.SIR(u0={"S": [99, 80], "I": [1, 2], "R": [0, 0]}, tspan=np.arange(1, 11.0),
+                   beta=0.2, gamma=0.1)
+    dist = lambda r, g, d: np.zeros((2, 1))
+    with pytest.raises(A.ABCError, match="'init_model' must be 'NULL' or a 'function'."):
+        si.abc(model, ["beta ~ uniform(0.1, 0.5)"], 10, dist, [[1.0]], init_model=3)
+    with pytest.raises(A.ABCError, match="'post_gen' must be 'NULL' or a 'function'."):
+        si.abc(model, ["beta ~ uniform(0.1, 0.5)"], 10, dist, [[1.0]], post_gen="x")
+    with pytest.raises(A.ABCError,
+                       match="'init_model' callback is not implemented for 'ldata'."):
+        si.abc(model, ["beta ~ uniform(0.1, 0.5)"], 10, dist, [[1.0]],
+               init_model=lambda m: m)

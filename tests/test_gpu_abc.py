@@ -74,3 +74,29 @@ def test_abc_gdata_one_run_per_proposal():
                  tolerance=[0.5, 0.2], seed=2)
     assert [x.shape for x in fit.x] == [(12, 1), (12, 1)]
     assert all(n >= 12 for n in fit.nprop)
+
+
+def test_abc_gdata_callbacks():
+    """init_model sees every gdata proposal's model before its run (here
+    it fixes gamma, which the distance then reads back); post_gen sees the
+    result after each generation."""
+    model = si.SIR_mparse(u0={"S": [99], "I": [5], "R": [0]}, tspan=np.arange(1, 31.0),
+                          beta=0.16, gamma=0.077)
+    print("case: abc gdata callbacks", flush=True)
+    calls, gens = [], []
+
+    def init_model(m):
+        calls.append(float(m.gdata[0]))
+        m.gdata[1] = 0.05
+        return m
+
+    def distance(result, generation, data):
+        assert result.gdata[1] == 0.05
+        return [abs(float(result.U[1, -1]) - 10.0) / 100.0]
+
+    fit = si.abc(model, ["beta ~ uniform(0.05, 0.5)"], n_particles=8, distance=distance,
+                 tolerance=[0.5, 0.3], seed=3, init_model=init_model,
+                 post_gen=lambda r: gens.append(len(r.x)))
+    assert gens == [1, 2]
+    assert len(calls) == sum(fit.nprop)
+    assert all(0.05 <= c <= 0.5 for c in calls)
