@@ -28,8 +28,12 @@ weights, O(n_particles^2) kernel evaluations per generation, run on the
 GPU.  Proposals and acceptance are host bookkeeping over arrays, as in the
 reference.  Random numbers come from numpy's generator seeded by
 ``seed`` (the reference draws from R's), so results agree in distribution
-only.  The adaptive tolerance (tolerance None with n_init) is not built:
-give a tolerance schedule.
+only.  With tolerance None and n_init, the tolerance is adaptive
+(abc_first_epsilon R/abc.R:408-431, abc_adaptive_tolerance :446-495,
+KLIEP R/density_ratio.R:19-135: the first generation keeps the n_particles
+closest of n_init proposals; later ones take the quantile the estimated
+density ratio of successive generations gives, and stop once it exceeds
+0.99 from generation 3).
 """
 import ctypes
 import re
@@ -241,6 +245,11 @@ def check_distance(distance, n):
 
 
 def accept(distance, tolerance):
+    if tolerance is None:
+        # Every proposal of the adaptive first generation is kept.
+        if distance.shape[1] != 1:
+            raise ABCError("Adaptive tolerance must have one summary statistic.")
+        return np.ones(distance.shape[0], dtype=bool)
     """abc_accept (R/abc.R:553-560)."""
     if distance.shape[1] != len(tolerance):
         raise ABCError("Mismatch between the number of summary statistics and tolerance.")
@@ -333,9 +342,108 @@ def _generation_gdata(model, pars, priors, n_particles, fn, generation, eps, x, 
     return np.vstack(xs), np.asarray(ancs), np.vstack(ds), nprop
 
 
-def abc(model, priors, n_particles, distance, tolerance, data=None, seed=None,
-        solver="ssm", lib=None, post_gen=None, init_model=None):
-    """ABC-SMC over the tolerance schedule (one column per generation).
+def kliep_kernel(x, y, sigma):
+    """Gaussian kernel matrix exp(-|x_i - y_j|^2 / (2 sigma^2))."""
+    d2 = ((x[:, None, :] - y[None, :, :]) ** 2).sum(axis=2)
+    return np.exp(-d2 / (2.0 * sigma * sigma))
+
+
+def _ginv1(a):
+    return 0.0 if a == 0.0 else 1.0 / a
+
+
+def kliep_alpha(alpha, mean_xde):
+    """Projection of the kernel weights (KLIEP_alpha)."""
+    alpha = alpha + mean_xde * (1.0 - float(mean_xde @ alpha)) * _ginv1(float(mean_xde @ mean_xde))
+    alpha = np.maximum(alpha, 0.0)
+    return alpha * _ginv1(float(mean_xde @ alpha))
+
+
+def kliep_learning(mean_xde, xnu):
+    """Gradient ascent on the mean log density ratio (KLIEP_learning)."""
+    alpha = kliep_alpha(np.ones(xnu.shape[1]), mean_xde)
+    xa = xnu @ alpha
+    score = np.mean(np.log(xa))
+    for eps in 10.0 ** np.arange(3, -4, -1):
+        for _ in range(100):
+            new = kliep_alpha(alpha + eps * (xnu.T @ (1.0 / xa)), mean_xde)
+            xa_new = xnu @ new
+            with np.errstate(divide="ignore"):
+                score_new = np.mean(np.log(xa_new))
+            if not score_new > score:
+                break
+            alpha, xa, score = new, xa_new, score_new
+    return alpha
+
+
+def kliep(xnu, xde, rng, fold=5, kernels=100):
+    """Kullback-Leibler importance estimation with cross validation of the
+    kernel width (KLIEP); returns (centers, sigma, weights)."""
+    if not (isinstance(xnu, np.ndarray) and xnu.ndim == 2 and xnu.dtype.kind in "fiu"):
+        raise ABCError("'xnu' must be a numeric matrix.")
+    if not (isinstance(xde, np.ndarray) and xde.ndim == 2 and xde.dtype.kind in "fiu"):
+        raise ABCError("'xde' must be a numeric matrix.")
+    if xnu.shape[1] != xde.shape[1]:
+        raise ABCError("'xnu' and 'xde' must have the same dimension.")
+    xnu = xnu.astype(np.float64)
+    xde = xde.astype(np.float64)
+    kernels = min(kernels, xnu.shape[0])
+    centers = xnu[rng.choice(xnu.shape[0], size=kernels, replace=False)]
+    sigma, score = 10.0, -np.inf
+    for e in np.arange(np.log10(sigma) - 1, -6, -1):
+        for _ in range(9):
+            sigma_new = sigma - 10.0 ** e
+            Xnu = kliep_kernel(xnu, centers, sigma_new)
+            mean_xde = kliep_kernel(xde, centers, sigma_new).mean(axis=0)
+            cv = rng.permutation(xnu.shape[0]) % fold
+            score_new = 0.0
+            for i in range(fold):
+                a = kliep_learning(mean_xde, Xnu[cv != i])
+                with np.errstate(divide="ignore"):
+                    score_new += np.mean(np.log(Xnu[cv == i] @ a)) / fold
+            if not score_new > score:
+                break
+            sigma, score = sigma_new, score_new
+    Xnu = kliep_kernel(xnu, centers, sigma)
+    mean_xde = kliep_kernel(xde, centers, sigma).mean(axis=0)
+    return centers, sigma, kliep_learning(mean_xde, Xnu)
+
+
+def kliep_density_ratio(x, centers, sigma, weights):
+    return kliep_kernel(np.atleast_2d(x), centers, sigma) @ weights
+
+
+def first_epsilon(distance, n_particles):
+    """The n_particles-th smallest distance (abc_first_epsilon)."""
+    i = np.argsort(distance.sum(axis=1), kind="stable")
+    return distance[i[n_particles - 1]]
+
+
+def adaptive_tolerance(xnu, xde, distance, generation, rng):
+    """Next tolerance from the estimated density ratio of the current
+    (xnu) and previous (xde) particles, or None when the stopping rule
+    applies (abc_adaptive_tolerance)."""
+    from scipy import optimize
+    centers, sigma, wts = kliep(xnu, xde, rng)
+    f = lambda z: -float(kliep_density_ratio(np.atleast_1d(z), centers, sigma, wts)[0])
+    if xnu.shape[1] > 1:
+        res = optimize.minimize(f, xnu[0], method="Nelder-Mead")
+        c_t = -res.fun
+    else:
+        res = optimize.minimize_scalar(f, bounds=(xnu.min(), xnu.max()), method="bounded")
+        c_t = -res.fun
+    q_t = 1.0 / c_t
+    if q_t > 0.99 and generation >= 3:
+        return None
+    i = np.argsort(distance.sum(axis=1), kind="stable")
+    return distance[i[int(np.ceil(q_t * distance.shape[0])) - 1]]
+
+
+def abc(model, priors, n_particles, distance, tolerance=None, data=None, seed=None,
+        solver="ssm", lib=None, post_gen=None, init_model=None, n_init=None):
+    """ABC-SMC (abc, R/abc.R:742-960).  With a tolerance schedule (one
+    column per generation) the generations follow it; with tolerance None
+    and n_init the tolerance is adaptive (see the module notes).
     distance(result, generation, data) returns one row of non-negative
     distances per proposal: per node for ldata parameters (the run model
     holds one node per proposal), one row for gdata parameters.
@@ -345,15 +453,23 @@ def abc(model, priors, n_particles, distance, tolerance, data=None, seed=None,
         raise ABCError("'init_model' must be 'NULL' or a 'function'.")
     if post_gen is not None and not callable(post_gen):
         raise ABCError("'post_gen' must be 'NULL' or a 'function'.")
+    if tolerance is None and n_init is None:
+        raise ABCError("Both 'n_init' and 'tolerance' can not be NULL.")
     lib = lib or _native.load()
-    if tolerance is None:
-        raise ABCError("The adaptive tolerance is not available: give 'tolerance'.")
     n_particles = int(n_particles)
     if n_particles < 2:
         raise ABCError("'n_particles' must be an integer > 1.")
+    if n_init is not None:
+        n_init = int(n_init)
+        if n_init <= 1:
+            raise ABCError("'n_init' must be an integer > 1.")
+        if n_init <= n_particles:
+            raise ABCError("'n_init' must be an integer > 'n_particles'.")
+        if tolerance is not None:
+            raise ABCError("'tolerance' must be NULL for adaptive distance.")
     priors = parse_priors(priors)
     target, pars = match_priors(model, priors)
-    tol = init_tolerance(tolerance)
+    tol = None if tolerance is None else init_tolerance(tolerance)
     rng = np.random.default_rng(seed)
     if target == "ldata" and init_model is not None:
         raise ABCError("'init_model' callback is not implemented for 'ldata'.")
@@ -361,21 +477,42 @@ def abc(model, priors, n_particles, distance, tolerance, data=None, seed=None,
     extra = {} if target == "ldata" else {"init_model": init_model}
     out = ABCResult([p.parameter for p in priors])
     x = w = None
-    for g in range(tol.shape[1]):
+    eps = None if tol is None else tol[:, 0]
+    g = 0
+    while True:
+        adaptive_first = tol is None and g == 0
         sigma = None if x is None else 2.0 * np.atleast_2d(np.cov(x, rowvar=False))
-        xx, anc, d, nprop = step(model, pars, priors, n_particles, distance, g + 1,
-                                 tol[:, g], x, w, sigma, rng, data, solver, lib, **extra)
+        xx, anc, d, nprop = step(model, pars, priors, n_init if adaptive_first else n_particles,
+                                 distance, g + 1, eps, x, w, sigma, rng, data, solver, lib,
+                                 **extra)
+        if adaptive_first:
+            eps = first_epsilon(d, n_particles)
+            keep = np.argsort(d.sum(axis=1), kind="stable")[:n_particles]
+            x_old = xx
+            xx, anc, d = xx[keep], np.asarray(anc)[keep], d[keep]
+        else:
+            x_old = x
         w = weights(priors, x, xx, anc, w, sigma, lib)
         x = xx
         out.x.append(xx)
         out.weight.append(w)
         out.distance.append(d)
-        out.tolerance.append(tol[:, g])
+        out.tolerance.append(np.asarray(eps, dtype=np.float64))
         out.ess.append(1.0 / np.sum(w * w))
         out.nprop.append(nprop)
         if post_gen is not None:
             post_gen(out)
+        g += 1
+        if tol is not None:
+            if g >= tol.shape[1]:
+                break
+            eps = tol[:, g]
+        else:
+            eps = adaptive_tolerance(x, x_old, d, g + 1, rng)
+            if eps is None:
+                break
     return out
 
 
-__all__ = ["abc", "ABCResult", "ABCError", "parse_priors", "proposals", "weights"]
+__all__ = ["abc", "ABCResult", "ABCError", "parse_priors", "proposals", "weights", "kliep",
+           "adaptive_tolerance", "first_epsilon"]

@@ -151,3 +151,66 @@ def test_callbacks_checked_before_any_run():
                        match="'init_model' callback is not implemented for 'ldata'."):
         si.abc(model, ["beta ~ uniform(0.1, 0.5)"], 10, dist, [[1.0]],
                init_model=lambda m: m)
+
+
+def test_kliep_messages():
+    """tests/density_ratio.R:24-42."""
+    rng = np.random.default_rng(0)
+    with pytest.raises(A.ABCError, match="'xnu' must be a numeric matrix."):
+        A.kliep(np.arange(1, 6), np.ones((5, 1)), rng)
+    with pytest.raises(A.ABCError, match="'xnu' must be a numeric matrix."):
+        A.kliep(np.array([list("abcde")]).T, np.ones((5, 1)), rng)
+    with pytest.raises(A.ABCError, match="'xde' must be a numeric matrix."):
+        A.kliep(np.arange(1, 6.0).reshape(-1, 1), np.arange(1, 6), rng)
+    with pytest.raises(A.ABCError, match="'xnu' and 'xde' must have the same dimension."):
+        A.kliep(np.arange(1, 6.0).reshape(-1, 1), np.arange(1, 11.0).reshape(-1, 2), rng)
+
+
+def test_kliep_density_ratio_of_gaussians():
+    """The estimated ratio N(0, 0.5^2) / N(0, 1) follows the closed form
+    2 exp(-3 x^2 / 2) near the numerator's mass."""
+    rng = np.random.default_rng(1)
+    xnu = rng.normal(0.0, 0.5, size=(400, 1))
+    xde = rng.normal(0.0, 1.0, size=(400, 1))
+    centers, sigma, w = A.kliep(xnu, xde, rng)
+    for x in (0.0, 0.3, -0.5):
+        est = float(A.kliep_density_ratio(np.array([x]), centers, sigma, w)[0])
+        exact = 2.0 * np.exp(-1.5 * x * x)
+        assert abs(est - exact) < 0.35 * exact, (x, est, exact)
+
+
+def test_adaptive_tolerance_rules():
+    """abc_first_epsilon and abc_adaptive_tolerance (R/abc.R:408-500):
+    the first tolerance is the n_particles-th smallest distance; a
+    generation much tighter than the last gives a quantile well inside
+    the distances."""
+    d = np.array([[5.0], [1.0], [3.0], [2.0], [4.0]])
+    np.testing.assert_array_equal(A.first_epsilon(d, 3), [3.0])
+    rng = np.random.default_rng(2)
+    xde = rng.normal(0.0, 1.0, size=(300, 1))
+    xnu = rng.normal(0.0, 0.3, size=(300, 1))
+    dist = np.abs(rng.normal(size=(300, 1)))
+    eps = A.adaptive_tolerance(xnu, xde, dist, 2, rng)
+    assert eps is not None and eps[0] < np.quantile(dist, 0.6)
+    # Identical particle sets: the ratio is flat (q_t ~ 1), so the rule
+    # stops from generation 3 and before that keeps nearly all distances.
+    same = rng.normal(0.0, 1.0, size=(300, 1))
+    assert A.adaptive_tolerance(same, same.copy(), dist, 3, rng) is None
+    eps2 = A.adaptive_tolerance(same, same.copy(), dist, 2, rng)
+    assert eps2[0] >= np.quantile(dist, 0.9)
+
+
+def test_adaptive_arguments():
+    """abc_init_n_particles and abc (R/abc.R:344-359, :750-752)."""
+    model = si.SIR(u0={"S": [99], "I": [1], "R": [0]}, tspan=np.arange(1, 11.0),
+                   beta=0.2, gamma=0.1)
+    dist = lambda r, g, d: np.zeros((1, 1))
+    pr = ["beta ~ uniform(0.1, 0.5)"]
+    with pytest.raises(A.ABCError, match="Both 'n_init' and 'tolerance' can not be NULL."):
+        si.abc(model, pr, 10, dist, None)
+    with pytest.raises(A.ABCError, match="'n_init' must be an integer > 'n_particles'."):
+        si.abc(model, pr, 10, dist, None, n_init=10)
+    with pytest.raises(A.ABCError, match="'n_init' must be an integer > 1."):
+        si.abc(model, pr, 10, dist, None, n_init=1)
+    with pytest.raises(A.ABCError, match="'tolerance' must be NULL for adaptive distance."):
+        si.abc(model, pr, 10, dist, [1.0], n_init=20)

@@ -100,3 +100,70 @@ def test_abc_gdata_callbacks():
     assert gens == [1, 2]
     assert len(calls) == sum(fit.nprop)
     assert all(0.05 <= c <= 0.5 for c in calls)
+
+
+class _Enough(Exception):
+    pass
+
+
+def test_abc_adaptive_ldata():
+    """Adaptive tolerance (tolerance None, n_init proposals first): the
+    first generation keeps the n_particles closest of n_init proposals and
+    its tolerance is the largest kept distance; later tolerances come from
+    the density ratio of successive generations and do not grow."""
+    truth = si.SIR(u0={"S": [990], "I": [10], "R": [0]}, tspan=np.arange(1, 61.0),
+                   beta=0.3, gamma=0.1)
+    si.run(truth, seed=11)
+    observed = _final_recovered(truth)[0]
+    print("case: abc adaptive ldata, observed R=%d" % observed, flush=True)
+
+    def distance(result, generation, data):
+        return np.abs(_final_recovered(result) - data).reshape(-1, 1) / 1000.0
+
+    def post_gen(r):
+        if len(r.x) == 4:
+            raise _Enough()
+
+    model = si.SIR(u0={"S": [990], "I": [10], "R": [0]}, tspan=np.arange(1, 61.0),
+                   beta=0.2, gamma=0.1)
+    seen = []
+    with pytest.raises(_Enough):
+        si.abc(model, ["beta ~ uniform(0.05, 0.8)"], n_particles=150, distance=distance,
+               data=observed, seed=7, n_init=450,
+               post_gen=lambda r: (seen.append(r), post_gen(r)))
+    fit = seen[-1]
+    assert fit.nprop[0] == 450 and len(fit.x) == 4
+    assert fit.distance[0].shape == (150, 1)
+    assert fit.tolerance[0][0] == fit.distance[0].max()
+    for g in range(4):
+        assert np.all(fit.distance[g] <= fit.tolerance[g])
+        assert abs(fit.weight[g].sum() - 1.0) < 1e-12
+    tol = [t[0] for t in fit.tolerance]
+    assert all(b <= a for a, b in zip(tol, tol[1:])) and tol[-1] < tol[0]
+    mean = float(np.sum(fit.weight[-1] * fit.x[-1][:, 0]))
+    assert 0.2 < mean < 0.45
+
+
+def test_abc_adaptive_gdata():
+    model = si.SIR_mparse(u0={"S": [99], "I": [5], "R": [0]}, tspan=np.arange(1, 31.0),
+                          beta=0.16, gamma=0.077)
+    print("case: abc adaptive gdata", flush=True)
+
+    def distance(result, generation, data):
+        return [abs(float(result.U[1, -1]) - 10.0) / 100.0]
+
+    seen = []
+
+    def post_gen(r):
+        seen.append(r)
+        if len(r.x) == 3:
+            raise _Enough()
+
+    with pytest.raises(_Enough):
+        si.abc(model, ["beta ~ uniform(0.05, 0.5)"], n_particles=10, distance=distance,
+               seed=4, n_init=25, post_gen=post_gen)
+    fit = seen[-1]
+    assert [x.shape for x in fit.x] == [(10, 1)] * 3
+    assert fit.nprop[0] == 25 and all(n >= 10 for n in fit.nprop[1:])
+    for g in range(3):
+        assert np.all(fit.distance[g] <= fit.tolerance[g])
