@@ -320,11 +320,20 @@ int SimInf_hip_plan_prevalence(SimInf_hip_plan *plan, unsigned int cases_mask,
 
 /* Trajectory of the compartments in comp_mask (in model order) for the
  * nodes in index (zero-based, or NULL for all) from the resident U, in
- * the layout of trajectory(..., format = "matrix")
- * (SimInf_trajectory.c:566-650): rows node-major, then compartment;
- * columns the Nrep*tlen U columns; column-major int32. */
+ * the layout of trajectory(..., format = "matrix") (trajectory_as_is,
+ * R/trajectory.R:73-85, called at R/trajectory.R:220-225): rows
+ * node-major, then compartment; columns the Nrep*tlen U columns;
+ * column-major int32. */
 int SimInf_hip_plan_trajectory(SimInf_hip_plan *plan, unsigned int comp_mask,
                                const int *index, int n_index, int *out);
+
+/* The same for continuous states: the states in state_mask (bit i is
+ * row i of v0) from the resident dense V, as trajectory(..., format =
+ * "matrix") returns them when only V states are selected
+ * (R/trajectory.R:227-228); column-major double.  Errors as above, and
+ * SIMINF_ERR_INVALID_SUMMARY when V was not kept dense or Nd is 0. */
+int SimInf_hip_plan_trajectory_v(SimInf_hip_plan *plan, unsigned int state_mask,
+                                 const int *index, int n_index, double *out);
 
 /* ABC-SMC weights of the current generation (SimInf_abc_weights,
  * SimInf_abc.c:283-406), on the device.  distribution holds one letter

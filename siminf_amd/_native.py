@@ -85,6 +85,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_pfilter",
     "SimInf_hip_plan_prevalence",
     "SimInf_hip_plan_trajectory",
+    "SimInf_hip_plan_trajectory_v",
     "SimInf_hip_abc_weights",
     "SimInf_hip_plan_set_lanes_per_wave",
     "SimInf_hip_plan_set_herd_map",
@@ -148,6 +149,9 @@ def load(path=None):
         ctypes.c_int, dp]
     lib.SimInf_hip_plan_trajectory.restype = ctypes.c_int
     lib.SimInf_hip_plan_trajectory.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int, ip]
+    lib.SimInf_hip_plan_trajectory_v.restype = ctypes.c_int
+    lib.SimInf_hip_plan_trajectory_v.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_double)]
     lib.SimInf_hip_abc_weights.restype = ctypes.c_int
     lib.SimInf_hip_abc_weights.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int,
                                            ctypes.c_int, dp, dp, dp, dp, dp]

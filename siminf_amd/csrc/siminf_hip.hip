@@ -2002,11 +2002,13 @@ __global__ __launch_bounds__(256) void k_prevalence_nodes(const int *U, int Nn, 
     out[g] = (double)x / (double)y;
 }
 
-/* Selected compartments of selected nodes, one lane per (node, column). */
-__global__ __launch_bounds__(256) void k_trajectory(const int *U, int Nn, int Nc,
+/* Selected rows (compartments of U, or continuous states of V) of
+ * selected nodes, one lane per (node, column). */
+template <typename T>
+__global__ __launch_bounds__(256) void k_trajectory(const T *U, int Nn, int Nc,
                                                     unsigned mask, int ncomp,
                                                     const int *index, int n_sel,
-                                                    long long total, int *out)
+                                                    long long total, T *out)
 {
     const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
     if (g >= total)
@@ -2014,8 +2016,8 @@ __global__ __launch_bounds__(256) void k_trajectory(const int *U, int Nn, int Nc
     const long long col = g / n_sel;
     const int k = (int)(g - col * n_sel);
     const int node = index ? index[k] : k;
-    const int *u = U + (col * Nn + node) * (long long)Nc;
-    int *o = out + g * ncomp;
+    const T *u = U + (col * Nn + node) * (long long)Nc;
+    T *o = out + g * ncomp;
     int j = 0;
     for (int c = 0; c < Nc; c++)
         if ((mask >> c) & 1u)
@@ -2113,6 +2115,51 @@ extern "C" int SimInf_hip_plan_prevalence(SimInf_hip_plan *P, unsigned int cases
     return T.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
 }
 
+/* Gather of the selected rows of the resident dense U (T int, Nr = Nc)
+ * or V (T double, Nr = Nd) for trajectory(..., format = "matrix"). */
+template <typename T>
+static int plan_trajectory(SimInf_hip_plan *P, const T *src, int Nr, unsigned int comp_mask,
+                           const int *index, int n_index, T *out)
+{
+    const int ncomp = __builtin_popcount(comp_mask);
+    const int n_sel = index ? n_index : P->Nn;
+    const long long total = (long long)n_sel * P->Nrep * P->tlen;
+    DeviceBuffers T_;
+    T_.stream = P->stream;
+    T_.pooled = P->buf.pooled;
+    const int *idx_d = index ? T_.upload(index, (size_t)n_index) : nullptr;
+    T *out_d = T_.alloc<T>((size_t)total * ncomp);
+    if (T_.status != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    hipLaunchKernelGGL(k_trajectory<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       P->stream, src, P->Nn, Nr, comp_mask, ncomp, idx_d, n_sel, total,
+                       out_d);
+    if (hipGetLastError() != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    T_.copy(out, out_d, (size_t)total * ncomp * sizeof(T), hipMemcpyDeviceToHost);
+    return T_.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
+}
+
+extern "C" int SimInf_hip_plan_trajectory_v(SimInf_hip_plan *P, unsigned int state_mask,
+                                            const int *index, int n_index, double *out)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (!P->has_run)
+        return SIMINF_ERR_EMPTY_TRAJECTORY;
+    if (!P->dense_V || P->Nd < 1 || P->Nd > 32 || !out || !state_mask ||
+        (state_mask & ~compartment_bits(P->Nd)))
+        return SIMINF_ERR_INVALID_SUMMARY;
+    if (index) {
+        if (n_index < 1)
+            return SIMINF_ERR_INVALID_SUMMARY;
+        for (int k = 0; k < n_index; k++)
+            if (index[k] < 0 || index[k] >= P->Nn || (k > 0 && index[k] <= index[k - 1]))
+                return SIMINF_ERR_INVALID_SUMMARY;
+    }
+    return plan_trajectory<double>(P, P->p.V, P->Nd, state_mask, index, n_index, out);
+}
+
 extern "C" int SimInf_hip_plan_trajectory(SimInf_hip_plan *P, unsigned int comp_mask,
                                           const int *index, int n_index, int *out)
 {
@@ -2121,23 +2168,7 @@ extern "C" int SimInf_hip_plan_trajectory(SimInf_hip_plan *P, unsigned int comp_
         return err;
     if (!out || !comp_mask || (comp_mask & ~compartment_bits(P->Nc)))
         return SIMINF_ERR_INVALID_SUMMARY;
-    const int ncomp = __builtin_popcount(comp_mask);
-    const int n_sel = index ? n_index : P->Nn;
-    const long long total = (long long)n_sel * P->Nrep * P->tlen;
-    DeviceBuffers T;
-    T.stream = P->stream;
-    T.pooled = P->buf.pooled;
-    const int *idx_d = index ? T.upload(index, (size_t)n_index) : nullptr;
-    int *out_d = T.alloc<int>((size_t)total * ncomp);
-    if (T.status != hipSuccess)
-        return SIMINF_ERR_DEVICE;
-    hipLaunchKernelGGL(k_trajectory, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       P->stream, P->p.U, P->Nn, P->Nc, comp_mask, ncomp, idx_d, n_sel,
-                       total, out_d);
-    if (hipGetLastError() != hipSuccess)
-        return SIMINF_ERR_DEVICE;
-    T.copy(out, out_d, (size_t)total * ncomp * sizeof(int), hipMemcpyDeviceToHost);
-    return T.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
+    return plan_trajectory<int>(P, P->p.U, P->Nc, comp_mask, index, n_index, out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -136,7 +136,22 @@ class Plan:
     def trajectory(self, compartments=None, index=None):
         """trajectory(model, compartments, index, format = "matrix") of the
         last run, gathered on the device: rows node-major then compartment
-        (model order), one column per U column."""
+        (model order), one column per U column.  Selecting only continuous
+        states (names of v0's rows) gathers them from V instead, as float64
+        (R/trajectory.R:220-228); U and V names together raise the
+        reference's message."""
+        names = summaries.selected_names(compartments)
+        vnames = [n for n in (names or []) if n in self.model.v0_names
+                  and n not in self.model.compartments]
+        if vnames:
+            bad = [n for n in names if n not in self.model.compartments
+                   and n not in self.model.v0_names]
+            if bad:
+                raise summaries.SummaryError("Non-existing compartment(s) in model: %s." %
+                                             ", ".join("'%s'" % x for x in bad))
+            if len(vnames) != len(names):
+                raise summaries.SummaryError("Cannot combine data from different slots.")
+            return self._trajectory_v(vnames, index)
         m = summaries.trajectory_mask(compartments, self.model.compartments)
         idx = summaries.node_index(index, self.model.Nn)
         n_sel = self.model.Nn if idx is None else len(idx)
@@ -146,6 +161,17 @@ class Plan:
         self._check(self.lib.SimInf_hip_plan_trajectory(
             self._handle, m, None if idx is None else idx.ctypes.data_as(ip),
             0 if idx is None else len(idx), out.ctypes.data_as(ip)))
+        return out
+
+    def _trajectory_v(self, names, index):
+        m = summaries.mask(names, self.model.v0_names)
+        idx = summaries.node_index(index, self.model.Nn)
+        n_sel = self.model.Nn if idx is None else len(idx)
+        out = np.zeros((n_sel * len(names), self._columns()), dtype=np.float64, order="F")
+        self._check(self.lib.SimInf_hip_plan_trajectory_v(
+            self._handle, m, None if idx is None else idx.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_int)),
+            0 if idx is None else len(idx), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
 

@@ -280,6 +280,15 @@ def node_index(index, n_nodes):
     return np.unique(idx.astype(np.int64)).astype(np.int32) - 1
 
 
+def selected_names(compartments_arg):
+    """The distinct, stripped names of a compartments argument (None for
+    all)."""
+    if compartments_arg is None:
+        return None
+    names = [compartments_arg] if isinstance(compartments_arg, str) else list(compartments_arg)
+    return list(dict.fromkeys(n.strip() for n in names))
+
+
 def trajectory_mask(compartments_arg, compartments):
     """Mask of the compartments to extract (all when None); the rows come
     out in model order."""

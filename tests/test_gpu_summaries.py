@@ -96,3 +96,29 @@ def test_large_population_level1():
                                                       m.compartments, 1))
         np.testing.assert_array_equal(p2, P.prevalence(m.U, m.compartments, ["E", "I"],
                                                        m.compartments, 2))
+
+
+def test_trajectory_of_continuous_state():
+    """trajectory(model, "phi", index, format = "matrix") gathers rows of
+    V (R/trajectory.R:73-85, :227-228); U and V names together raise the
+    reference's message."""
+    _case("trajectory of V (SISe phi), 2 replicates")
+    n = 12
+    m = si.SISe(u0={"S": np.arange(50, 50 + n), "I": np.arange(n) % 4},
+                tspan=np.arange(1, 41.0), phi=np.linspace(0.0, 1.0, n), upsilon=0.02,
+                gamma=0.1, alpha=1.0, beta_t1=0.12, beta_t2=0.1, beta_t3=0.12, beta_t4=0.1,
+                end_t1=91, end_t2=182, end_t3=273, end_t4=365, epsilon=0.001,
+                replicates=2)
+    assert m.Nd == 1 and m.v0_names == ["phi"]
+    with si.Plan(m) as plan:
+        plan.run(seed=5)
+        plan.download()
+        V = np.asarray(m.V)
+        assert V.shape == (n, 2 * 40) and np.any(V != V[:, :1])
+        np.testing.assert_array_equal(plan.trajectory("phi"), V)
+        np.testing.assert_array_equal(plan.trajectory(["phi"], index=[7, 2, 11]),
+                                      V[[1, 6, 10]])
+        with pytest.raises(S.SummaryError, match="Cannot combine data from different slots."):
+            plan.trajectory(["I", "phi"])
+        with pytest.raises(S.SummaryError, match="Non-existing compartment"):
+            plan.trajectory(["phi", "Q"])
