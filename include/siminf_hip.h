@@ -81,6 +81,12 @@ typedef enum {
     SIMINF_ERR_ABC_WEIGHTS          = -107
 } SimInf_error_code;
 
+/* The most compartments (Nc), transitions (Nt) and continuous states
+ * (Nd) a model may have: the library holds sets of them as 32-bit masks.
+ * Plans of larger models are refused with SIMINF_ERR_INVALID_MODEL (the
+ * reference has no such limit). */
+#define SIMINF_MAX_SET 32
+
 /* Plugin signatures (SimInf.h:92-107). */
 typedef double (*TRFun)(const int *u, const double *v, const double *ldata,
                         const double *gdata, double t);

@@ -322,10 +322,16 @@ def _generation_gdata(model, pars, priors, n_particles, fn, generation, eps, x, 
     nprop = 0
     while len(xs) < n_particles:
         xx, anc = proposals(priors, 1, x, w, sigma, rng)
-        m = SimInfModel(model.plugin, model.G, model.S, model.tspan, model.u0,
-                        ldata=model.ldata, gdata=model.gdata.copy(),
-                        v0=model.v0 if model.Nd else None, E=model.E, N=model.N,
-                        events=model.events, compartments=model.compartments,
+        # Each proposal's model owns copies of the arrays an init_model
+        # callback may change in place (the reference's models are R
+        # values, copied on modification).
+        events = {k: model.events[k].copy() for k in
+                  ("event", "time", "node", "dest", "n", "proportion", "select", "shift")}
+        m = SimInfModel(model.plugin, model.G, model.S, model.tspan, model.u0.copy(),
+                        ldata=model.ldata.copy(), gdata=model.gdata.copy(),
+                        v0=model.v0.copy() if model.Nd else None, E=model.E.copy(),
+                        N=None if model.N is None else model.N.copy(),
+                        events=events, compartments=model.compartments,
                         v0_names=model.v0_names, ldata_names=model.ldata_names,
                         gdata_names=model.gdata_names)
         for k, r in enumerate(pars):
@@ -432,11 +438,18 @@ def adaptive_tolerance(xnu, xde, distance, generation, rng):
     else:
         res = optimize.minimize_scalar(f, bounds=(xnu.min(), xnu.max()), method="bounded")
         c_t = -res.fun
+    if not (np.isfinite(c_t) and c_t > 0.0):
+        raise ABCError("Unable to compute the adaptive tolerance: the density ratio's "
+                       "supremum is not finite and positive.")
     q_t = 1.0 / c_t
     if q_t > 0.99 and generation >= 3:
         return None
+    # The rank is clamped to [1, n]: an optimizer that stops below the
+    # supremum of the ratio (q_t > 1) takes the largest distance.
     i = np.argsort(distance.sum(axis=1), kind="stable")
-    return distance[i[int(np.ceil(q_t * distance.shape[0])) - 1]]
+    n = distance.shape[0]
+    rank = min(max(int(np.ceil(q_t * n)), 1), n)
+    return distance[i[rank - 1]]
 
 
 def abc(model, priors, n_particles, distance, tolerance=None, data=None, seed=None,

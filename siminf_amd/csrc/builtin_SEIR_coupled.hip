@@ -1,0 +1,7 @@
+/*
+ * builtin_SEIR_coupled.hip -- the coupled kernels (external transfers)
+ * instantiated for the built-in model SEIR.
+ */
+#include "siminf_builtin_models.h"
+
+SIMINF_COUPLED_INSTANCE(, siminf::ModelSEIR);

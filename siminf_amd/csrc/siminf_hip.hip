@@ -40,139 +40,32 @@ using namespace siminf;
 /* ------------------------------------------------------------------ */
 
 
-struct ModelSIR {
-    static constexpr int NC = 3, NT = 2, ND = 0, NLD = 2, NGD = 0;
-    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
-    static constexpr int S[NT][NC] =
-        {{-1, 1, 0}, {0, -1, 1}};
-    static constexpr int G[NT][NT] =
-        {{1, 1}, {1, 1}};
-    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
-    {
-        if constexpr (J == 0) return SIR_S_to_I(u, v, ld, gd, t);
-        else return SIR_I_to_R(u, v, ld, gd, t);
-    }
-    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
-    {
-        return siminf_pts_noop(vn, u, v, ld, gd, node, t);
-    }
-};
-
-struct ModelSEIR {
-    static constexpr int NC = 4, NT = 3, ND = 0, NLD = 3, NGD = 0;
-    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
-    static constexpr int S[NT][NC] =
-        {{-1, 1, 0, 0}, {0, -1, 1, 0}, {0, 0, -1, 1}};
-    static constexpr int G[NT][NT] =
-        {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};
-    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
-    {
-        if constexpr (J == 0) return SEIR_S_to_E(u, v, ld, gd, t);
-        else if constexpr (J == 1) return SEIR_E_to_I(u, v, ld, gd, t);
-        else return SEIR_I_to_R(u, v, ld, gd, t);
-    }
-    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
-    {
-        return siminf_pts_noop(vn, u, v, ld, gd, node, t);
-    }
-};
-
-struct ModelSIS {
-    static constexpr int NC = 2, NT = 2, ND = 0, NLD = 2, NGD = 0;
-    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
-    static constexpr int S[NT][NC] =
-        {{-1, 1}, {1, -1}};
-    static constexpr int G[NT][NT] =
-        {{1, 1}, {1, 1}};
-    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
-    {
-        if constexpr (J == 0) return SIS_S_to_I(u, v, ld, gd, t);
-        else return SIS_I_to_S(u, v, ld, gd, t);
-    }
-    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
-    {
-        return siminf_pts_noop(vn, u, v, ld, gd, node, t);
-    }
-};
-
-struct ModelSISe {
-    static constexpr int NC = 2, NT = 2, ND = 1, NLD = 4, NGD = 8;
-    static constexpr int S[NT][NC] =
-        {{-1, 1}, {1, -1}};
-    static constexpr int G[NT][NT] =
-        {{1, 1}, {1, 1}};
-    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
-    {
-        if constexpr (J == 0) return SISe_S_to_I(u, v, ld, gd, t);
-        else return SISe_I_to_S(u, v, ld, gd, t);
-    }
-    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
-    {
-        return SISe_pts(vn, u, v, ld, gd, node, t);
-    }
-};
-
-struct ModelSISe3 {
-    static constexpr int NC = 6, NT = 6, ND = 1, NLD = 4, NGD = 12;
-    static constexpr int S[NT][NC] =
-        {{-1, 1, 0, 0, 0, 0}, {1, -1, 0, 0, 0, 0}, {0, 0, -1, 1, 0, 0},
-         {0, 0, 1, -1, 0, 0}, {0, 0, 0, 0, -1, 1}, {0, 0, 0, 0, 1, -1}};
-    static constexpr int G[NT][NT] =
-        {{1, 1, 0, 0, 0, 0}, {1, 1, 0, 0, 0, 0}, {0, 0, 1, 1, 0, 0},
-         {0, 0, 1, 1, 0, 0}, {0, 0, 0, 0, 1, 1}, {0, 0, 0, 0, 1, 1}};
-    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
-    {
-        if constexpr (J == 0) return SISe3_S_1_to_I_1(u, v, ld, gd, t);
-        else if constexpr (J == 1) return SISe3_I_1_to_S_1(u, v, ld, gd, t);
-        else if constexpr (J == 2) return SISe3_S_2_to_I_2(u, v, ld, gd, t);
-        else if constexpr (J == 3) return SISe3_I_2_to_S_2(u, v, ld, gd, t);
-        else if constexpr (J == 4) return SISe3_S_3_to_I_3(u, v, ld, gd, t);
-        else return SISe3_I_3_to_S_3(u, v, ld, gd, t);
-    }
-    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
-    {
-        return SISe3_pts(vn, u, v, ld, gd, node, t);
-    }
-};
-
-struct ModelSIRmparse {
-    static constexpr int NC = 3, NT = 2, ND = 0, NLD = 0, NGD = 2;
-    static constexpr bool PTS_NOOP = true;   /* pts never changes anything */
-    static constexpr int S[NT][NC] =
-        {{-1, 1, 0}, {0, -1, 1}};
-    static constexpr int G[NT][NT] =
-        {{1, 1}, {1, 1}};
-    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
-    {
-        if constexpr (J == 0) return SIR_mparse_trFun1(u, v, ld, gd, t);
-        else return SIR_mparse_trFun2(u, v, ld, gd, t);
-    }
-    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
-    {
-        return siminf_pts_noop(vn, u, v, ld, gd, node, t);
-    }
-};
-
 /* gdata lengths: SISe.R:177-178 (8 values), SISe3.R:188-190 (12), the
  * mparse SIR (beta, gamma); SIR, SEIR and SIS read ldata only.  S and G
  * are the constructors' matrices (SIR.R:127-133, SEIR.R:136-143,
- * SIS.R:122-128, SISe.R:146-151, SISe3.R:162-178). */
-static const SimInf_hip_model builtin_models[] = {
-    SIMINF_MODEL_ENTRY("SIR", ModelSIR, 0),
-    SIMINF_MODEL_ENTRY("SEIR", ModelSEIR, 0),
-    SIMINF_MODEL_ENTRY("SIS", ModelSIS, 0),
-    SIMINF_MODEL_ENTRY("SISe", ModelSISe, 8),
-    SIMINF_MODEL_ENTRY("SISe3", ModelSISe3, 12),
-    SIMINF_MODEL_ENTRY("SIR_mparse", ModelSIRmparse, 2),
+ * SIS.R:122-128, SISe.R:146-151, SISe3.R:162-178).  Each record is
+ * defined in the model's own translation unit (builtin_*.hip). */
+const SimInf_hip_model *siminf_builtin_SIR();
+const SimInf_hip_model *siminf_builtin_SEIR();
+const SimInf_hip_model *siminf_builtin_SIS();
+const SimInf_hip_model *siminf_builtin_SISe();
+const SimInf_hip_model *siminf_builtin_SISe3();
+const SimInf_hip_model *siminf_builtin_SIR_mparse();
+
+static const SimInf_hip_model *(*const builtin_models[])() = {
+    siminf_builtin_SIR, siminf_builtin_SEIR, siminf_builtin_SIS,
+    siminf_builtin_SISe, siminf_builtin_SISe3, siminf_builtin_SIR_mparse,
 };
 
 extern "C" const SimInf_hip_model *SimInf_hip_model_builtin(const char *name)
 {
     if (!name)
         return nullptr;
-    for (const SimInf_hip_model &m : builtin_models)
-        if (std::strcmp(m.name, name) == 0)
-            return &m;
+    for (auto get : builtin_models) {
+        const SimInf_hip_model *m = get();
+        if (std::strcmp(m->name, name) == 0)
+            return m;
+    }
     return nullptr;
 }
 
@@ -437,6 +330,10 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     const int Nrep = a->Nrep < 1 ? 1 : a->Nrep;
     if (Nn <= 0 || Nc != model->Nc || Nt != model->Nt || Nd != model->Nd ||
         a->tlen <= 0 || a->Nld < 0 || a->len < 0)
+        return SIMINF_ERR_INVALID_MODEL;
+    /* S, G, E and the summary masks are 32-bit sets of compartments,
+     * transitions and continuous states. */
+    if (Nc > SIMINF_MAX_SET || Nt > SIMINF_MAX_SET || Nd > SIMINF_MAX_SET)
         return SIMINF_ERR_INVALID_MODEL;
     if (a->solver != 0 && a->solver != 1)
         return SIMINF_ERR_UNKNOWN_SOLVER;

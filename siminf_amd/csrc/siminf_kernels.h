@@ -1013,8 +1013,8 @@ namespace siminf {
 /* Models without E1 events get a kernel without the event code, which
  * needs fewer registers. */
 template <class M>
-static void launch_ssm_nodes(const KParams &p, bool events, bool canon,
-                             unsigned grid, unsigned block, hipStream_t st)
+void launch_ssm_nodes(const KParams &p, bool events, bool canon,
+                      unsigned grid, unsigned block, hipStream_t st)
 {
     if (events && canon)
         hipLaunchKernelGGL((k_ssm_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
@@ -1027,8 +1027,8 @@ static void launch_ssm_nodes(const KParams &p, bool events, bool canon,
 }
 
 template <class M>
-static void launch_aem_nodes(const KParams &p, bool events, bool canon,
-                             unsigned grid, unsigned block, hipStream_t st)
+void launch_aem_nodes(const KParams &p, bool events, bool canon,
+                      unsigned grid, unsigned block, hipStream_t st)
 {
     if (events && canon)
         hipLaunchKernelGGL((k_aem_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
@@ -1042,7 +1042,7 @@ static void launch_aem_nodes(const KParams &p, bool events, bool canon,
 
 /* Clears a grid launch's barrier words (a kernel, not a memset, so that
  * the launch sequence holds kernels only). */
-__global__ void k_clear_u32(unsigned *w, int n)
+static __global__ void k_clear_u32(unsigned *w, int n)
 {
     if ((int)threadIdx.x < n)
         w[threadIdx.x] = 0u;
@@ -1072,9 +1072,9 @@ static const void *coupled_aem_fn()
  * workgroups that must all be resident; max_blocks (out, optional)
  * reports how many can be, for the caller's planning. */
 template <class M>
-static hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
-                                 unsigned grid, unsigned block, size_t lds,
-                                 hipStream_t st, int *max_blocks)
+hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
+                          unsigned grid, unsigned block, size_t lds,
+                          hipStream_t st, int *max_blocks)
 {
     if (p.coupled_aem && !grid_mode)
         return hipErrorInvalidValue;

@@ -39,9 +39,17 @@ def _fun_lines(parsed):
             mp.C_ptsFun(parsed.pts_fun))
 
 
+# The library holds sets of compartments, transitions and continuous
+# states as 32-bit masks (SIMINF_MAX_SET in include/siminf_hip.h).
+MAX_SET = 32
+
+
 def device_source(parsed, name):
     NC, NT = parsed.S.shape
     ND = parsed.Nd
+    if max(NC, NT, ND) > MAX_SET:
+        raise ValueError("The GPU solver supports at most %d compartments, %d transitions "
+                         "and %d continuous states." % (MAX_SET, MAX_SET, MAX_SET))
     NLD = parsed.Nld
     NGD = parsed.Ngdata
     S = [[int(parsed.S[c, k]) for c in range(NC)] for k in range(NT)]
@@ -132,7 +140,7 @@ def compile_device(parsed, name="mparse"):
 
 def _compile_device(parsed, name):
     src = device_source(parsed, name)
-    flags = list(_build.FLAGS)
+    flags = list(_build.FLAGS) + ["-shared"]
     key = _digest(src, flags)
     os.makedirs(CACHE, exist_ok=True)
     so = os.path.join(CACHE, "dev_%s.so" % key)

@@ -163,8 +163,11 @@ class PFilterServer:
 def host_pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5,
                  seed=0, lib=None, obs_dist="binomial", obs_offset=0.0):
     """The filter driven from the host with one run() per interval (the
-    plain path).  Returns (loglik, ess)."""
+    plain path).  Returns (loglik, ess).  Models with scheduled events are
+    refused: this baseline does not split them over the intervals."""
     from scipy import stats
+    if len(model.events["event"]):
+        raise ValueError("host_pfilter runs models without scheduled events.")
     lib = lib or _native.load()
     tspan2, obs, obs_nodes = particle_filter.prepare(model, obs_compartment, times, obs)
     P = int(n_particles)

@@ -214,3 +214,55 @@ def test_adaptive_arguments():
         si.abc(model, pr, 10, dist, None, n_init=1)
     with pytest.raises(A.ABCError, match="'tolerance' must be NULL for adaptive distance."):
         si.abc(model, pr, 10, dist, [1.0], n_init=20)
+
+
+def test_adaptive_tolerance_rank_is_clamped(monkeypatch):
+    """A density-ratio supremum below 1 (an optimizer stopping short of it)
+    gives q_t > 1: the rank is clamped to the largest distance instead of
+    indexing past the end; a ratio of 0 is an error."""
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(50, 1))
+    dist = np.arange(50.0).reshape(50, 1)[::-1].copy()
+    monkeypatch.setattr(A, "kliep", lambda xnu, xde, rng: (None, None, None))
+    monkeypatch.setattr(A, "kliep_density_ratio",
+                        lambda z, c, s, w: np.full(len(np.atleast_2d(z)), 0.5))
+    np.testing.assert_array_equal(A.adaptive_tolerance(x, x, dist, 2, rng), [49.0])
+    monkeypatch.setattr(A, "kliep_density_ratio",
+                        lambda z, c, s, w: np.zeros(len(np.atleast_2d(z))))
+    with pytest.raises(A.ABCError, match="adaptive tolerance"):
+        A.adaptive_tolerance(x, x, dist, 2, rng)
+
+
+def test_init_model_gets_copies(monkeypatch):
+    """An init_model callback that edits its model in place (ldata, v0,
+    events) changes neither the caller's model nor later proposals."""
+    model = si.SISe(u0={"S": [90, 80], "I": [1, 2]}, tspan=np.arange(1, 6.0), phi=[0.1, 0.2],
+                    upsilon=0.01, gamma=0.1, alpha=1.0, beta_t1=0.1, beta_t2=0.1,
+                    beta_t3=0.1, beta_t4=0.1, end_t1=91, end_t2=182, end_t3=273,
+                    end_t4=365, epsilon=0.0,
+                    events={"event": [0], "time": [2], "node": [1], "dest": [0], "n": [1],
+                            "proportion": [0.0], "select": [1], "shift": [0]})
+    before = (model.ldata.copy(), model.v0.copy(), model.u0.copy(),
+              {k: v.copy() for k, v in model.events.items() if isinstance(v, np.ndarray)})
+    seen = []
+
+    def edit(m):
+        seen.append((m.ldata[0, 0], m.v0[0, 0], int(m.events["n"][0])))
+        m.ldata[0, 0] += 1.0
+        m.v0[0, 0] += 1.0
+        m.events["n"][0] += 1
+        m.u0[0, 0] += 1
+        return m
+
+    monkeypatch.setattr(A, "run_model", lambda m, **kw: None)
+    priors = A.parse_priors(["upsilon ~ uniform(0.005, 0.02)"])
+    _, pars = A.match_priors(model, priors)
+    A._generation_gdata(model, pars, priors, 3, lambda m, g, d: np.zeros((1, 1)), 1,
+                        np.array([1.0]), None, None, None, np.random.default_rng(1), None,
+                        "ssm", None, init_model=edit)
+    assert len(seen) == 3 and len(set(seen)) == 1
+    np.testing.assert_array_equal(model.ldata, before[0])
+    np.testing.assert_array_equal(model.v0, before[1])
+    np.testing.assert_array_equal(model.u0, before[2])
+    for k, v in before[3].items():
+        np.testing.assert_array_equal(model.events[k], v)

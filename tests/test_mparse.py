@@ -215,3 +215,19 @@ def test_plugin_built_against_other_headers_is_refused():
     ctypes.memmove(ctypes.addressof(fake) + 8, ctypes.byref(ctypes.c_longlong(12345)), 8)
     assert lib.SimInf_hip_model_dims(ctypes.cast(fake, ctypes.c_void_p), None, None,
                                      None) == -9     # SIMINF_ERR_INVALID_MODEL
+
+
+def test_models_over_32_compartments_are_refused():
+    """S, G, E and the summary masks are 32-bit sets: a parsed model of 33
+    compartments is refused before any device code is generated."""
+    import siminf_amd as si
+    from siminf_amd import plugins
+    comps = ["C%d" % k for k in range(33)]
+    trans = ["C%d -> 0.1*C%d -> C%d" % (k, k, k + 1) for k in range(32)]
+    m = si.mparse(trans, comps, u0={c: [1] for c in comps}, tspan=[1.0, 2.0])
+    with pytest.raises(ValueError, match="at most 32 compartments"):
+        plugins.device_source(m.plugin.parsed, "wide")
+    # 32 compartments and 31 transitions are accepted.
+    m = si.mparse(trans[:31], comps[:32], u0={c: [1] for c in comps[:32]},
+                  tspan=[1.0, 2.0])
+    assert "NC = 32, NT = 31" in plugins.device_source(m.plugin.parsed, "wide")
