@@ -266,6 +266,31 @@ int SimInf_hip_plan_autotune(SimInf_hip_plan *plan, unsigned long int seed, int 
 
 void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
 
+/* Batches (no reference counterpart): the runs of several plans -- any
+ * models, solvers, node counts and replicate counts -- captured once in a
+ * graph whose branches run concurrently, then launched as one graph per
+ * SimInf_hip_batch_run.  Plans whose coupled kernel spans the grid (one
+ * replicate with external transfers) need a cooperative launch and are
+ * refused (SIMINF_ERR_INVALID_MODEL), as is a plan given twice.  A plan
+ * must outlive the batches that hold it, and runs of a plan inside and
+ * outside a batch must not overlap. */
+typedef struct SimInf_hip_batch SimInf_hip_batch;
+
+int SimInf_hip_batch_create(SimInf_hip_batch **batch, SimInf_hip_plan *const *plans,
+                            int n);
+
+/* One launch of the batch, plan i run with seeds[i]; blocks until it has
+ * finished.  Each plan's results (SimInf_hip_plan_stats, _download, the
+ * summaries) are those of a SimInf_hip_plan_run with that seed, bit for
+ * bit.  Returns the first plan's error code in batch order, or 0. */
+int SimInf_hip_batch_run(SimInf_hip_batch *batch, const unsigned long int *seeds);
+
+/* Device time of the last launch in milliseconds (HIP events around the
+ * graph launch). */
+int SimInf_hip_batch_time(const SimInf_hip_batch *batch, double *ms);
+
+void SimInf_hip_batch_destroy(SimInf_hip_batch *batch);
+
 /* Bootstrap particle filter with the state of every particle kept in HBM
  * between observation intervals (pfilter_internal, R/pfilter.R:315-499;
  * systematic resampling, SimInf_particle_filter.c:32-66).

@@ -6,7 +6,7 @@ from .model import SimInfModel, SimInfError, make_events, run, trajectory
 from .models import SIR, SEIR, SIS, SISe, SISe3, SIR_mparse
 from .modelparser import mparse
 from .particle_filter import pfilter, PFilterResult
-from .plan import Plan
+from .plan import Plan, Batch
 from .punchcard import punchcard
 from .abc_smc import abc, ABCResult
 from .summaries import SummaryError
@@ -14,4 +14,4 @@ from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
            "SIR", "SEIR", "SIS", "SISe", "SISe3", "SIR_mparse", "mparse",
-           "pfilter", "PFilterResult", "Plan", "SummaryError", "punchcard", "abc", "ABCResult"]
+           "pfilter", "PFilterResult", "Plan", "Batch", "SummaryError", "punchcard", "abc", "ABCResult"]

@@ -82,6 +82,10 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_download",
     "SimInf_hip_plan_stats",
     "SimInf_hip_plan_kernel_times",
+    "SimInf_hip_batch_create",
+    "SimInf_hip_batch_run",
+    "SimInf_hip_batch_time",
+    "SimInf_hip_batch_destroy",
     "SimInf_hip_pfilter",
     "SimInf_hip_plan_prevalence",
     "SimInf_hip_plan_trajectory",
@@ -167,6 +171,15 @@ def load(path=None):
     lib.SimInf_hip_plan_autotune.argtypes = [vp, ctypes.c_ulong, ctypes.c_int, ip, dp]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
+    lib.SimInf_hip_batch_create.restype = ctypes.c_int
+    lib.SimInf_hip_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                            ctypes.c_int]
+    lib.SimInf_hip_batch_run.restype = ctypes.c_int
+    lib.SimInf_hip_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_ulong)]
+    lib.SimInf_hip_batch_time.restype = ctypes.c_int
+    lib.SimInf_hip_batch_time.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    lib.SimInf_hip_batch_destroy.restype = None
+    lib.SimInf_hip_batch_destroy.argtypes = [vp]
     if path == LIB_PATH:
         _lib = lib
     return lib

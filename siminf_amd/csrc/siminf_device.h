@@ -70,6 +70,9 @@ struct KParams {
     int Nn, Nrep, Nld, tlen, len, Nshift;
     long long lanes;                 /* Nn * Nrep */
     unsigned long long seed;
+    /* When not null, the run's seed is read from here instead (a run
+     * captured in a graph takes a new seed on every replay). */
+    const unsigned long long *seed_dev;
     double tt0, next0;               /* tspan[0], floor(tspan[0]) + 1 */
 
     /* Model inputs. */
@@ -187,10 +190,15 @@ struct KParams {
      * through the whole run; or a classify pass over every lane, which
      * hands nodes frozen from the start to the fill kernel and lists the
      * others in active (count in *active_count), then an active pass
-     * over the listed lanes, packed 1 << lpw_shift to a wavefront. */
+     * over the listed lanes, packed 1 << lpw_shift to a wavefront
+     * (lpw_shift -1: the power of two that gives about two wavefronts per
+     * SIMD of the device's simds, at most 64 lanes).  The active pass
+     * reads the count on the device: its grid is fixed, and each
+     * wavefront takes the wave slots w, w + (wavefronts in the grid),
+     * ... below the slots the count needs. */
     int mode;
     int lpw_shift;
-    int n_active;
+    int simds;
     int *active;
     int *active_count;
 
@@ -1463,17 +1471,52 @@ __device__ __forceinline__ void record_frozen(const KParams &p, long long g,
 
 enum { NODES_ALL = 0, NODES_CLASSIFY = 1, NODES_ACTIVE = 2 };
 
-/* The lane (replicate * Nn + node) a thread of a node kernel runs, or -1. */
-__device__ __forceinline__ long long node_lane(const KParams &p)
+/* The seed of the run. */
+__device__ __forceinline__ unsigned long long run_seed(const KParams &p)
+{
+    return p.seed_dev ? *p.seed_dev : p.seed;
+}
+
+/* The wave slots a wavefront of a node kernel runs: in the active pass
+ * slot w holds the listed lanes w << shift .. (w + 1) << shift (at most
+ * n of them), and the wavefront takes w, w + stride, ... below waves;
+ * in the other modes one slot, the thread's own lane. */
+struct WaveSlots {
+    long long w, waves, stride;
+    int shift, n;
+};
+
+__device__ __forceinline__ WaveSlots wave_slots(const KParams &p)
+{
+    WaveSlots a{0, 1, 1, 0, 0};
+    if (p.mode != NODES_ACTIVE)
+        return a;
+    a.n = __hip_atomic_load(p.active_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int shift = p.lpw_shift;
+    if (shift < 0) {
+        shift = 0;
+        while (shift < 6 && ((long long)a.n >> shift) > 2LL * p.simds)
+            shift++;
+    }
+    a.shift = shift;
+    a.waves = ((long long)a.n + (1 << shift) - 1) >> shift;
+    a.w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    a.stride = ((long long)gridDim.x * blockDim.x) >> 6;
+    return a;
+}
+
+/* The lane (replicate * Nn + node) a thread of a node kernel runs in
+ * slot a.w, or -1. */
+__device__ __forceinline__ long long node_lane(const KParams &p, const WaveSlots &a)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p.mode != NODES_ACTIVE)
         return t < p.lanes ? t : -1;
     const int l = (int)(t & 63);
-    if (l >= (1 << p.lpw_shift))
+    if (l >= (1 << a.shift))
         return -1;
-    const long long i = ((t >> 6) << p.lpw_shift) + l;
-    return i < p.n_active ? (long long)p.active[i] : -1;
+    const long long i = (a.w << a.shift) + l;
+    return i < a.n ? (long long)p.active[i] : -1;
 }
 
 /* Append lane g to the active list (one atomic per wavefront). */

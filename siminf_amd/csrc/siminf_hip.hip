@@ -193,6 +193,7 @@ struct SimInf_hip_plan {
     int simds = 1024;                     /* SIMDs of the device (4 per CU) */
     hipEvent_t ev_start = nullptr, ev_mid = nullptr, ev_stop = nullptr;
     unsigned grid = 0, block = 0;
+    unsigned grid_active = 0;             /* node kernels' active pass */
     size_t lds = 0;
     bool has_e1 = false;
     bool canon = false;
@@ -837,6 +838,12 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
                 hipSuccess && cus > 0)
             P->simds = 4 * cus;
+        p.simds = P->simds;
+        /* The active pass's fixed grid: up to four wavefronts per SIMD
+         * (twice what the default packing needs), never more wavefronts
+         * than lanes. */
+        const long long waves = std::min<long long>(lanes, 4LL * P->simds);
+        P->grid_active = (unsigned)((waves * 64 + P->block - 1) / P->block);
     }
     *out = P;
     return 0;
@@ -849,56 +856,68 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     return plan_create_impl(out, a, model, false);
 }
 
-extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
+/* Clears a run's result words and counters on the device (a kernel, not
+ * memcpys or memsets, so that a run's launch sequence is kernels only and
+ * can be captured in a graph). */
+__global__ void k_run_init(unsigned long long *err_key, unsigned long long *transitions,
+                           unsigned long long *frozen_cols, int *active_count,
+                           unsigned *gsync)
 {
-    if (!P)
+    if (threadIdx.x == 0) {
+        *err_key = ~0ull;
+        *transitions = 0;
+        if (frozen_cols)
+            *frozen_cols = 0;
+        if (active_count)
+            *active_count = 0;
+    }
+    if (gsync && threadIdx.x < 2)
+        gsync[threadIdx.x] = 0u;
+}
+
+/* Enqueue one run of the plan on its streams: kernels and stream-event
+ * dependencies only, nothing that waits for the device, so the sequence
+ * can be captured in a graph (capture: no timing events, and a grid-wide
+ * coupled plan, whose kernel needs a cooperative launch, is refused).
+ * *filled tells whether the frozen-node fill kernel ran. */
+static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, bool *filled)
+{
+    KParams p = base;
+    hipStream_t st = P->stream;
+    *filled = false;
+    if (capture && P->kernel == K_COUPLED_GRID)
         return SIMINF_ERR_INVALID_MODEL;
-    P->has_run = false;
-    KParams p = P->p;
-    p.seed = (unsigned long long)seed;
-
-    const unsigned long long init_key = NO_ERROR_KEY, zero = 0;
-    if (hipMemcpyAsync(p.err_key, &init_key, sizeof(init_key),
-                       hipMemcpyHostToDevice, P->stream) != hipSuccess ||
-        hipMemcpyAsync(p.transitions, &zero, sizeof(zero),
-                       hipMemcpyHostToDevice, P->stream) != hipSuccess ||
-        (p.frozen_cols && hipMemcpyAsync(p.frozen_cols, &zero, sizeof(zero),
-                                         hipMemcpyHostToDevice, P->stream) != hipSuccess))
-        return SIMINF_ERR_DEVICE;
-
-    (void)hipEventRecord(P->ev_start, P->stream);
-    bool filled = false;
+    if (!capture)
+        (void)hipEventRecord(P->ev_start, st);
+    hipLaunchKernelGGL(k_run_init, dim3(1), dim3(64), 0, st, p.err_key, p.transitions,
+                       p.frozen_cols, p.active_count, p.gsync);
     switch (P->kernel) {
     case K_SSM_NODES:
     case K_AEM_NODES: {
         auto launch = [&](const KParams &q, unsigned grid) {
             if (P->kernel == K_SSM_NODES)
-                P->model->ssm_nodes(q, P->has_e1, P->canon, grid, P->block, P->stream);
+                P->model->ssm_nodes(q, P->has_e1, P->canon, grid, P->block, st);
             else
-                P->model->aem_nodes(q, P->has_e1, P->canon, grid, P->block, P->stream);
+                P->model->aem_nodes(q, P->has_e1, P->canon, grid, P->block, st);
         };
         if (!p.frozen_from) {
             p.mode = NODES_ALL;
             launch(p, P->grid);
-            (void)hipEventRecord(P->ev_mid, P->stream);
+            if (!capture)
+                (void)hipEventRecord(P->ev_mid, st);
             break;
         }
         /* Classify every lane; nodes frozen from the start go to the fill
          * kernel on the second stream while the active pass runs the
-         * others, packed so that they spread over the SIMDs. */
-        const int zero_i = 0;
-        if (hipMemcpyAsync(p.active_count, &zero_i, sizeof(int), hipMemcpyHostToDevice,
-                           P->stream) != hipSuccess)
-            return SIMINF_ERR_DEVICE;
+         * others, packed so that they spread over the SIMDs (the pass
+         * reads how many there are on the device). */
         p.mode = NODES_CLASSIFY;
         launch(p, P->grid);
-        int n_active = 0;
-        if (hipMemcpyAsync(&n_active, p.active_count, sizeof(int), hipMemcpyDeviceToHost,
-                           P->stream) != hipSuccess ||
-            hipEventRecord(P->ev_classified, P->stream) != hipSuccess ||
+        if (hipEventRecord(P->ev_classified, st) != hipSuccess ||
             hipStreamWaitEvent(P->fill_stream, P->ev_classified, 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
-        (void)hipEventRecord(P->ev_fill0, P->fill_stream);
+        if (!capture)
+            (void)hipEventRecord(P->ev_fill0, P->fill_stream);
         {
             const dim3 fg(P->grid), fb(P->block);
             hipStream_t fs = P->fill_stream;
@@ -921,55 +940,51 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
             }
 #undef SIMINF_FILL
         }
-        (void)hipEventRecord(P->ev_fill1, P->fill_stream);
-        if (hipStreamSynchronize(P->stream) != hipSuccess)
+        if (hipEventRecord(P->ev_fill1, P->fill_stream) != hipSuccess)
             return SIMINF_ERR_DEVICE;
-        if (n_active > 0) {
-            /* Lanes per wavefront: the power of two that gives about two
-             * wavefronts per SIMD, at most 64. */
-            int shift = 0;
-            while (shift < 6 && ((long long)n_active >> shift) > 2LL * P->simds)
-                shift++;
-            if (P->lpw_fixed >= 0)
-                shift = P->lpw_fixed;
-            p.mode = NODES_ACTIVE;
-            p.n_active = n_active;
-            p.lpw_shift = shift;
-            const long long waves = ((long long)n_active + (1 << shift) - 1) >> shift;
-            const unsigned grid = (unsigned)((waves * 64 + P->block - 1) / P->block);
-            (void)hipEventRecord(P->ev_mid, P->stream);
-            launch(p, grid);
-        } else {
-            (void)hipEventRecord(P->ev_mid, P->stream);
-        }
-        (void)hipEventRecord(P->ev_active1, P->stream);
-        if (hipStreamWaitEvent(P->stream, P->ev_fill1, 0) != hipSuccess)
+        p.mode = NODES_ACTIVE;
+        p.lpw_shift = P->lpw_fixed;
+        if (!capture)
+            (void)hipEventRecord(P->ev_mid, st);
+        launch(p, P->grid_active);
+        if (!capture)
+            (void)hipEventRecord(P->ev_active1, st);
+        if (hipStreamWaitEvent(st, P->ev_fill1, 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
-        filled = true;
+        *filled = true;
         break;
     }
     case K_COUPLED_GRID:
         /* One cooperative launch per replicate. */
         for (int r = 0; r < P->Nrep; r++) {
             p.rep0 = r;
-            if (P->model->coupled(p, P->canon, true, P->grid, P->block, P->lds,
-                                  P->stream, nullptr) != hipSuccess)
+            if (P->model->coupled(p, P->canon, true, P->grid, P->block, P->lds, st,
+                                  nullptr) != hipSuccess)
                 return SIMINF_ERR_DEVICE;
         }
         break;
     case K_COUPLED_REPS:
-        if (P->model->coupled(p, P->canon, false, P->grid, P->block, P->lds,
-                              P->stream, nullptr) != hipSuccess)
+        if (P->model->coupled(p, P->canon, false, P->grid, P->block, P->lds, st,
+                              nullptr) != hipSuccess)
             return SIMINF_ERR_DEVICE;
         break;
     }
-    if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES)
-        (void)hipEventRecord(P->ev_mid, P->stream);
-    (void)hipEventRecord(P->ev_stop, P->stream);
+    if (!capture) {
+        if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES)
+            (void)hipEventRecord(P->ev_mid, st);
+        (void)hipEventRecord(P->ev_stop, st);
+    }
     if (hipGetLastError() != hipSuccess)
         return SIMINF_ERR_DEVICE;
+    return 0;
+}
 
+/* Read a finished run's error key and counts (and, when timed, its
+ * kernel times); returns the run's error code. */
+static int plan_collect(SimInf_hip_plan *P, bool timed, bool filled)
+{
     unsigned long long key = NO_ERROR_KEY, trans = 0, fill_cols = 0;
+    const KParams &p = P->p;
     if (hipMemcpyAsync(&key, p.err_key, sizeof(key), hipMemcpyDeviceToHost,
                        P->stream) != hipSuccess ||
         hipMemcpyAsync(&trans, p.transitions, sizeof(trans),
@@ -978,27 +993,44 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
                                          hipMemcpyDeviceToHost, P->stream) != hipSuccess) ||
         hipStreamSynchronize(P->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
-
-    /* Device times: the whole run; the solver (classify and active
-     * passes, or the one kernel); the fill kernel on its own stream. */
-    float ms = 0.0f, ms_solver = 0.0f, ms_active = 0.0f, ms_fill = 0.0f;
-    (void)hipEventElapsedTime(&ms, P->ev_start, P->ev_stop);
-    if (filled) {
-        (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_classified);
-        (void)hipEventElapsedTime(&ms_active, P->ev_mid, P->ev_active1);
-        ms_solver += ms_active;
-        (void)hipEventElapsedTime(&ms_fill, P->ev_fill0, P->ev_fill1);
-    } else {
-        (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_mid);
+    if (timed) {
+        /* Device times: the whole run; the solver (classify and active
+         * passes, or the one kernel); the fill kernel on its own
+         * stream. */
+        float ms = 0.0f, ms_solver = 0.0f, ms_active = 0.0f, ms_fill = 0.0f;
+        (void)hipEventElapsedTime(&ms, P->ev_start, P->ev_stop);
+        if (filled) {
+            (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_classified);
+            (void)hipEventElapsedTime(&ms_active, P->ev_mid, P->ev_active1);
+            ms_solver += ms_active;
+            (void)hipEventElapsedTime(&ms_fill, P->ev_fill0, P->ev_fill1);
+        } else {
+            (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_mid);
+        }
+        P->kernel_ms = ms;
+        P->solver_ms = ms_solver;
+        P->fill_ms = ms_fill;
     }
-    P->kernel_ms = ms;
-    P->solver_ms = ms_solver;
-    P->fill_ms = ms_fill;
     P->fill_cols = fill_cols;
     P->transitions = trans;
     const int code = key_to_code(std::min<uint64_t>(key, P->host_err_key));
     P->has_run = code == 0;
     return code;
+}
+
+extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    P->has_run = false;
+    KParams p = P->p;
+    p.seed = (unsigned long long)seed;
+    p.seed_dev = nullptr;
+    bool filled = false;
+    const int err = plan_enqueue(P, p, false, &filled);
+    if (err)
+        return err;
+    return plan_collect(P, true, filled);
 }
 
 extern "C" int SimInf_hip_plan_download(SimInf_hip_plan *P, int *U, double *V,
@@ -1248,6 +1280,155 @@ extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
     if (P->stream)
         (void)hipStreamDestroy(P->stream);
     delete P;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batches: several plans in one graph launch.                         */
+/* ------------------------------------------------------------------ */
+
+struct SimInf_hip_batch {
+    std::vector<SimInf_hip_plan *> plans;
+    std::vector<char> filled;
+    std::vector<hipEvent_t> fork, join;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long *seeds = nullptr;   /* device, one per plan */
+    double ms = 0.0;
+};
+
+extern "C" void SimInf_hip_batch_destroy(SimInf_hip_batch *B)
+{
+    if (!B)
+        return;
+    if (B->stream)
+        (void)hipStreamSynchronize(B->stream);
+    if (B->exec)
+        (void)hipGraphExecDestroy(B->exec);
+    if (B->graph)
+        (void)hipGraphDestroy(B->graph);
+    for (hipEvent_t e : B->fork)
+        if (e)
+            (void)hipEventDestroy(e);
+    for (hipEvent_t e : B->join)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (B->ev0)
+        (void)hipEventDestroy(B->ev0);
+    if (B->ev1)
+        (void)hipEventDestroy(B->ev1);
+    if (B->seeds)
+        (void)hipFree(B->seeds);
+    if (B->stream)
+        (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+extern "C" int SimInf_hip_batch_create(SimInf_hip_batch **out, SimInf_hip_plan *const *plans,
+                                       int n)
+{
+    if (!out || !plans || n < 1)
+        return SIMINF_ERR_INVALID_MODEL;
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!plans[i] || plans[i]->kernel == K_COUPLED_GRID)
+            return SIMINF_ERR_INVALID_MODEL;
+        for (int j = 0; j < i; j++)
+            if (plans[j] == plans[i])
+                return SIMINF_ERR_INVALID_MODEL;
+    }
+    SimInf_hip_batch *B = new SimInf_hip_batch();
+    B->plans.assign(plans, plans + n);
+    B->filled.assign(n, 0);
+    B->fork.assign(n, nullptr);
+    B->join.assign(n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&B->ev0) == hipSuccess && hipEventCreate(&B->ev1) == hipSuccess &&
+              hipMalloc((void **)&B->seeds, sizeof(unsigned long long) * n) == hipSuccess;
+    for (int i = 0; ok && i < n; i++)
+        ok = hipEventCreateWithFlags(&B->fork[i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        SimInf_hip_batch_destroy(B);
+        return SIMINF_ERR_DEVICE;
+    }
+    /* Every plan's run on its own branch: fork from the batch's stream
+     * into the plan's streams, the plan's kernels with the seed read from
+     * the batch's seed table, join back. */
+    int err = 0;
+    if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        SimInf_hip_batch_destroy(B);
+        return SIMINF_ERR_DEVICE;
+    }
+    for (int i = 0; i < n && !err; i++) {
+        SimInf_hip_plan *P = B->plans[i];
+        KParams p = P->p;
+        p.seed_dev = B->seeds + i;
+        bool filled = false;
+        if (hipEventRecord(B->fork[i], B->stream) != hipSuccess ||
+            hipStreamWaitEvent(P->stream, B->fork[i], 0) != hipSuccess) {
+            err = SIMINF_ERR_DEVICE;
+            break;
+        }
+        err = plan_enqueue(P, p, true, &filled);
+        B->filled[i] = filled;
+        if (!err && (hipEventRecord(B->join[i], P->stream) != hipSuccess ||
+                     hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess))
+            err = SIMINF_ERR_DEVICE;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(B->stream, &g);
+    B->graph = g;
+    if (!err && (ec != hipSuccess || !g ||
+                 hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0) != hipSuccess))
+        err = SIMINF_ERR_DEVICE;
+    (void)hipGetLastError();
+    if (err) {
+        SimInf_hip_batch_destroy(B);
+        return err;
+    }
+    *out = B;
+    return 0;
+}
+
+extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *seeds)
+{
+    if (!B || !seeds)
+        return SIMINF_ERR_INVALID_MODEL;
+    const int n = (int)B->plans.size();
+    std::vector<unsigned long long> sd(seeds, seeds + n);
+    for (SimInf_hip_plan *P : B->plans)
+        P->has_run = false;
+    if (hipMemcpyAsync(B->seeds, sd.data(), sizeof(sd[0]) * n, hipMemcpyHostToDevice,
+                       B->stream) != hipSuccess ||
+        hipEventRecord(B->ev0, B->stream) != hipSuccess ||
+        hipGraphLaunch(B->exec, B->stream) != hipSuccess ||
+        hipEventRecord(B->ev1, B->stream) != hipSuccess ||
+        hipStreamSynchronize(B->stream) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, B->ev0, B->ev1);
+    B->ms = ms;
+    /* Each plan's results; its kernel times are the launch's. */
+    int first = 0;
+    for (int i = 0; i < n; i++) {
+        SimInf_hip_plan *P = B->plans[i];
+        const int code = plan_collect(P, false, B->filled[i] != 0);
+        P->kernel_ms = P->solver_ms = ms;
+        P->fill_ms = 0.0;
+        if (code && !first)
+            first = code;
+    }
+    return first;
+}
+
+extern "C" int SimInf_hip_batch_time(const SimInf_hip_batch *B, double *ms)
+{
+    if (!B || !ms)
+        return SIMINF_ERR_INVALID_MODEL;
+    *ms = B->ms;
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -82,11 +82,12 @@ __device__ __forceinline__ int e1_day(const KParams &p, const EvTab &tab,
     return 0;
 }
 
+/* One lane's run in a node kernel (g: replicate * Nn + node, or -1 for a
+ * lane without a node, which still helps its wavefront's SSA). */
 template <class M, bool EVENTS, bool CANON>
-__global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
+__device__ __forceinline__ void ssm_lane(const KParams &p, long long g)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const long long g = node_lane(p);
     const bool live = g >= 0;
     unsigned long long fired = 0, frozen_cols = 0;
     bool push = false;
@@ -227,11 +228,20 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
     add_transitions(p, fired);
 }
 
+/* ssm / mssm over nodes that never interact: each wavefront runs its
+ * wave slots (one, its own lanes, except in the active pass). */
 template <class M, bool EVENTS, bool CANON>
-__global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
+__global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
+{
+    p.seed = run_seed(p);
+    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
+        ssm_lane<M, EVENTS, CANON>(p, node_lane(p, a));
+}
+
+template <class M, bool EVENTS, bool CANON>
+__device__ __forceinline__ void aem_lane(const KParams &p, long long g)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const long long g = node_lane(p);
     unsigned long long fired = 0, frozen_cols = 0;
     bool push = false;
 
@@ -374,6 +384,15 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
         add_count(p.frozen_cols, frozen_cols);
     }
     add_transitions(p, fired);
+}
+
+/* aem over nodes that never interact, by wave slots as k_ssm_nodes. */
+template <class M, bool EVENTS, bool CANON>
+__global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
+{
+    p.seed = run_seed(p);
+    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
+        aem_lane<M, EVENTS, CANON>(p, node_lane(p, a));
 }
 
 /* ------------------------------------------------------------------ */
@@ -655,6 +674,7 @@ template <class M, bool CANON, bool GRID, int BLOCK, bool AEM = false>
 __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    p.seed = run_seed(p);
     extern __shared__ int4 lds4[];
     unsigned char *lds = (unsigned char *)lds4;
     const int rep = GRID ? p.rep0 : (int)blockIdx.x;

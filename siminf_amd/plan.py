@@ -175,4 +175,50 @@ class Plan:
         return out
 
 
-__all__ = ["Plan", "SimInfError"]
+class Batch:
+    """Several plans -- any models, solvers and sizes -- run as one graph
+    launch (SimInf_hip_batch_*; no reference counterpart).  Each plan's
+    results after run() are those of its own Plan.run with the same seed.
+    The plans must outlive the batch."""
+
+    def __init__(self, plans, lib=None):
+        self.lib = lib or _native.load()
+        self.plans = list(plans)
+        handles = (ctypes.c_void_p * len(self.plans))(*[p._handle for p in self.plans])
+        self._handle = ctypes.c_void_p()
+        err = self.lib.SimInf_hip_batch_create(ctypes.byref(self._handle), handles,
+                                               len(self.plans))
+        if err:
+            _raise(self.lib, err)
+
+    def run(self, seeds):
+        """One launch, plan i with seeds[i]; returns the launch's device
+        time in milliseconds."""
+        sd = (ctypes.c_ulong * len(self.plans))(
+            *[int(x) & 0xFFFFFFFFFFFFFFFF for x in seeds])
+        err = self.lib.SimInf_hip_batch_run(self._handle, sd)
+        if err:
+            _raise(self.lib, err)
+        ms = ctypes.c_double()
+        self.lib.SimInf_hip_batch_time(self._handle, ctypes.byref(ms))
+        return ms.value
+
+    def close(self):
+        if self._handle:
+            self.lib.SimInf_hip_batch_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["Plan", "Batch", "SimInfError"]
