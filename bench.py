@@ -1,19 +1,23 @@
 """Benchmark of the MI355X solver core.
 
-Default workload: SISe3 on 1600 herds over 1460 days with scheduled enter,
-exit, internal- and external-transfer events (the second benchmark
-configuration).  A step is one complete simulation of that model with all
-inputs and the trajectory resident in HBM (SimInf_hip_plan_run); the seed
-changes every step.
+Default workload: SEIR on 1,000,000 nodes (sizes ~ Poisson(500), 0.5 % of
+nodes seeded with 5 infected) over 365 days, run under both the ssm and
+the aem solver in one batched launch: the two plans' kernels (classify,
+frozen-node fill, active pass of each) are captured once in a HIP graph
+and every step is one graph launch (SimInf_hip_batch_run) with new seeds,
+inputs and trajectories resident in HBM.  Before the timed steps each
+plan's autotuner picks the active pass's lanes per wavefront (results do
+not depend on it).
 
-Before the timed steps the plan's autotuner picks the node kernels' lanes
-per wavefront (results do not depend on it; the coupled kernel of the
-default workload has no such choice).
-
-Prints one JSON line: node-days/s (value), transitions/s, p50/p99 step
-latency, the roofline of the solver kernel (algorithmic bytes over the
-kernel time measured with HIP events on the plan's stream) and a CPU
-baseline (the C checker, one core, on the same workload).
+Prints one JSON line: node-days/s of both solvers together (value),
+transitions/s, p50/p99 step latency, the roofline of the kernel that moves
+the step's bytes (the frozen-node fill; its time from event-record nodes
+inside the graph, on the stream it runs on) and of the SSA active passes
+beside it, the PCIe-inclusive time of the blocking entry point, a CPU
+baseline (the C checker with the reference's OpenMP node-block partition,
+on all host cores and on one), and, as extra keys, the SISe3 workload with
+scheduled events (1600 herds, 1460 days) and a particle-filter request of
+2000 particles with binomial observations.
 
 --workload pfilter_stream measures the particle-filter serving loop
 instead: a stream of filter requests (SIR on log-uniform node counts,
@@ -21,6 +25,7 @@ instead: a stream of filter requests (SIR on log-uniform node counts,
 siminf_amd.serving.PFilterServer, reported as particle-simulations/s
 with p50/p99 request latency, beside the plain path (one run() per
 interval, resampling on the host) on a sample of the same distribution.
+Other --workload names time one plan of one solver.
 """
 import argparse
 import ctypes
@@ -86,26 +91,240 @@ def make_workload(name):
     raise SystemExit("unknown workload %s" % name)
 
 
-def cpu_baseline(model, solver, budget_s=20.0):
-    """The C checker on one host core: full runs of the same workload,
-    as many as fit in about budget_s seconds (at least one)."""
+def host_cores():
+    """Cores this process may use (its CPU affinity), capped by
+    OMP_NUM_THREADS when set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def cpu_baseline(model, solvers, threads, budget_s=20.0, slice_nodes=None):
+    """The C checker (the reference's loops; node loops over `threads`
+    OpenMP threads in contiguous node blocks, the reference's partition)
+    on the same workload, or on its first slice_nodes nodes: runs of each
+    solver, repeated while they fit in about budget_s seconds.  Times
+    cover the solver call only."""
     from oracle import bridge
-    times = []
+    m = model if slice_nodes is None else _node_slice(model, slice_nodes)
+    node_days = m.Nn * m.replicates * _ndays(m)
+    secs, trans, runs = 0.0, 0, 0
     t_start = time.perf_counter()
     while True:
-        t0 = time.perf_counter()
-        r = bridge.run(model, solver=solver, seed=1000 + len(times))
-        times.append(time.perf_counter() - t0)
-        if r.err:
-            raise SystemExit("checker failed with %d" % r.err)
-        if time.perf_counter() - t_start > budget_s or len(times) >= 5:
+        for solver in solvers:
+            r = bridge.run(m, solver=solver, seed=1000 + runs, threads=threads)
+            if r.err:
+                raise SystemExit("checker failed with %d" % r.err)
+            secs += r.seconds
+            trans += r.transitions
+            runs += 1
+        if time.perf_counter() - t_start > budget_s or runs >= 4 * len(solvers):
             break
-    t = statistics.median(times)
-    node_days = model.Nn * model.replicates * _ndays(model)
-    return {"value": node_days / t, "unit": "node-days/s", "cores": 1,
+    what = ("the full workload" if slice_nodes is None
+            else "the first %d nodes of the workload" % m.Nn)
+    return {"value": node_days * runs / secs, "unit": "node-days/s", "cores": threads,
             "kind": "port",
-            "sample": "%d full run(s) of the same workload, median %.3f s, "
-                      "%d transitions in the last" % (len(times), t, r.transitions)}
+            "transitions_per_s": trans / secs,
+            "sample": "%d run(s) (%s) of %s, %.2f s of solver time" % (
+                runs, "+".join(solvers), what, secs)}
+
+
+def _node_slice(model, n):
+    """The first n nodes of a model without events (same parameters)."""
+    from siminf_amd.model import SimInfModel
+    return SimInfModel(model.plugin, model.G, model.S, model.tspan,
+                       model.u0[:, :n].copy(), ldata=model.ldata[:, :n].copy(),
+                       gdata=model.gdata, v0=model.v0[:, :n].copy() if model.Nd else None,
+                       E=model.E, compartments=model.compartments)
+
+
+def _plan_times(plan):
+    lib = plan.lib
+    sm, fm, fc = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+    lib.SimInf_hip_plan_kernel_times(plan._handle, ctypes.byref(sm), ctypes.byref(fm),
+                                     ctypes.byref(fc))
+    tr, nd, ms = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+    lib.SimInf_hip_plan_stats(plan._handle, ctypes.byref(tr), ctypes.byref(nd),
+                              ctypes.byref(ms))
+    return {"solver_ms": sm.value, "fill_ms": fm.value, "fill_cols": fc.value,
+            "transitions": tr.value, "node_days": nd.value, "run_ms": ms.value}
+
+
+def bench_batch(args):
+    """SEIR 1e6 nodes x 365 days, ssm and aem in one graph launch per
+    step."""
+    import siminf_amd as si
+    from siminf_amd import workloads as W
+    model = W.seir_million()
+    solvers = ("ssm", "aem")
+    plans = [si.Plan(model, solver=s) for s in solvers]
+    tuned = {}
+    for s, p in zip(solvers, plans):
+        for i in range(args.warmup):
+            p.run(100 + i)
+        if not args.no_autotune:
+            best, _ = p.autotune(seed=200, reps=2)
+            tuned[s] = {"lanes_per_wave_shift": best}
+    batch = si.Batch(plans)
+    for i in range(args.warmup):
+        batch.run([300 + 2 * i, 301 + 2 * i])
+    lat, launch_ms, per = [], [], {s: [] for s in solvers}
+    node_days = trans = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        launch_ms.append(batch.run([1 + 2 * i, 2 + 2 * i]))
+        lat.append(time.perf_counter() - s0)
+        for s, p in zip(solvers, plans):
+            t = _plan_times(p)
+            per[s].append(t)
+            node_days += t["node_days"]
+            trans += t["transitions"]
+    wall = time.perf_counter() - t0
+
+    # The kernel that moves the step's bytes: each plan's frozen-node fill
+    # (Nc ints per node-column it writes), timed by the event-record nodes
+    # around it inside the graph.  The active passes (latency-bound SSA
+    # chains) are reported beside it with the bytes the plan writes for
+    # its active nodes.
+    per_col = 4 * model.Nc + 8 * model.Nd
+    fill_ms = [t["fill_ms"] for s in solvers for t in per[s]]
+    fill_bytes = [t["fill_cols"] * per_col for s in solvers for t in per[s]]
+    fill_gbs = statistics.mean(fill_bytes) / (statistics.mean(fill_ms) / 1e3) / 1e9
+    total_bytes = algorithmic_bytes(model)
+    solver_rows = {}
+    for s in solvers:
+        sm = statistics.mean(t["solver_ms"] for t in per[s])
+        fb = statistics.mean(t["fill_cols"] for t in per[s]) * per_col
+        solver_rows[s] = {"ms": sm, "fill_ms": statistics.mean(t["fill_ms"] for t in per[s]),
+                          "transitions_per_run": statistics.mean(
+                              t["transitions"] for t in per[s]),
+                          "achieved_gbs": (total_bytes - fb) / (sm / 1e3) / 1e9}
+    traffic = pmc_traffic("seir_batch", "k_fill_frozen")
+    out = {
+        "metric": "node-days/sec (SSA transitions/sec and p50/p99 simulation latency alongside)",
+        "value": node_days / wall,
+        "unit": "node-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 state, f64 rates/time",
+        "data": "synthetic (seeded generator of the configuration's shape)",
+        "config": {"workload": "SEIR 1e6 nodes x 365 days, ssm+aem in one batched "
+                               "graph launch",
+                   "solvers": list(solvers), "nodes": model.Nn, "days": _ndays(model),
+                   "seeded_nodes": int((model.u0[2] > 0).sum())},
+        "transitions_per_s": trans / wall,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "launch_ms_mean": statistics.mean(launch_ms),
+        "autotuned": tuned,
+        "roofline": {"bound": "hbm", "achieved": fill_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": fill_gbs / HBM_PEAK_GBS,
+                     "traffic": traffic, "kernel": "k_fill_frozen<4,false>",
+                     "algorithmic_bytes_per_launch": statistics.mean(fill_bytes),
+                     "kernel_ms_mean": statistics.mean(fill_ms),
+                     "traffic_source": "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE + "
+                                       "WRITE_SIZE of the fill kernels in this launch, "
+                                       "scaled by a copy kernel)" if traffic else None},
+        "active_passes": solver_rows,
+    }
+    batch.close()
+    for p in plans:
+        p.close()
+    if not args.no_extras:
+        for key, fn in (("pcie_inclusive", lambda: pcie_inclusive(model)),
+                        ("sise3_herds", lambda: bench_sise3(args)),
+                        ("pfilter_request", lambda: bench_pfilter_request(args))):
+            try:
+                out[key] = fn()
+            except Exception as e:          # reported, not fatal to the line
+                out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            print("bench: %s done" % key, file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        T = host_cores()
+        out["cpu_baseline"] = cpu_baseline(model, solvers, T)
+        out["cpu_baseline_1core"] = cpu_baseline(model, solvers, 1, budget_s=10.0,
+                                                 slice_nodes=100_000)
+    return out
+
+
+def pcie_inclusive(model):
+    """The blocking entry point (SimInf_hip_run: plan, upload, run and the
+    download of U into host memory), timed once per solver."""
+    from siminf_amd import _native
+    lib = _native.load()
+    dev = lib.SimInf_hip_model_builtin(model.plugin.encode())
+    out = {}
+    for s in ("ssm", "aem"):
+        b = _native.ArgsBundle(model, solver=0 if s == "ssm" else 1, seed=7)
+        t0 = time.perf_counter()
+        err = lib.SimInf_hip_run(ctypes.byref(b.args), ctypes.c_void_p(dev))
+        dt = time.perf_counter() - t0
+        if err:
+            raise SystemExit("SimInf_hip_run failed: %d" % err)
+        out[s] = {"seconds": dt, "node_days_per_s": model.Nn * _ndays(model) / dt,
+                  "u_bytes_copied": int(b.U.nbytes)}
+        del b
+    return out
+
+
+def bench_sise3(args, steps=10):
+    """SISe3, 1600 herds x 1460 days with scheduled events (one replicate,
+    grid-wide coupled kernel): mean of `steps` resident runs after the
+    autotuner."""
+    import siminf_amd as si
+    from siminf_amd import workloads as W
+    model = W.sise3_herds()
+    with si.Plan(model) as p:
+        p.run(100)
+        if not args.no_autotune:
+            p.autotune(seed=200, reps=2)
+        solo, pack = p.herd_map()
+        nd = tr = 0
+        ms = []
+        t0 = time.perf_counter()
+        for i in range(steps):
+            a, b, c = p.run(1 + i)
+            tr += a
+            nd += b
+            ms.append(c)
+        wall = time.perf_counter() - t0
+    return {"workload": "SISe3 1600 herds x 1460 days, %d scheduled events"
+                        % len(model.events["event"]),
+            "node_days_per_s": nd / wall, "transitions_per_s": tr / wall,
+            "ms_per_run": wall / steps * 1e3, "kernel_ms_mean": statistics.mean(ms),
+            "herd_map": {"solo_herds": solo, "herds_per_wavefront": pack},
+            "algorithmic_bytes_per_run": algorithmic_bytes(model)}
+
+
+def bench_pfilter_request(args):
+    """One particle-filter request: SIR on 10,000 nodes, 2000 particles,
+    30 weekly observations of the infected total, binomial observation
+    process with p = 0.5 (observations thinned from one simulated
+    trajectory of the same model)."""
+    import siminf_amd as si
+    from siminf_amd import workloads as W
+    spec = W.pfilter_binomial_request(Nn=10_000, n_particles=2000, seed=23)
+    t0 = time.perf_counter()
+    res = si.pfilter(spec.model, 1, spec.times, spec.obs, spec.n_particles,
+                     obs_prob=0.5, seed=spec.seed)
+    dt = time.perf_counter() - t0
+    return {"workload": "pfilter request: SIR 10000 nodes, 2000 particles, 30 weekly "
+                        "intervals, binomial p=0.5",
+            "seconds": dt, "particle_sims_per_s": spec.n_particles / dt,
+            "particle_node_days_per_s": spec.n_particles * spec.model.Nn *
+            (spec.times[-1] - spec.model.tspan[0]) / dt,
+            "loglik": res.loglik, "min_ess": float(np.min(res.ess))}
 
 
 def _ndays(model):
@@ -232,8 +451,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sise3_herds")
+    ap.add_argument("--workload", default="seir_batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="seir_batch: skip the PCIe-inclusive, SISe3 and filter-request "
+                         "measurements")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default lanes-per-wavefront rule of the node kernels")
     ap.add_argument("--requests", type=int, default=24,
@@ -265,6 +487,9 @@ def main():
         sys.exit(2)
     if args.workload == "pfilter_stream":
         bench_pfilter_stream(args)
+        return
+    if args.workload == "seir_batch":
+        print(json.dumps(bench_batch(args)), flush=True)
         return
 
     from siminf_amd import _native
@@ -369,7 +594,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(model, solver)
+        out["cpu_baseline"] = cpu_baseline(model, (solver,), host_cores())
     print(json.dumps(out), flush=True)
 
 

@@ -2,6 +2,7 @@
 by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg."""
 import ctypes
 import os
+import time
 
 import numpy as np
 
@@ -60,6 +61,8 @@ def load():
         lib.oracle_abc_weights.restype = ctypes.c_int
         lib.oracle_abc_weights.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int,
                                            ctypes.c_int, dp, dp, dp, dp, dp]
+        lib.oracle_set_threads.restype = None
+        lib.oracle_set_threads.argtypes = [ctypes.c_int]
         lib.oracle_resample.restype = ctypes.c_int
         lib.oracle_sample.restype = ctypes.c_int
         lib.oracle_sample.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ip,
@@ -96,14 +99,24 @@ def _set_plugin(lib, model, b):
         b.args.pts_fun = ctypes.c_void_p.in_dll(h, "siminf_pts_fun")
 
 
-def run(model, solver="ssm", seed=0, crn=False):
-    """Run the checker on the same SimInf_solver_args the GPU path gets."""
+def run(model, solver="ssm", seed=0, crn=False, threads=1):
+    """Run the checker on the same SimInf_solver_args the GPU path gets;
+    threads > 1 runs the node loops over that many OpenMP threads (node
+    blocks, the reference's partition; the same results)."""
     lib = load()
     b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
                            crn=1 if crn else 0, seed=int(seed))
     _set_plugin(lib, model, b)
-    err = lib.oracle_run(ctypes.byref(b.args))
-    return OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
+    lib.oracle_set_threads(int(threads))
+    try:
+        t0 = time.perf_counter()
+        err = lib.oracle_run(ctypes.byref(b.args))
+        seconds = time.perf_counter() - t0
+    finally:
+        lib.oracle_set_threads(1)
+    r = OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
+    r.seconds = seconds      # the solver call alone
+    return r
 
 
 def pfilter(model, obs_compartment, times, obs, n_particles, obs_prob=0.5, seed=0,

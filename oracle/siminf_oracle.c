@@ -31,6 +31,21 @@
  * the post-time-step function, then writeback; for sorted events this is
  * the reference's order (SimInf_solver_ssm.c:144-362).
  *
+ * Threads (oracle_set_threads): the node loops of the SSA / aem phase and
+ * of the post-time-step phase, and the trajectory copy, run over T
+ * contiguous node blocks, one per OpenMP thread, the static partition of
+ * SimInf_solver.c:1094-1110; events run on one thread, as the
+ * reference's E2 (SimInf_solver_ssm.c:244-250).  Every node keeps its own
+ * stream, so results do not depend on T (with several errors in one day
+ * the reported one may, as in the reference).  T = 1 is the checker.
+ *
+ * The stream schedule is this library's, not the reference's: the SSA
+ * takes one aligned pair of uniforms per attempt (the reference draws the
+ * second only when a transition fires, SimInf_solver_ssm.c:155, :165), E2
+ * events draw from their own streams (the reference continues thread 0's
+ * stream, SimInf_solver_ssm.c:249), so agreement with the reference is in
+ * distribution, and bit-exact agreement is with the GPU path.
+ *
  * Parity status: the loops are pinned by the reference's RNG-free golden
  * vectors (tests/SEIR.R, tests/phi.R, tests/sample_select.R), invariants
  * and closed-form distributions in tests/; the GSL samplers it replaces are
@@ -46,8 +61,18 @@
 #include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static long long oracle_transitions_count;
+static int oracle_threads = 1;
+
+/* Threads of the node loops (1: one thread, the checker). */
+void oracle_set_threads(int n)
+{
+    oracle_threads = n < 1 ? 1 : n;
+}
 
 /* ---------------------------------------------------------------- */
 /* Sampling (SimInf_solver.c:62-324).                                */
@@ -416,8 +441,18 @@ static void oracle_store(
 
     while (*U_it < a->tlen && tt > a->tspan[*U_it]) {
         if (a->U) {
-            memcpy(&a->U[(ptrdiff_t)(rep * tlen + *U_it) * Nn * Nc], u,
-                   (size_t)(Nn * Nc) * sizeof(int));
+            int *dst = &a->U[(ptrdiff_t)(rep * tlen + *U_it) * Nn * Nc];
+            if (oracle_threads > 1) {
+                /* Each thread copies its node block. */
+                const int T = oracle_threads;
+#pragma omp parallel for schedule(static) num_threads(T)
+                for (int b = 0; b < T; b++) {
+                    const ptrdiff_t n0 = Nn * b / T, n1 = Nn * (b + 1) / T;
+                    memcpy(dst + n0 * Nc, u + n0 * Nc, (size_t)((n1 - n0) * Nc) * sizeof(int));
+                }
+            } else {
+                memcpy(dst, u, (size_t)(Nn * Nc) * sizeof(int));
+            }
         } else {
             const ptrdiff_t col = rep * tlen + *U_it;
             for (int j = a->jcU[col]; j < a->jcU[col + 1]; j++)
@@ -449,6 +484,114 @@ static int oracle_rate_ok(double rate)
 /* Direct-method day loop for one replicate (ssm.c:74-373 with one     */
 /* thread, mssm.c:105-383 for one replicate).                          */
 /* ---------------------------------------------------------------- */
+
+/* The SSA of one node over [t_time[node], next_unit_of_time)
+ * (ssm.c:144-235); returns 0 or an error code and counts the firings in
+ * *fired. */
+static int oracle_ssa_node(SimInf_solver_args *a, int node, int Nc, int Nt, int Nd,
+                           int Nld, int *u, const double *v, double *t_rate,
+                           double *sum_t_rate, double *t_time, siminf_stream *rng,
+                           double next_unit_of_time, long long *fired)
+{
+    int error = 0;
+    for (;;) {
+        double cum, rnd, tau, u_tau, u_tr, delta = 0.0;
+        int tr;
+
+        if (sum_t_rate[node] <= 0.0) {
+            t_time[node] = next_unit_of_time;
+            break;
+        }
+        /* One aligned pair of uniforms per attempt: the waiting
+         * time from the first, the transition from the second. */
+        siminf_uniform2(&rng[node], &u_tau, &u_tr);
+        tau = -siminf_log(u_tau) / sum_t_rate[node];
+        if ((tau + t_time[node]) >= next_unit_of_time) {
+            t_time[node] = next_unit_of_time;
+            break;
+        }
+        t_time[node] += tau;
+
+        rnd = u_tr * sum_t_rate[node];
+        tr = 0;
+        cum = t_rate[node * Nt];
+        while (tr < Nt && rnd > cum) {
+            tr++;
+            if (tr < Nt)
+                cum += t_rate[node * Nt + tr];
+        }
+
+        if (tr >= Nt)
+            tr = Nt - 1;
+        if (t_rate[node * Nt + tr] == 0.0) {
+            while (tr > 0 && t_rate[node * Nt + tr] == 0.0)
+                tr--;
+            if (t_rate[node * Nt + tr] == 0.0) {
+                sum_t_rate[node] = 0.0;
+                break;
+            }
+        }
+
+        (*fired)++;
+
+        for (int j = a->jcS[tr]; j < a->jcS[tr + 1]; j++) {
+            u[node * Nc + a->irS[j]] += a->prS[j];
+            if (u[node * Nc + a->irS[j]] < 0)
+                error = SIMINF_ERR_NEGATIVE_STATE;
+        }
+
+        for (int j = a->jcG[tr]; j < a->jcG[tr + 1]; j++) {
+            const int k = a->irG[j];
+            const double old = t_rate[node * Nt + k];
+            const double rate = a->tr_fun[k](&u[node * Nc], &v[node * Nd],
+                                             &a->ldata[node * Nld],
+                                             a->gdata, t_time[node]);
+            t_rate[node * Nt + k] = rate;
+            delta += rate - old;
+            if (!oracle_rate_ok(rate))
+                error = SIMINF_ERR_INVALID_RATE;
+        }
+        sum_t_rate[node] += delta;
+
+        /* The node stops at its first failing step; the
+         * reference keeps drawing until the day ends. */
+        if (error)
+            break;
+    }
+    return error;
+}
+
+/* Post time step and rate refresh of one node (ssm.c:262-302): returns
+ * the post-time-step function's error code (< 0), else 0; an invalid
+ * rate sets *error and the loop goes on, as in the reference. */
+static int oracle_pts_node(SimInf_solver_args *a, int node, int Nc, int Nt, int Nd,
+                           int Nld, const int *u, const double *v, double *v_new,
+                           double *t_rate, double *sum_t_rate, int *update_node,
+                           double tt, int *error)
+{
+    const int rc = a->pts_fun(&v_new[node * Nd], &u[node * Nc],
+                              &v[node * Nd], &a->ldata[node * Nld],
+                              a->gdata, node, tt);
+    if (rc < 0)
+        return rc;
+    if (rc > 0 || update_node[node]) {
+        double delta = 0.0;
+        for (int j = 0; j < Nt; j++) {
+            const double old = t_rate[node * Nt + j];
+            const double rate = a->tr_fun[j](&u[node * Nc],
+                                             &v_new[node * Nd],
+                                             &a->ldata[node * Nld],
+                                             a->gdata, tt);
+            t_rate[node * Nt + j] = rate;
+            delta += rate - old;
+            if (!oracle_rate_ok(rate))
+                *error = SIMINF_ERR_INVALID_RATE;
+        }
+        sum_t_rate[node] += delta;
+        update_node[node] = 0;
+    }
+    return 0;
+}
 
 static int oracle_direct_replicate(oracle_ctx *c, int rep, int is_mssm)
 {
@@ -492,72 +635,31 @@ static int oracle_direct_replicate(oracle_ctx *c, int rep, int is_mssm)
     }
 
     while (!error) {
-        /* (1) SSA per node (ssm.c:144-235). */
-        for (int node = 0; node < Nn && !error; node++) {
-            for (;;) {
-                double cum, rnd, tau, u_tau, u_tr, delta = 0.0;
-                int tr;
-
-                if (sum_t_rate[node] <= 0.0) {
-                    t_time[node] = next_unit_of_time;
-                    break;
-                }
-                /* One aligned pair of uniforms per attempt: the waiting
-                 * time from the first, the transition from the second. */
-                siminf_uniform2(&rng[node], &u_tau, &u_tr);
-                tau = -siminf_log(u_tau) / sum_t_rate[node];
-                if ((tau + t_time[node]) >= next_unit_of_time) {
-                    t_time[node] = next_unit_of_time;
-                    break;
-                }
-                t_time[node] += tau;
-
-                rnd = u_tr * sum_t_rate[node];
-                tr = 0;
-                cum = t_rate[node * Nt];
-                while (tr < Nt && rnd > cum) {
-                    tr++;
-                    if (tr < Nt)
-                        cum += t_rate[node * Nt + tr];
-                }
-
-                if (tr >= Nt)
-                    tr = Nt - 1;
-                if (t_rate[node * Nt + tr] == 0.0) {
-                    while (tr > 0 && t_rate[node * Nt + tr] == 0.0)
-                        tr--;
-                    if (t_rate[node * Nt + tr] == 0.0) {
-                        sum_t_rate[node] = 0.0;
-                        break;
-                    }
-                }
-
-                oracle_transitions_count++;
-
-                for (int j = a->jcS[tr]; j < a->jcS[tr + 1]; j++) {
-                    u[node * Nc + a->irS[j]] += a->prS[j];
-                    if (u[node * Nc + a->irS[j]] < 0)
-                        error = SIMINF_ERR_NEGATIVE_STATE;
-                }
-
-                for (int j = a->jcG[tr]; j < a->jcG[tr + 1]; j++) {
-                    const int k = a->irG[j];
-                    const double old = t_rate[node * Nt + k];
-                    const double rate = a->tr_fun[k](&u[node * Nc], &v[node * Nd],
-                                                     &a->ldata[node * Nld],
-                                                     a->gdata, t_time[node]);
-                    t_rate[node * Nt + k] = rate;
-                    delta += rate - old;
-                    if (!oracle_rate_ok(rate))
-                        error = SIMINF_ERR_INVALID_RATE;
-                }
-                sum_t_rate[node] += delta;
-
-                /* The node stops at its first failing step; the
-                 * reference keeps drawing until the day ends. */
-                if (error)
-                    break;
+        /* (1) SSA per node (ssm.c:144-235): one thread, or the node blocks
+         * of the threads. */
+        if (oracle_threads <= 1) {
+            long long fired = 0;
+            for (int node = 0; node < Nn && !error; node++)
+                error = oracle_ssa_node(a, node, Nc, Nt, Nd, Nld, u, v, t_rate, sum_t_rate,
+                                        t_time, rng, next_unit_of_time, &fired);
+            oracle_transitions_count += fired;
+        } else {
+            long long fired = 0;
+            const int T = oracle_threads;
+#pragma omp parallel num_threads(T) reduction(+:fired)
+            {
+                int my = 0;
+#pragma omp for schedule(static)
+                for (int node = 0; node < Nn; node++)
+                    if (!my)
+                        my = oracle_ssa_node(a, node, Nc, Nt, Nd, Nld, u, v, t_rate,
+                                             sum_t_rate, t_time, rng, next_unit_of_time,
+                                             &fired);
+#pragma omp critical
+                if (my && !error)
+                    error = my;
             }
+            oracle_transitions_count += fired;
         }
 
         /* (2, 3) Scheduled events. */
@@ -566,28 +668,32 @@ static int oracle_direct_replicate(oracle_ctx *c, int rep, int is_mssm)
                                           &cursor1, &cursor2);
 
         /* (4) Post time step and rate refresh (ssm.c:262-302). */
-        for (int node = 0; node < Nn && (!is_mssm || !error); node++) {
-            const int rc = a->pts_fun(&v_new[node * Nd], &u[node * Nc],
-                                      &v[node * Nd], &a->ldata[node * Nld],
-                                      a->gdata, node, tt);
-            if (rc < 0) {
-                error = rc;
-                break;
-            } else if (rc > 0 || update_node[node]) {
-                double delta = 0.0;
-                for (int j = 0; j < Nt; j++) {
-                    const double old = t_rate[node * Nt + j];
-                    const double rate = a->tr_fun[j](&u[node * Nc],
-                                                     &v_new[node * Nd],
-                                                     &a->ldata[node * Nld],
-                                                     a->gdata, tt);
-                    t_rate[node * Nt + j] = rate;
-                    delta += rate - old;
-                    if (!oracle_rate_ok(rate))
-                        error = SIMINF_ERR_INVALID_RATE;
+        if (oracle_threads <= 1) {
+            for (int node = 0; node < Nn && (!is_mssm || !error); node++) {
+                const int rc = oracle_pts_node(a, node, Nc, Nt, Nd, Nld, u, v, v_new, t_rate,
+                                               sum_t_rate, update_node, tt, &error);
+                if (rc < 0) {
+                    error = rc;
+                    break;
                 }
-                sum_t_rate[node] += delta;
-                update_node[node] = 0;
+            }
+        } else {
+            const int T = oracle_threads;
+#pragma omp parallel num_threads(T)
+            {
+                int my = 0;
+#pragma omp for schedule(static)
+                for (int node = 0; node < Nn; node++) {
+                    if (my < 0 && my != SIMINF_ERR_INVALID_RATE)
+                        continue;
+                    const int rc = oracle_pts_node(a, node, Nc, Nt, Nd, Nld, u, v, v_new,
+                                                   t_rate, sum_t_rate, update_node, tt, &my);
+                    if (rc < 0)
+                        my = rc;
+                }
+#pragma omp critical
+                if (my && !error)
+                    error = my;
             }
         }
 
@@ -707,8 +813,113 @@ static void oracle_count_ties(const double *times, int Nt)
 {
     for (int i = 0; i < Nt; i++)
         for (int j = i + 1; j < Nt; j++)
-            if (!isinf(times[i]) && times[i] == times[j])
+            if (!isinf(times[i]) && times[i] == times[j]) {
+#pragma omp atomic
                 oracle_aem_finite_ties++;
+            }
+}
+
+/* The all-events method of one node up to next_unit_of_time
+ * (aem.c:188-304); returns 0 or an error code and counts the firings in
+ * *fired. */
+static int oracle_aem_node(SimInf_solver_args *a, int node, int Nc, int Nt, int Nd,
+                           int Nld, int *u, const double *v, double *t_rate,
+                           double *t_time, siminf_stream *rng_vec, int *reactHeap,
+                           int *reactNode, double *reactTimes, double *reactInf,
+                           double next_unit_of_time, long long *fired)
+{
+    int error = 0;
+    for (;;) {
+        int tr;
+        double old_t_rate, rate;
+
+        oracle_count_ties(&reactTimes[Nt * node], Nt);
+        t_time[node] = reactTimes[Nt * node];
+        if (isinf(t_time[node]) || t_time[node] >= next_unit_of_time) {
+            t_time[node] = next_unit_of_time;
+            break;
+        }
+
+        tr = reactNode[Nt * node] % Nt;
+        (*fired)++;
+
+        for (int j = a->jcS[tr]; j < a->jcS[tr + 1]; j++) {
+            u[node * Nc + a->irS[j]] += a->prS[j];
+            if (u[node * Nc + a->irS[j]] < 0)
+                error = SIMINF_ERR_NEGATIVE_STATE;
+        }
+
+        for (int ii = a->jcG[tr]; ii < a->jcG[tr + 1]; ii++) {
+            const int j = a->irG[ii];
+            if (j == tr)
+                continue;
+            old_t_rate = t_rate[node * Nt + j];
+            rate = a->tr_fun[j](&u[node * Nc], &v[node * Nd],
+                                &a->ldata[node * Nld], a->gdata,
+                                t_time[node]);
+            t_rate[node * Nt + j] = rate;
+            if (!oracle_rate_ok(rate))
+                error = SIMINF_ERR_INVALID_RATE;
+            oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + j]],
+                              &reactInf[Nt * node + j], t_time[node],
+                              old_t_rate, t_rate[node * Nt + j],
+                              &rng_vec[Nt * node + j]);
+            heap_update(reactHeap[Nt * node + j], &reactTimes[Nt * node],
+                        &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+        }
+
+        old_t_rate = t_rate[node * Nt + tr];
+        rate = a->tr_fun[tr](&u[node * Nc], &v[node * Nd],
+                             &a->ldata[node * Nld], a->gdata,
+                             t_time[node]);
+        t_rate[node * Nt + tr] = rate;
+        if (!oracle_rate_ok(rate))
+            error = SIMINF_ERR_INVALID_RATE;
+        oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + tr]],
+                          &reactInf[Nt * node + tr], t_time[node],
+                          old_t_rate, t_rate[node * Nt + tr],
+                          &rng_vec[Nt * node + tr]);
+        heap_update(reactHeap[Nt * node + tr], &reactTimes[Nt * node],
+                    &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+        if (error)
+            break;
+    }
+    return error;
+}
+
+/* aem's post time step and refresh through calcTimes of one node
+ * (aem.c:329-384); as oracle_pts_node. */
+static int oracle_aem_pts_node(SimInf_solver_args *a, int node, int Nc, int Nt, int Nd,
+                               int Nld, const int *u, const double *v, double *v_new,
+                               double *t_rate, double *t_time, siminf_stream *rng_vec,
+                               int *reactHeap, int *reactNode, double *reactTimes,
+                               double *reactInf, int *update_node, double tt, int *error)
+{
+    const int rc = a->pts_fun(&v_new[node * Nd], &u[node * Nc],
+                              &v[node * Nd], &a->ldata[node * Nld],
+                              a->gdata, node, tt);
+    if (rc < 0)
+        return rc;
+    if (rc > 0 || update_node[node]) {
+        for (int j = 0; j < Nt; j++) {
+            const double old = t_rate[node * Nt + j];
+            const double rate = a->tr_fun[j](&u[node * Nc],
+                                             &v_new[node * Nd],
+                                             &a->ldata[node * Nld],
+                                             a->gdata, tt);
+            t_rate[node * Nt + j] = rate;
+            if (!oracle_rate_ok(rate))
+                *error = SIMINF_ERR_INVALID_RATE;
+            oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + j]],
+                              &reactInf[Nt * node + j], t_time[node],
+                              old, t_rate[node * Nt + j],
+                              &rng_vec[Nt * node + j]);
+            heap_update(reactHeap[Nt * node + j], &reactTimes[Nt * node],
+                        &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+        }
+        update_node[node] = 0;
+    }
+    return 0;
 }
 
 static int oracle_aem(oracle_ctx *c)
@@ -765,93 +976,66 @@ static int oracle_aem(oracle_ctx *c)
     }
 
     while (!error) {
-        for (int node = 0; node < Nn && !error; node++) {
-            for (;;) {
-                int tr;
-                double old_t_rate, rate;
-
-                oracle_count_ties(&reactTimes[Nt * node], Nt);
-                t_time[node] = reactTimes[Nt * node];
-                if (isinf(t_time[node]) || t_time[node] >= next_unit_of_time) {
-                    t_time[node] = next_unit_of_time;
-                    break;
-                }
-
-                tr = reactNode[Nt * node] % Nt;
-                oracle_transitions_count++;
-
-                for (int j = a->jcS[tr]; j < a->jcS[tr + 1]; j++) {
-                    u[node * Nc + a->irS[j]] += a->prS[j];
-                    if (u[node * Nc + a->irS[j]] < 0)
-                        error = SIMINF_ERR_NEGATIVE_STATE;
-                }
-
-                for (int ii = a->jcG[tr]; ii < a->jcG[tr + 1]; ii++) {
-                    const int j = a->irG[ii];
-                    if (j == tr)
-                        continue;
-                    old_t_rate = t_rate[node * Nt + j];
-                    rate = a->tr_fun[j](&u[node * Nc], &v[node * Nd],
-                                        &a->ldata[node * Nld], a->gdata,
-                                        t_time[node]);
-                    t_rate[node * Nt + j] = rate;
-                    if (!oracle_rate_ok(rate))
-                        error = SIMINF_ERR_INVALID_RATE;
-                    oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + j]],
-                                      &reactInf[Nt * node + j], t_time[node],
-                                      old_t_rate, t_rate[node * Nt + j],
-                                      &rng_vec[Nt * node + j]);
-                    heap_update(reactHeap[Nt * node + j], &reactTimes[Nt * node],
-                                &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
-                }
-
-                old_t_rate = t_rate[node * Nt + tr];
-                rate = a->tr_fun[tr](&u[node * Nc], &v[node * Nd],
-                                     &a->ldata[node * Nld], a->gdata,
-                                     t_time[node]);
-                t_rate[node * Nt + tr] = rate;
-                if (!oracle_rate_ok(rate))
-                    error = SIMINF_ERR_INVALID_RATE;
-                oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + tr]],
-                                  &reactInf[Nt * node + tr], t_time[node],
-                                  old_t_rate, t_rate[node * Nt + tr],
-                                  &rng_vec[Nt * node + tr]);
-                heap_update(reactHeap[Nt * node + tr], &reactTimes[Nt * node],
-                            &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
-                if (error)
-                    break;
+        if (oracle_threads <= 1) {
+            long long fired = 0;
+            for (int node = 0; node < Nn && !error; node++)
+                error = oracle_aem_node(a, node, Nc, Nt, Nd, Nld, u, v, t_rate, t_time,
+                                        rng_vec, reactHeap, reactNode, reactTimes, reactInf,
+                                        next_unit_of_time, &fired);
+            oracle_transitions_count += fired;
+        } else {
+            long long fired = 0;
+            const int T = oracle_threads;
+#pragma omp parallel num_threads(T) reduction(+:fired)
+            {
+                int my = 0;
+#pragma omp for schedule(static)
+                for (int node = 0; node < Nn; node++)
+                    if (!my)
+                        my = oracle_aem_node(a, node, Nc, Nt, Nd, Nld, u, v, t_rate, t_time,
+                                             rng_vec, reactHeap, reactNode, reactTimes,
+                                             reactInf, next_unit_of_time, &fired);
+#pragma omp critical
+                if (my && !error)
+                    error = my;
             }
+            oracle_transitions_count += fired;
         }
 
         if (!error)
             error = oracle_process_events(c, 0, tt, u, update_node, node_rng,
                                           &cursor1, &cursor2);
 
-        for (int node = 0; node < Nn; node++) {
-            const int rc = a->pts_fun(&v_new[node * Nd], &u[node * Nc],
-                                      &v[node * Nd], &a->ldata[node * Nld],
-                                      a->gdata, node, tt);
-            if (rc < 0) {
-                error = rc;
-                break;
-            } else if (rc > 0 || update_node[node]) {
-                for (int j = 0; j < Nt; j++) {
-                    const double old = t_rate[node * Nt + j];
-                    const double rate = a->tr_fun[j](&u[node * Nc],
-                                                     &v_new[node * Nd],
-                                                     &a->ldata[node * Nld],
-                                                     a->gdata, tt);
-                    t_rate[node * Nt + j] = rate;
-                    if (!oracle_rate_ok(rate))
-                        error = SIMINF_ERR_INVALID_RATE;
-                    oracle_calc_times(&reactTimes[Nt * node + reactHeap[Nt * node + j]],
-                                      &reactInf[Nt * node + j], t_time[node],
-                                      old, t_rate[node * Nt + j],
-                                      &rng_vec[Nt * node + j]);
-                    heap_update(reactHeap[Nt * node + j], &reactTimes[Nt * node],
-                                &reactNode[Nt * node], &reactHeap[Nt * node], Nt);
+        if (oracle_threads <= 1) {
+            for (int node = 0; node < Nn; node++) {
+                const int rc = oracle_aem_pts_node(a, node, Nc, Nt, Nd, Nld, u, v, v_new,
+                                                   t_rate, t_time, rng_vec, reactHeap,
+                                                   reactNode, reactTimes, reactInf,
+                                                   update_node, tt, &error);
+                if (rc < 0) {
+                    error = rc;
+                    break;
                 }
-                update_node[node] = 0;
+            }
+        } else {
+            const int T = oracle_threads;
+#pragma omp parallel num_threads(T)
+            {
+                int my = 0;
+#pragma omp for schedule(static)
+                for (int node = 0; node < Nn; node++) {
+                    if (my < 0 && my != SIMINF_ERR_INVALID_RATE)
+                        continue;
+                    const int rc = oracle_aem_pts_node(a, node, Nc, Nt, Nd, Nld, u, v, v_new,
+                                                       t_rate, t_time, rng_vec, reactHeap,
+                                                       reactNode, reactTimes, reactInf,
+                                                       update_node, tt, &my);
+                    if (rc < 0)
+                        my = rc;
+                }
+#pragma omp critical
+                if (my && !error)
+                    error = my;
             }
         }
 

@@ -158,11 +158,17 @@ struct KParams {
     double *prV;
 
     /* Coupled kernel: compartment state published for the E2 phase and
-     * the update flags it sets; rep0 is the replicate a grid-wide launch
-     * runs. */
+     * the update flags it sets.  A grid-wide launch runs the replicates
+     * rep0, rep0 + 1, ... with rep_blocks workgroups each (0: one
+     * replicate over the whole grid), the workgroups of each replicate
+     * joined by barrier words of their own; rep_err (one per replicate,
+     * or null) keeps each replicate's first error, so that a replicate's
+     * exit decision never depends on another's errors. */
     int *u_work;                     /* Nc x (Nn*Nrep) */
     int *upd_work;                   /* Nn*Nrep */
     int rep0;
+    int rep_blocks;
+    unsigned long long *rep_err;
     /* The coupled kernel runs the all-events method instead of the SSA
      * (one replicate, grid launch). */
     int coupled_aem;
@@ -171,8 +177,9 @@ struct KParams {
      * run (Nn). */
     const int *lane_node;
     unsigned long long *node_fired;
-    /* Grid launch: the barrier words (arrivals, last day whose E2 work is
-     * done), cleared before each launch. */
+    /* Grid launch: the barrier words of each replicate of the launch
+     * (arrivals, last day whose E2 work is done), cleared before each
+     * launch. */
     unsigned *gsync;
 
     /* Node kernels with a dense trajectory: a node whose state can no
@@ -210,7 +217,10 @@ struct KParams {
 __device__ __forceinline__ void record_error(const KParams &p, int rep, int day,
                                              int phase, long long pos, int code)
 {
-    atomicMin(p.err_key, (unsigned long long)error_key(rep, day, phase, pos, code));
+    const unsigned long long key = error_key(rep, day, phase, pos, code);
+    atomicMin(p.err_key, key);
+    if (p.rep_err)
+        atomicMin(p.rep_err + rep, key);
 }
 
 __device__ __forceinline__ bool rate_ok(double r)

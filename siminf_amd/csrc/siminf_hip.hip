@@ -190,6 +190,8 @@ struct SimInf_hip_plan {
     hipStream_t fill_stream = nullptr;   /* the fill kernel, beside the active pass */
     hipEvent_t ev_classified = nullptr, ev_fill0 = nullptr, ev_fill1 = nullptr,
                ev_active1 = nullptr;
+    /* The fill stream's fork from and join to the plan's stream. */
+    hipEvent_t dep_fork = nullptr, dep_join = nullptr;
     int simds = 1024;                     /* SIMDs of the device (4 per CU) */
     hipEvent_t ev_start = nullptr, ev_mid = nullptr, ev_stop = nullptr;
     unsigned grid = 0, block = 0;
@@ -212,6 +214,9 @@ struct SimInf_hip_plan {
      * other herds per wavefront, and the workgroups a grid launch may
      * hold. */
     int solo = 0, pack = SIMINF_GRID_BLOCK, grid_max = 0;
+    /* Grid-wide coupled plans: the workgroups one cooperative launch may
+     * hold, and the replicates a launch runs under the current map. */
+    int max_blocks = 0, reps_per_launch = 1;
 
     /* Last run. */
     bool has_run = false;       /* a run completed without error */
@@ -610,7 +615,9 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         hipEventCreate(&P->ev_active1) != hipSuccess ||
         hipEventCreate(&P->ev_start) != hipSuccess ||
         hipEventCreate(&P->ev_mid) != hipSuccess ||
-        hipEventCreate(&P->ev_stop) != hipSuccess) {
+        hipEventCreate(&P->ev_stop) != hipSuccess ||
+        hipEventCreateWithFlags(&P->dep_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&P->dep_join, hipEventDisableTiming) != hipSuccess) {
         SimInf_hip_plan_destroy(P);
         return SIMINF_ERR_DEVICE;
     }
@@ -743,9 +750,13 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
             cus = 1;
         size_t budget = std::min<size_t>((size_t)lds_max, 128 * 1024);
         if (P->kernel == K_COUPLED_GRID) {
-            /* The herds' workgroups and the one that runs E2. */
-            const int blocks = (Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1;
-            const int per_cu = (blocks + cus - 1) / cus;
+            /* The herds' workgroups and the one that runs E2, per
+             * replicate; replicates share a launch, up to 16 one-wavefront
+             * workgroups per CU. */
+            const long long blocks1 = (Nn + SIMINF_GRID_BLOCK - 1) / SIMINF_GRID_BLOCK + 1;
+            const long long blocks = std::min<long long>(blocks1 * Nrep,
+                                                         std::max<long long>(blocks1, 16LL * cus));
+            const long long per_cu = (blocks + cus - 1) / cus;
             const size_t share = (size_t)lds_cu / (size_t)per_cu;
             budget = std::min<size_t>(budget, share > 2048 ? share - 1024 : 1024);
         }
@@ -802,9 +813,14 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
              * wavefront, within what one co-resident grid holds. */
             P->grid_max = std::min<int>(
                 max_blocks, (Nn + SIMINF_PACK_MIN - 1) / SIMINF_PACK_MIN + SIMINF_SOLO_MAX + 1);
+            P->max_blocks = max_blocks;
             p.lane_node = B.alloc<int>((size_t)P->grid_max * SIMINF_GRID_BLOCK);
             p.node_fired = B.alloc<unsigned long long>((size_t)Nn);
-            p.gsync = B.alloc<unsigned>(2);
+            /* Barrier words for the most replicates one launch can hold,
+             * and each replicate's error key. */
+            p.gsync = B.alloc<unsigned>(2 * (size_t)std::min(Nrep, max_blocks));
+            if (Nrep > 1)
+                p.rep_err = B.alloc<unsigned long long>((size_t)Nrep);
         } else {
             P->block = (unsigned)(((Nn + 63) / 64) * 64);
             P->grid = (unsigned)Nrep;
@@ -856,12 +872,29 @@ extern "C" int SimInf_hip_plan_create(SimInf_hip_plan **out,
     return plan_create_impl(out, a, model, false);
 }
 
+/* A grid-wide coupled plan's replicates, reps_per_launch to a
+ * cooperative launch (P->grid workgroups each). */
+static hipError_t launch_grid_reps(SimInf_hip_plan *P, KParams q, hipStream_t st)
+{
+    const int R = std::max(1, P->reps_per_launch);
+    q.rep_blocks = (int)P->grid;
+    for (int r0 = 0; r0 < P->Nrep; r0 += R) {
+        const int n = std::min(R, P->Nrep - r0);
+        q.rep0 = r0;
+        const hipError_t e = P->model->coupled(q, P->canon, true, P->grid * (unsigned)n,
+                                               P->block, P->lds, st, nullptr);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 /* Clears a run's result words and counters on the device (a kernel, not
  * memcpys or memsets, so that a run's launch sequence is kernels only and
  * can be captured in a graph). */
 __global__ void k_run_init(unsigned long long *err_key, unsigned long long *transitions,
                            unsigned long long *frozen_cols, int *active_count,
-                           unsigned *gsync)
+                           unsigned *gsync, unsigned long long *rep_err, int nrep)
 {
     if (threadIdx.x == 0) {
         *err_key = ~0ull;
@@ -873,13 +906,17 @@ __global__ void k_run_init(unsigned long long *err_key, unsigned long long *tran
     }
     if (gsync && threadIdx.x < 2)
         gsync[threadIdx.x] = 0u;
+    if (rep_err)
+        for (int i = (int)threadIdx.x; i < nrep; i += (int)blockDim.x)
+            rep_err[i] = ~0ull;
 }
 
-/* Enqueue one run of the plan on its streams: kernels and stream-event
- * dependencies only, nothing that waits for the device, so the sequence
- * can be captured in a graph (capture: no timing events, and a grid-wide
- * coupled plan, whose kernel needs a cooperative launch, is refused).
- * *filled tells whether the frozen-node fill kernel ran. */
+/* Enqueue one run of the plan on its streams: kernels, timing events and
+ * stream-event dependencies only, nothing that waits for the device, so
+ * the sequence can be captured in a graph (where the timing events become
+ * event-record nodes; a grid-wide coupled plan, whose kernel needs a
+ * cooperative launch, is refused).  *filled tells whether the
+ * frozen-node fill kernel ran. */
 static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, bool *filled)
 {
     KParams p = base;
@@ -887,10 +924,13 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
     *filled = false;
     if (capture && P->kernel == K_COUPLED_GRID)
         return SIMINF_ERR_INVALID_MODEL;
-    if (!capture)
-        (void)hipEventRecord(P->ev_start, st);
-    hipLaunchKernelGGL(k_run_init, dim3(1), dim3(64), 0, st, p.err_key, p.transitions,
-                       p.frozen_cols, p.active_count, p.gsync);
+    auto mark = [&](hipEvent_t ev, hipStream_t s) {
+        (void)hipEventRecordWithFlags(ev, s, capture ? hipEventRecordExternal
+                                                     : hipEventRecordDefault);
+    };
+    mark(P->ev_start, st);
+    hipLaunchKernelGGL(k_run_init, dim3(1), dim3(256), 0, st, p.err_key, p.transitions,
+                       p.frozen_cols, p.active_count, p.gsync, p.rep_err, P->Nrep);
     switch (P->kernel) {
     case K_SSM_NODES:
     case K_AEM_NODES: {
@@ -903,8 +943,7 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
         if (!p.frozen_from) {
             p.mode = NODES_ALL;
             launch(p, P->grid);
-            if (!capture)
-                (void)hipEventRecord(P->ev_mid, st);
+            mark(P->ev_mid, st);
             break;
         }
         /* Classify every lane; nodes frozen from the start go to the fill
@@ -913,11 +952,11 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
          * reads how many there are on the device). */
         p.mode = NODES_CLASSIFY;
         launch(p, P->grid);
-        if (hipEventRecord(P->ev_classified, st) != hipSuccess ||
-            hipStreamWaitEvent(P->fill_stream, P->ev_classified, 0) != hipSuccess)
+        mark(P->ev_classified, st);
+        if (hipEventRecord(P->dep_fork, st) != hipSuccess ||
+            hipStreamWaitEvent(P->fill_stream, P->dep_fork, 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
-        if (!capture)
-            (void)hipEventRecord(P->ev_fill0, P->fill_stream);
+        mark(P->ev_fill0, P->fill_stream);
         {
             const dim3 fg(P->grid), fb(P->block);
             hipStream_t fs = P->fill_stream;
@@ -940,28 +979,23 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
             }
 #undef SIMINF_FILL
         }
-        if (hipEventRecord(P->ev_fill1, P->fill_stream) != hipSuccess)
+        mark(P->ev_fill1, P->fill_stream);
+        if (hipEventRecord(P->dep_join, P->fill_stream) != hipSuccess)
             return SIMINF_ERR_DEVICE;
         p.mode = NODES_ACTIVE;
         p.lpw_shift = P->lpw_fixed;
-        if (!capture)
-            (void)hipEventRecord(P->ev_mid, st);
+        mark(P->ev_mid, st);
         launch(p, P->grid_active);
-        if (!capture)
-            (void)hipEventRecord(P->ev_active1, st);
-        if (hipStreamWaitEvent(st, P->ev_fill1, 0) != hipSuccess)
+        mark(P->ev_active1, st);
+        if (hipStreamWaitEvent(st, P->dep_join, 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
         *filled = true;
         break;
     }
     case K_COUPLED_GRID:
-        /* One cooperative launch per replicate. */
-        for (int r = 0; r < P->Nrep; r++) {
-            p.rep0 = r;
-            if (P->model->coupled(p, P->canon, true, P->grid, P->block, P->lds, st,
-                                  nullptr) != hipSuccess)
-                return SIMINF_ERR_DEVICE;
-        }
+        /* Cooperative launches of reps_per_launch replicates each. */
+        if (launch_grid_reps(P, p, st) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
         break;
     case K_COUPLED_REPS:
         if (P->model->coupled(p, P->canon, false, P->grid, P->block, P->lds, st,
@@ -969,11 +1003,9 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
             return SIMINF_ERR_DEVICE;
         break;
     }
-    if (!capture) {
-        if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES)
-            (void)hipEventRecord(P->ev_mid, st);
-        (void)hipEventRecord(P->ev_stop, st);
-    }
+    if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES)
+        mark(P->ev_mid, st);
+    mark(P->ev_stop, st);
     if (hipGetLastError() != hipSuccess)
         return SIMINF_ERR_DEVICE;
     return 0;
@@ -981,7 +1013,7 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
 
 /* Read a finished run's error key and counts (and, when timed, its
  * kernel times); returns the run's error code. */
-static int plan_collect(SimInf_hip_plan *P, bool timed, bool filled)
+static int plan_collect(SimInf_hip_plan *P, bool filled)
 {
     unsigned long long key = NO_ERROR_KEY, trans = 0, fill_cols = 0;
     const KParams &p = P->p;
@@ -993,7 +1025,7 @@ static int plan_collect(SimInf_hip_plan *P, bool timed, bool filled)
                                          hipMemcpyDeviceToHost, P->stream) != hipSuccess) ||
         hipStreamSynchronize(P->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
-    if (timed) {
+    {
         /* Device times: the whole run; the solver (classify and active
          * passes, or the one kernel); the fill kernel on its own
          * stream. */
@@ -1030,7 +1062,7 @@ extern "C" int SimInf_hip_plan_run(SimInf_hip_plan *P, unsigned long int seed)
     const int err = plan_enqueue(P, p, false, &filled);
     if (err)
         return err;
-    return plan_collect(P, true, filled);
+    return plan_collect(P, filled);
 }
 
 extern "C" int SimInf_hip_plan_download(SimInf_hip_plan *P, int *U, double *V,
@@ -1126,6 +1158,7 @@ static int plan_set_herd_map(SimInf_hip_plan *P, const int *nodes, int n, int pa
     P->grid = (unsigned)grid;
     P->solo = n;
     P->pack = pack;
+    P->reps_per_launch = std::max(1, std::min(P->Nrep, P->max_blocks / grid));
     return 0;
 }
 
@@ -1273,6 +1306,10 @@ extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
         (void)hipEventDestroy(P->ev_fill1);
     if (P->ev_active1)
         (void)hipEventDestroy(P->ev_active1);
+    if (P->dep_fork)
+        (void)hipEventDestroy(P->dep_fork);
+    if (P->dep_join)
+        (void)hipEventDestroy(P->dep_join);
     if (P->fill_stream)
         (void)hipStreamDestroy(P->fill_stream);
     if (P->ev_stop)
@@ -1410,13 +1447,11 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, B->ev0, B->ev1);
     B->ms = ms;
-    /* Each plan's results; its kernel times are the launch's. */
+    /* Each plan's results and kernel times (its timing events are
+     * event-record nodes of the graph). */
     int first = 0;
     for (int i = 0; i < n; i++) {
-        SimInf_hip_plan *P = B->plans[i];
-        const int code = plan_collect(P, false, B->filled[i] != 0);
-        P->kernel_ms = P->solver_ms = ms;
-        P->fill_ms = 0.0;
+        const int code = plan_collect(B->plans[i], B->filled[i] != 0);
         if (code && !first)
             first = code;
     }
@@ -1843,6 +1878,12 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
     std::vector<unsigned long long> keys(n_intervals, NO_ERROR_KEY);
     std::vector<int> perrs(n_intervals, 0);
     unsigned long long *keys_d = B.upload(keys.data(), keys.size());
+    if (P->p.rep_err) {
+        /* Each particle's error key of a grid-wide coupled plan. */
+        const std::vector<unsigned long long> none((size_t)n_particles, NO_ERROR_KEY);
+        B.copy(P->p.rep_err, none.data(), none.size() * sizeof(none[0]),
+               hipMemcpyHostToDevice);
+    }
     int *perr_d = B.upload(perrs.data(), perrs.size());
     const double zero = 0.0;
     if (B.status != hipSuccess ||
@@ -1883,11 +1924,7 @@ extern "C" int SimInf_hip_pfilter(const SimInf_solver_args *a,
         } else if (P->kernel == K_COUPLED_REPS) {
             model->coupled(q, P->canon, false, P->grid, P->block, P->lds, P->stream, nullptr);
         } else {
-            for (int r = 0; r < n_particles; r++) {
-                q.rep0 = r;
-                model->coupled(q, P->canon, true, P->grid, P->block, P->lds, P->stream,
-                               nullptr);
-            }
+            (void)launch_grid_reps(P, q, P->stream);
         }
         hipLaunchKernelGGL(k_pf_weights, dim3(n_particles), dim3(64), 0, P->stream, q.U,
                            Nn, Nc, q.tlen, obs_compartment, obs_dist, obs_p1, obs_p2,
@@ -2341,24 +2378,30 @@ __global__ __launch_bounds__(256) void k_abc_normalize(double *ww, const int *fl
         ww[i] = ww[i] / sum;
 }
 
-extern "C" int SimInf_hip_abc_weights(const char *distribution, const double *p1,
-                                      const double *p2, int n_particles, int n_parameters,
-                                      const double *x, const double *xx, const double *w,
-                                      const double *sigma, double *ww)
+/* The host part of the ABC weights: argument checks, each prior's kind
+ * and log-gamma term, and the Cholesky factor L of sigma with its
+ * log-normaliser.  Returns 0, or an error code; *uniform set when w is
+ * NULL (every weight 1/n, already written to ww). */
+static int abc_prepare(const char *distribution, const double *p1, const double *p2, int n,
+                       int d, const double *x, const double *xx, const double *w,
+                       const double *sigma, double *ww, std::vector<int> &kind,
+                       std::vector<double> &lg, std::vector<double> &L, double *lognorm,
+                       bool *uniform)
 {
-    const int n = n_particles, d = n_parameters;
+    *uniform = false;
     if (n < 1 || !ww)
         return SIMINF_ERR_INVALID_MODEL;
     if (!w) {
         for (int i = 0; i < n; i++)
             ww[i] = 1.0 / (double)n;
+        *uniform = true;
         return 0;
     }
     if (d < 1 || d > SIMINF_ABC_MAX_PARS || !distribution || !p1 || !p2 || !x || !xx ||
         !sigma)
         return SIMINF_ERR_INVALID_MODEL;
-    std::vector<int> kind(d);
-    std::vector<double> lg(d, 0.0);
+    kind.assign(d, 0);
+    lg.assign(d, 0.0);
     for (int k = 0; k < d; k++) {
         kind[k] = distribution[k];
         if (kind[k] != SIMINF_PRIOR_GAMMA && kind[k] != SIMINF_PRIOR_LOGNORMAL &&
@@ -2367,10 +2410,58 @@ extern "C" int SimInf_hip_abc_weights(const char *distribution, const double *p1
         if (kind[k] == SIMINF_PRIOR_GAMMA)
             lg[k] = std::lgamma(p1[k]);
     }
-    std::vector<double> L((size_t)d * d);
-    double lognorm = 0.0;
-    if (siminf_cholesky(sigma, d, L.data(), &lognorm))
+    L.assign((size_t)d * d, 0.0);
+    *lognorm = 0.0;
+    if (siminf_cholesky(sigma, d, L.data(), lognorm))
         return SIMINF_ERR_ABC_WEIGHTS;
+    return 0;
+}
+
+/* Device operands of the ABC weight kernels. */
+struct AbcOperands {
+    const double *x, *xx, *w, *L, *p1, *p2, *lg;
+    const int *kind;
+    double *zc, *zp, *logw;
+    int *flag, *err;
+};
+
+/* The weight kernels on stream st over operands already in device memory
+ * (err cleared); then the error code and the weights copied to the host,
+ * complete on return. */
+static int abc_launch(const AbcOperands &o, int n, int d, double lognorm, double *ww,
+                      hipStream_t st)
+{
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_abc_transform, dim3(grid), dim3(256), 0, st, o.xx, n, d, o.L, o.zc);
+    hipLaunchKernelGGL(k_abc_transform, dim3(grid), dim3(256), 0, st, o.x, n, d, o.L, o.zp);
+    hipLaunchKernelGGL(k_abc_logw, dim3(grid), dim3(256), 0, st, o.xx, o.x, o.zc, o.zp, o.w,
+                       n, d, o.kind, o.p1, o.p2, o.lg, lognorm, o.logw, o.flag);
+    hipLaunchKernelGGL(k_abc_normalize, dim3(1), dim3(256), 0, st, o.logw, o.flag, n, o.err);
+    if (hipGetLastError() != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    int err = 0;
+    if (hipMemcpyAsync(&err, o.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(ww, o.logw, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    return err;
+}
+
+extern "C" int SimInf_hip_abc_weights(const char *distribution, const double *p1,
+                                      const double *p2, int n_particles, int n_parameters,
+                                      const double *x, const double *xx, const double *w,
+                                      const double *sigma, double *ww)
+{
+    const int n = n_particles, d = n_parameters;
+    std::vector<int> kind;
+    std::vector<double> lg, L;
+    double lognorm = 0.0;
+    bool uniform = false;
+    const int perr = abc_prepare(distribution, p1, p2, n, d, x, xx, w, sigma, ww, kind, lg,
+                                 L, &lognorm, &uniform);
+    if (perr || uniform)
+        return perr;
 
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
@@ -2383,38 +2474,22 @@ extern "C" int SimInf_hip_abc_weights(const char *distribution, const double *p1
         B.stream = st;
         B.pooled = pool_allocation(dev);
         const size_t nd = (size_t)n * d;
-        const double *x_d = B.upload(x, nd), *xx_d = B.upload(xx, nd);
-        const double *w_d = B.upload(w, (size_t)n);
-        const double *L_d = B.upload(L.data(), L.size());
-        const int *kind_d = B.upload(kind.data(), kind.size());
-        const double *p1_d = B.upload(p1, (size_t)d), *p2_d = B.upload(p2, (size_t)d);
-        const double *lg_d = B.upload(lg.data(), lg.size());
-        double *zc = B.alloc<double>(nd), *zp = B.alloc<double>(nd);
-        double *logw = B.alloc<double>((size_t)n);
-        int *flag = B.alloc<int>((size_t)n);
+        AbcOperands o;
+        o.x = B.upload(x, nd);
+        o.xx = B.upload(xx, nd);
+        o.w = B.upload(w, (size_t)n);
+        o.L = B.upload(L.data(), L.size());
+        o.kind = B.upload(kind.data(), kind.size());
+        o.p1 = B.upload(p1, (size_t)d);
+        o.p2 = B.upload(p2, (size_t)d);
+        o.lg = B.upload(lg.data(), lg.size());
+        o.zc = B.alloc<double>(nd);
+        o.zp = B.alloc<double>(nd);
+        o.logw = B.alloc<double>((size_t)n);
+        o.flag = B.alloc<int>((size_t)n);
         const int zero = 0;
-        int *err_d = B.upload(&zero, 1);
-        if (B.status != hipSuccess) {
-            rc = SIMINF_ERR_DEVICE;
-        } else {
-            const unsigned grid = (unsigned)((n + 255) / 256);
-            hipLaunchKernelGGL(k_abc_transform, dim3(grid), dim3(256), 0, st, xx_d, n, d, L_d,
-                               zc);
-            hipLaunchKernelGGL(k_abc_transform, dim3(grid), dim3(256), 0, st, x_d, n, d, L_d,
-                               zp);
-            hipLaunchKernelGGL(k_abc_logw, dim3(grid), dim3(256), 0, st, xx_d, x_d, zc, zp, w_d,
-                               n, d, kind_d, p1_d, p2_d, lg_d, lognorm, logw, flag);
-            hipLaunchKernelGGL(k_abc_normalize, dim3(1), dim3(256), 0, st, logw, flag, n,
-                               err_d);
-            int err = 0;
-            if (hipGetLastError() != hipSuccess) {
-                rc = SIMINF_ERR_DEVICE;
-            } else {
-                B.copy(&err, err_d, sizeof(int), hipMemcpyDeviceToHost);
-                B.copy(ww, logw, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
-                rc = B.status != hipSuccess ? SIMINF_ERR_DEVICE : err;
-            }
-        }
+        o.err = B.upload(&zero, 1);
+        rc = B.status != hipSuccess ? SIMINF_ERR_DEVICE : abc_launch(o, n, d, lognorm, ww, st);
         B.release();
     }
     (void)hipStreamDestroy(st);

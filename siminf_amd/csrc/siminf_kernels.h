@@ -406,7 +406,8 @@ __device__ __forceinline__ bool error_by(const KParams &p, int rep, int day,
                                          int phase)
 {
     const unsigned long long k = __hip_atomic_load(
-        p.err_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p.rep_err ? p.rep_err + rep : p.err_key, __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
     return (k >> 27) <= (error_key(rep, day, phase, 0, -1) >> 27);
 }
 
@@ -677,9 +678,14 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     p.seed = run_seed(p);
     extern __shared__ int4 lds4[];
     unsigned char *lds = (unsigned char *)lds4;
-    const int rep = GRID ? p.rep0 : (int)blockIdx.x;
-    const int lane_g = GRID ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
-                            : (int)threadIdx.x;
+    /* Grid launch: G workgroups per replicate, the launch's replicate rb
+     * of them (rep0 + rb), local the workgroup within its replicate. */
+    const int G = GRID ? (p.rep_blocks > 0 ? p.rep_blocks : (int)gridDim.x) : 1;
+    const int rb = GRID ? (int)blockIdx.x / G : 0;
+    const int local = GRID ? (int)blockIdx.x - rb * G : 0;
+    const int rep = GRID ? p.rep0 + rb : (int)blockIdx.x;
+    const int lane_g = GRID ? (int)(local * blockDim.x + threadIdx.x) : (int)threadIdx.x;
+    unsigned *gsync = GRID ? p.gsync + 2 * rb : nullptr;
     const int node = GRID && p.lane_node ? p.lane_node[lane_g] : lane_g;
     const bool active = node >= 0 && node < p.Nn;
     const long long base = (long long)rep * p.Nn;
@@ -704,7 +710,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
 
     /* The workgroup that runs the E2 levels: in a grid launch the last,
      * which holds no herds. */
-    const bool e2wg = !GRID || blockIdx.x == gridDim.x - 1;
+    const bool e2wg = !GRID || local == G - 1;
     E2Day e2d{0, 0, 0, 0, 0, 0, E2_NONE};
 
     /* E and N into LDS for the workgroup that runs the E2 levels. */
@@ -725,16 +731,16 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         __syncthreads();
     }
 
-    /* Grid launch: every workgroup is one wavefront.  p.gsync[0] counts
-     * arrivals (each workgroup once at the start, then each herd
-     * workgroup once a day after its SSA and E1); p.gsync[1] is the last
-     * day whose E2 work is done.  A wavefront publishes its writes
-     * device-wide before it arrives or signals, and observes the others'
-     * after its wait. */
+    /* Grid launch: every workgroup is one wavefront.  gsync[0] counts
+     * the replicate's arrivals (each of its G workgroups once at the
+     * start, then each herd workgroup once a day after its SSA and E1);
+     * gsync[1] is the last day whose E2 work is done.  A wavefront
+     * publishes its writes device-wide before it arrives or signals, and
+     * observes the others' after its wait. */
     auto gs_arrive = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (threadIdx.x == 0)
-            __hip_atomic_fetch_add(p.gsync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gsync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
     auto gs_wait = [&](const unsigned *at, unsigned target) {
         if (threadIdx.x == 0)
@@ -745,7 +751,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     auto barrier = [&]() {
         if constexpr (GRID) {
             gs_arrive();
-            gs_wait(p.gsync, gridDim.x);
+            gs_wait(gsync, (unsigned)G);
         } else {
             __syncthreads();
         }
@@ -864,15 +870,15 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
              * orders every error of the day's SSA, E1 and E2 before it. */
             if (e2wg) {
                 e2d = e2_stage<NC>(p, rep, day, lds);
-                gs_wait(p.gsync, gridDim.x + (unsigned)(day + 1) * (gridDim.x - 1));
+                gs_wait(gsync, (unsigned)G + (unsigned)(day + 1) * (unsigned)(G - 1));
                 myerr |= e2_run<NC>(p, tab, rep, day, uw, updw, lds, e2d);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 if (threadIdx.x == 0)
-                    __hip_atomic_store(p.gsync + 1, (unsigned)(day + 1), __ATOMIC_RELAXED,
+                    __hip_atomic_store(gsync + 1, (unsigned)(day + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             } else {
                 gs_arrive();
-                gs_wait(p.gsync + 1, (unsigned)(day + 1));
+                gs_wait(gsync + 1, (unsigned)(day + 1));
             }
         } else {
             __syncthreads();
@@ -947,7 +953,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
         if (U_it >= p.tlen)
             break;
     }
-    if (GRID && active && p.node_fired)
+    if (GRID && active && p.node_fired && rep == 0)
         p.node_fired[node] = fired;
     add_transitions(p, fired);
 }
@@ -1064,8 +1070,8 @@ void launch_aem_nodes(const KParams &p, bool events, bool canon,
  * the launch sequence holds kernels only). */
 static __global__ void k_clear_u32(unsigned *w, int n)
 {
-    if ((int)threadIdx.x < n)
-        w[threadIdx.x] = 0u;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x)
+        w[i] = 0u;
 }
 
 /* Block sizes of the coupled kernel: 64 lanes per workgroup when one
@@ -1124,7 +1130,9 @@ hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
     if (grid_mode) {
         if (!p.gsync)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, p.gsync, 2);
+        /* Two barrier words per replicate of the launch. */
+        const int nrep = p.rep_blocks > 0 ? (int)grid / p.rep_blocks : 1;
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, st, p.gsync, 2 * nrep);
         KParams q = p;
         void *args[] = {&q};
         return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args,

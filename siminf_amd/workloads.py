@@ -285,3 +285,40 @@ def pfilter_requests(n_requests=24, seed=5, nodes=(100, 100000), n_particles=200
         out.append(FilterSpec(model, times, obs.reshape(-1, 1), n_particles, obs_prob,
                               int(rng.integers(0, 2**62)), float(arrivals[k]), beta, gamma))
     return out
+
+
+def pfilter_binomial_request(Nn=10_000, n_particles=2000, seed=23, n_obs=30, every=7,
+                             obs_prob=0.5, min_infected=1000, lib=None):
+    """A particle-filter request of the request-stream's shape with a
+    binomial observation process: SIR (S=99, I=5, R=0 per node), beta ~
+    U(0.05, 0.4), gamma ~ U(0.03, 0.2), the infected total observed at
+    days 1 + every*k as Binomial(I, obs_prob) of one trajectory simulated
+    on the GPU with the same parameters.  The reference's filter stops
+    with "Invalid observation process." as soon as one particle's weight
+    is -Inf (R/pfilter.R:437-442), which a binomial observation gives any
+    particle holding fewer infected than observed; so (beta, gamma) are
+    drawn until the simulated infected total stays at or above
+    min_infected at every observation time, where particles near the
+    truth hold several times the observed count."""
+    from .model import run as run_model
+    rng = np.random.default_rng(seed)
+    times = 1.0 + every * np.arange(1, n_obs + 1, dtype=float)
+    u0 = {"S": np.full(Nn, 99), "I": np.full(Nn, 5), "R": np.zeros(Nn, int)}
+    for _ in range(200):
+        beta = float(rng.uniform(0.05, 0.4))
+        gamma = float(rng.uniform(0.03, 0.2))
+        # The truth starts where the filter does (time 1); its first column
+        # is not observed.
+        truth = models.SIR_mparse(u0=u0, tspan=np.r_[1.0, times], beta=beta, gamma=gamma)
+        run_model(truth, seed=int(rng.integers(0, 2**62)), lib=lib)
+        I = truth.U.reshape(Nn, 3, -1)[:, 1, 1:].sum(axis=0)
+        if I.min() >= min_infected:
+            break
+    else:
+        raise RuntimeError("no parameters of the prior keep the infected total above %d"
+                           % min_infected)
+    obs = rng.binomial(I, obs_prob)
+    model = models.SIR_mparse(u0=u0, tspan=np.array([1.0, times[-1]]), beta=beta,
+                              gamma=gamma)
+    return FilterSpec(model, times, obs.reshape(-1, 1), n_particles, obs_prob,
+                      int(rng.integers(0, 2**62)), 0.0, beta, gamma)

@@ -421,3 +421,22 @@ def test_reference_error_cases(oracle):
     m = si.SIR(u0={"S": [9, 9], "I": 1, "R": 0}, tspan=np.arange(1, 5.0),
                beta=0.16, gamma=0.1, replicates=2)
     assert oracle.run(m, solver="aem", seed=1).err == -19
+
+
+@pytest.mark.parametrize("solver", ["ssm", "aem"])
+def test_threads_give_the_same_results(oracle, solver):
+    """The checker's node loops over several OpenMP threads (the
+    reference's static node-block partition) give the one-thread results:
+    every node keeps its own stream."""
+    for m in (W.seir_million(Nn=4000, days=60, seed=3, seeded_frac=0.05),
+              W.sise3_herds(Nn=200, days=80, seed=2)):
+        a = oracle.run(m, solver=solver, seed=4)
+        b = oracle.run(m, solver=solver, seed=4, threads=4)
+        assert a.err == b.err == 0
+        np.testing.assert_array_equal(a.U, b.U)
+        if a.V is not None:
+            np.testing.assert_array_equal(a.V, b.V)
+        assert a.transitions == b.transitions
+    m = si.SIR(u0={"S": [99] * 50, "I": 1, "R": 0}, tspan=np.arange(1, 11.0),
+               beta=0.16, gamma=-0.077)
+    assert oracle.run(m, solver=solver, seed=1, threads=4).err == -11
