@@ -285,6 +285,11 @@ int SimInf_hip_batch_create(SimInf_hip_batch **batch, SimInf_hip_plan *const *pl
  * bit.  Returns the first plan's error code in batch order, or 0. */
 int SimInf_hip_batch_run(SimInf_hip_batch *batch, const unsigned long int *seeds);
 
+/* use_graph 0: SimInf_hip_batch_run enqueues the plans' kernels directly
+ * on their own streams (the same kernels and results) instead of
+ * launching the graph; 1 (the default): the graph. */
+int SimInf_hip_batch_set_graph(SimInf_hip_batch *batch, int use_graph);
+
 /* Device time of the last launch in milliseconds (HIP events around the
  * graph launch). */
 int SimInf_hip_batch_time(const SimInf_hip_batch *batch, double *ms);

@@ -84,6 +84,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_kernel_times",
     "SimInf_hip_batch_create",
     "SimInf_hip_batch_run",
+    "SimInf_hip_batch_set_graph",
     "SimInf_hip_batch_time",
     "SimInf_hip_batch_destroy",
     "SimInf_hip_pfilter",
@@ -156,9 +157,11 @@ def load(path=None):
     lib.SimInf_hip_plan_trajectory_v.restype = ctypes.c_int
     lib.SimInf_hip_plan_trajectory_v.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_double)]
-    lib.SimInf_hip_abc_weights.restype = ctypes.c_int
-    lib.SimInf_hip_abc_weights.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int,
-                                           ctypes.c_int, dp, dp, dp, dp, dp]
+    for name in ("SimInf_hip_abc_weights",):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp,
+                      dp]
     lib.SimInf_hip_plan_set_lanes_per_wave.restype = ctypes.c_int
     lib.SimInf_hip_plan_set_lanes_per_wave.argtypes = [vp, ctypes.c_int]
     lib.SimInf_hip_plan_set_herd_map.restype = ctypes.c_int
@@ -176,6 +179,8 @@ def load(path=None):
                                             ctypes.c_int]
     lib.SimInf_hip_batch_run.restype = ctypes.c_int
     lib.SimInf_hip_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_ulong)]
+    lib.SimInf_hip_batch_set_graph.restype = ctypes.c_int
+    lib.SimInf_hip_batch_set_graph.argtypes = [vp, ctypes.c_int]
     lib.SimInf_hip_batch_time.restype = ctypes.c_int
     lib.SimInf_hip_batch_time.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     lib.SimInf_hip_batch_destroy.restype = None

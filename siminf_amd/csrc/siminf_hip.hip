@@ -1333,6 +1333,7 @@ struct SimInf_hip_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long *seeds = nullptr;   /* device, one per plan */
     double ms = 0.0;
+    bool use_graph = true;
 };
 
 extern "C" void SimInf_hip_batch_destroy(SimInf_hip_batch *B)
@@ -1429,11 +1430,66 @@ extern "C" int SimInf_hip_batch_create(SimInf_hip_batch **out, SimInf_hip_plan *
     return 0;
 }
 
+extern "C" int SimInf_hip_batch_set_graph(SimInf_hip_batch *B, int use_graph)
+{
+    if (!B)
+        return SIMINF_ERR_INVALID_MODEL;
+    B->use_graph = use_graph != 0;
+    return 0;
+}
+
+/* A batch run without the graph: every plan's run enqueued on its own
+ * streams behind one event of the batch's stream, which then waits for
+ * all of them. */
+static int batch_run_streams(SimInf_hip_batch *B, const unsigned long *seeds)
+{
+    const int n = (int)B->plans.size();
+    if (hipEventRecord(B->ev0, B->stream) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    for (int i = 0; i < n; i++) {
+        SimInf_hip_plan *P = B->plans[i];
+        KParams p = P->p;
+        p.seed = (unsigned long long)seeds[i];
+        p.seed_dev = nullptr;
+        bool filled = false;
+        if (hipStreamWaitEvent(P->stream, B->ev0, 0) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        const int err = plan_enqueue(P, p, false, &filled);
+        if (err)
+            return err;
+        B->filled[i] = filled;
+        if (hipEventRecord(B->join[i], P->stream) != hipSuccess ||
+            hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+    }
+    if (hipEventRecord(B->ev1, B->stream) != hipSuccess ||
+        hipStreamSynchronize(B->stream) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    return 0;
+}
+
 extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *seeds)
 {
     if (!B || !seeds)
         return SIMINF_ERR_INVALID_MODEL;
     const int n = (int)B->plans.size();
+    if (!B->use_graph) {
+        for (SimInf_hip_plan *P : B->plans)
+            P->has_run = false;
+        const int err = batch_run_streams(B, seeds);
+        if (err)
+            return err;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, B->ev0, B->ev1);
+        B->ms = ms;
+        int first = 0;
+        for (int i = 0; i < n; i++) {
+            const int code = plan_collect(B->plans[i], B->filled[i] != 0);
+            if (code && !first)
+                first = code;
+        }
+        return first;
+    }
     std::vector<unsigned long long> sd(seeds, seeds + n);
     for (SimInf_hip_plan *P : B->plans)
         P->has_run = false;

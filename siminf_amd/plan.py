@@ -191,6 +191,13 @@ class Batch:
         if err:
             _raise(self.lib, err)
 
+    def set_graph(self, use_graph):
+        """True: each run is one graph launch (the default); False: the
+        same kernels enqueued directly on the plans' streams."""
+        err = self.lib.SimInf_hip_batch_set_graph(self._handle, int(bool(use_graph)))
+        if err:
+            _raise(self.lib, err)
+
     def run(self, seeds):
         """One launch, plan i with seeds[i]; returns the launch's device
         time in milliseconds."""
