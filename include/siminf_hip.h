@@ -389,6 +389,28 @@ int SimInf_hip_abc_weights(const char *distribution, const double *p1,
                            const double *x, const double *xx, const double *w,
                            const double *sigma, double *ww);
 
+/* Bytes of device memory SimInf_hip_abc_weights_noalloc may use. */
+#define SIMINF_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* SimInf_hip_abc_weights without allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * next to graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): it calls no hipMalloc, hipFree or allocating library
+ * routine, and stages its inputs, intermediates and results in a fixed
+ * device array of SIMINF_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends (found with hipGetSymbolAddress on the first call on each
+ * device).  Same arguments, results (bit for bit) and errors as
+ * SimInf_hip_abc_weights, on the current device's legacy default stream,
+ * blocking; SIMINF_ERR_ALLOC_MEMORY_BUFFER when n_particles and
+ * n_parameters need more than the array (x, xx and two transformed copies
+ * take 32 bytes per particle and parameter, plus 20 bytes per particle;
+ * about 250,000 particles of 3 parameters).  Calls that overlap on one
+ * GPU share the array and race; the caller serialises them. */
+int SimInf_hip_abc_weights_noalloc(const char *distribution, const double *p1,
+                                   const double *p2, int n_particles, int n_parameters,
+                                   const double *x, const double *xx, const double *w,
+                                   const double *sigma, double *ww);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_download",
     "SimInf_hip_plan_stats",
     "SimInf_hip_plan_kernel_times",
+    "SimInf_hip_abc_weights_noalloc",
     "SimInf_hip_batch_create",
     "SimInf_hip_batch_run",
     "SimInf_hip_batch_set_graph",
@@ -157,7 +158,7 @@ def load(path=None):
     lib.SimInf_hip_plan_trajectory_v.restype = ctypes.c_int
     lib.SimInf_hip_plan_trajectory_v.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_double)]
-    for name in ("SimInf_hip_abc_weights",):
+    for name in ("SimInf_hip_abc_weights", "SimInf_hip_abc_weights_noalloc"):
         f = getattr(lib, name)
         f.restype = ctypes.c_int
         f.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp,

@@ -2551,3 +2551,114 @@ extern "C" int SimInf_hip_abc_weights(const char *distribution, const double *p1
     (void)hipStreamDestroy(st);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* ABC weights without allocation.                                     */
+/* ------------------------------------------------------------------ */
+
+/* Device memory of SimInf_hip_abc_weights_noalloc: a fixed array, one
+ * copy per GPU, held until the process ends.  Host code never passes the
+ * array's name to a kernel or a copy (on the host it is only a handle to
+ * the symbol): it uses the address hipGetSymbolAddress returns for the
+ * current device. */
+static __device__ __align__(256) unsigned char scratch[SIMINF_NOALLOC_SCRATCH_BYTES];
+
+/* The scratch array's address on device dev (looked up once per device,
+ * with dev current), or nullptr. */
+static unsigned char *scratch_on(int dev)
+{
+    static std::mutex mu;
+    static void *ptr[64];
+    if (dev < 0 || dev >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return nullptr;
+        ptr[dev] = p;
+    }
+    return (unsigned char *)ptr[dev];
+}
+
+/* Bytes of scratch the weights of n particles and d parameters take:
+ * x, xx, zc, zp (n*d doubles each), w, logw (n doubles), L (d*d), p1,
+ * p2, lg (d doubles), kind (d ints), flag (n ints), err (one int), each
+ * at a 256-byte boundary. */
+static size_t abc_scratch_bytes(long long n, long long d)
+{
+    auto a = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t nd = (size_t)(n * d);
+    return 4 * a(nd * 8) + 2 * a((size_t)n * 8) + a((size_t)(d * d) * 8) + 3 * a((size_t)d * 8) +
+           a((size_t)d * 4) + a((size_t)n * 4) + a(4);
+}
+
+extern "C" int SimInf_hip_abc_weights_noalloc(const char *distribution, const double *p1,
+                                              const double *p2, int n_particles,
+                                              int n_parameters, const double *x,
+                                              const double *xx, const double *w,
+                                              const double *sigma, double *ww)
+{
+    const int n = n_particles, d = n_parameters;
+    std::vector<int> kind;
+    std::vector<double> lg, L;
+    double lognorm = 0.0;
+    bool uniform = false;
+    const int perr = abc_prepare(distribution, p1, p2, n, d, x, xx, w, sigma, ww, kind, lg,
+                                 L, &lognorm, &uniform);
+    if (perr || uniform)
+        return perr;
+    if (abc_scratch_bytes(n, d) > (size_t)SIMINF_NOALLOC_SCRATCH_BYTES)
+        return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    unsigned char *base = scratch_on(dev);
+    if (!base)
+        return SIMINF_ERR_DEVICE;
+
+    /* Carve the operands out of the scratch array. */
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *q = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    const size_t nd = (size_t)n * d;
+    double *x_d = (double *)take(nd * 8), *xx_d = (double *)take(nd * 8);
+    double *w_d = (double *)take((size_t)n * 8), *L_d = (double *)take((size_t)d * d * 8);
+    double *p1_d = (double *)take((size_t)d * 8), *p2_d = (double *)take((size_t)d * 8);
+    double *lg_d = (double *)take((size_t)d * 8);
+    int *kind_d = (int *)take((size_t)d * 4);
+    AbcOperands o;
+    o.x = x_d;
+    o.xx = xx_d;
+    o.w = w_d;
+    o.L = L_d;
+    o.p1 = p1_d;
+    o.p2 = p2_d;
+    o.lg = lg_d;
+    o.kind = kind_d;
+    o.zc = (double *)take(nd * 8);
+    o.zp = (double *)take(nd * 8);
+    o.logw = (double *)take((size_t)n * 8);
+    o.flag = (int *)take((size_t)n * 4);
+    o.err = (int *)take(4);
+
+    /* Inputs into scratch on the legacy default stream; err starts at 0
+     * (the previous call's value stays in the array otherwise). */
+    hipStream_t st = nullptr;
+    const int zero = 0;
+    struct Up {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    } ups[] = {{x_d, x, nd * 8},       {xx_d, xx, nd * 8},          {w_d, w, (size_t)n * 8},
+               {L_d, L.data(), L.size() * 8}, {p1_d, p1, (size_t)d * 8},
+               {p2_d, p2, (size_t)d * 8},     {lg_d, lg.data(), (size_t)d * 8},
+               {kind_d, kind.data(), (size_t)d * 4}, {o.err, &zero, 4}};
+    for (const Up &u : ups)
+        if (hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+    return abc_launch(o, n, d, lognorm, ww, st);
+}
