@@ -163,14 +163,29 @@ def bench_batch(args):
     model = W.seir_million()
     solvers = ("ssm", "aem")
     plans = [si.Plan(model, solver=s) for s in solvers]
-    tuned = {}
-    for s, p in zip(solvers, plans):
+    for p in plans:
         for i in range(args.warmup):
             p.run(100 + i)
-        if not args.no_autotune:
-            best, _ = p.autotune(seed=200, reps=2)
-            tuned[s] = {"lanes_per_wave_shift": best}
     batch = si.Batch(plans)
+    tuned = {}
+    if args.shape:
+        # A launch shape given on the command line (as a profiled run
+        # repeats the bench's autotuned one): lanes-per-wavefront shifts
+        # and "graph" or "streams".
+        spec = json.loads(args.shape)
+        for s, p in zip(solvers, plans):
+            p.set_lanes_per_wave(int(spec[s]))
+        batch.close()
+        batch = si.Batch(plans)
+        batch.set_graph(spec.get("launch", "streams") == "graph")
+        tuned = {"lanes_per_wave_shift": {s: int(spec[s]) for s in solvers},
+                 "launch": spec.get("launch", "streams"), "given": True}
+    elif not args.no_autotune:
+        # The launch shape is tuned for the plans together: each plan's
+        # lanes per wavefront and the graph against the direct enqueue.
+        ms, shifts, graph = batch.autotune(seed=200, reps=2)
+        tuned = {"lanes_per_wave_shift": dict(zip(solvers, shifts)),
+                 "launch": "graph" if graph else "streams", "tuned_ms": ms}
     for i in range(args.warmup):
         batch.run([300 + 2 * i, 301 + 2 * i])
     lat, launch_ms, per = [], [], {s: [] for s in solvers}
@@ -453,6 +468,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="seir_batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help='seir_batch: launch shape instead of the autotuner, as JSON '
+                         '{"ssm": shift, "aem": shift, "launch": "graph"|"streams"}')
     ap.add_argument("--no-extras", action="store_true",
                     help="seir_batch: skip the PCIe-inclusive, SISe3 and filter-request "
                          "measurements")

@@ -290,8 +290,23 @@ int SimInf_hip_batch_run(SimInf_hip_batch *batch, const unsigned long int *seeds
  * launching the graph; 1 (the default): the graph. */
 int SimInf_hip_batch_set_graph(SimInf_hip_batch *batch, int use_graph);
 
+/* Autotuner of the batch's launch shape (no reference counterpart): the
+ * plans share the device, so each plan's lanes per wavefront (see
+ * SimInf_hip_plan_set_lanes_per_wave) is chosen by coordinate descent on
+ * the batch's launch time (two passes, every shift of one plan with the
+ * others held, reps launches each, seeds from seed), then the graph or
+ * the direct enqueue, whichever is faster, is kept.  best_ms (optional)
+ * gets the kept shape's mean launch time.  Results do not depend on it. */
+int SimInf_hip_batch_autotune(SimInf_hip_batch *batch, unsigned long int seed, int reps,
+                              double *best_ms);
+
+/* Plan i's kept lanes-per-wavefront shift (-1: the default rule) and 1 if
+ * the batch launches its graph, 0 if it enqueues directly. */
+int SimInf_hip_batch_get_shape(const SimInf_hip_batch *batch, int i, int *shift,
+                               int *use_graph);
+
 /* Device time of the last launch in milliseconds (HIP events around the
- * graph launch). */
+ * graph launch, or around the direct enqueue). */
 int SimInf_hip_batch_time(const SimInf_hip_batch *batch, double *ms);
 
 void SimInf_hip_batch_destroy(SimInf_hip_batch *batch);

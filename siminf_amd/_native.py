@@ -86,6 +86,8 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_batch_create",
     "SimInf_hip_batch_run",
     "SimInf_hip_batch_set_graph",
+    "SimInf_hip_batch_autotune",
+    "SimInf_hip_batch_get_shape",
     "SimInf_hip_batch_time",
     "SimInf_hip_batch_destroy",
     "SimInf_hip_pfilter",
@@ -180,6 +182,12 @@ def load(path=None):
                                             ctypes.c_int]
     lib.SimInf_hip_batch_run.restype = ctypes.c_int
     lib.SimInf_hip_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_ulong)]
+    lib.SimInf_hip_batch_autotune.restype = ctypes.c_int
+    lib.SimInf_hip_batch_autotune.argtypes = [vp, ctypes.c_ulong, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_double)]
+    lib.SimInf_hip_batch_get_shape.restype = ctypes.c_int
+    lib.SimInf_hip_batch_get_shape.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int)]
     lib.SimInf_hip_batch_set_graph.restype = ctypes.c_int
     lib.SimInf_hip_batch_set_graph.argtypes = [vp, ctypes.c_int]
     lib.SimInf_hip_batch_time.restype = ctypes.c_int

@@ -855,10 +855,11 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                 hipSuccess && cus > 0)
             P->simds = 4 * cus;
         p.simds = P->simds;
-        /* The active pass's fixed grid: up to four wavefronts per SIMD
-         * (twice what the default packing needs), never more wavefronts
-         * than lanes. */
-        const long long waves = std::min<long long>(lanes, 4LL * P->simds);
+        /* The active pass's fixed grid: eight wavefronts per SIMD (what
+         * the node kernels' register use lets the device hold, so that a
+         * packing that needs up to that many gives every wave slot a
+         * wavefront of its own), never more wavefronts than lanes. */
+        const long long waves = std::min<long long>(lanes, 8LL * P->simds);
         P->grid_active = (unsigned)((waves * 64 + P->block - 1) / P->block);
     }
     *out = P;
@@ -958,7 +959,7 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
             return SIMINF_ERR_DEVICE;
         mark(P->ev_fill0, P->fill_stream);
         {
-            const dim3 fg(P->grid), fb(P->block);
+            const dim3 fg(fill_grid(P->p.lanes, P->tlen)), fb(256);
             hipStream_t fs = P->fill_stream;
             const bool v = P->Nd > 0;
 #define SIMINF_FILL(NCV)                                                          \
@@ -1363,6 +1364,50 @@ extern "C" void SimInf_hip_batch_destroy(SimInf_hip_batch *B)
     delete B;
 }
 
+/* Capture the batch's graph (replacing a previous one): every plan's run
+ * on its own branch, forked from the batch's stream into the plan's
+ * streams, the plan's kernels with the seed read from the batch's seed
+ * table and its current launch shape, joined back. */
+static int batch_capture(SimInf_hip_batch *B)
+{
+    if (B->exec) {
+        (void)hipGraphExecDestroy(B->exec);
+        B->exec = nullptr;
+    }
+    if (B->graph) {
+        (void)hipGraphDestroy(B->graph);
+        B->graph = nullptr;
+    }
+    const int n = (int)B->plans.size();
+    int err = 0;
+    if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    for (int i = 0; i < n && !err; i++) {
+        SimInf_hip_plan *P = B->plans[i];
+        KParams p = P->p;
+        p.seed_dev = B->seeds + i;
+        bool filled = false;
+        if (hipEventRecord(B->fork[i], B->stream) != hipSuccess ||
+            hipStreamWaitEvent(P->stream, B->fork[i], 0) != hipSuccess) {
+            err = SIMINF_ERR_DEVICE;
+            break;
+        }
+        err = plan_enqueue(P, p, true, &filled);
+        B->filled[i] = filled;
+        if (!err && (hipEventRecord(B->join[i], P->stream) != hipSuccess ||
+                     hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess))
+            err = SIMINF_ERR_DEVICE;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(B->stream, &g);
+    B->graph = g;
+    if (!err && (ec != hipSuccess || !g ||
+                 hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0) != hipSuccess))
+        err = SIMINF_ERR_DEVICE;
+    (void)hipGetLastError();
+    return err;
+}
+
 extern "C" int SimInf_hip_batch_create(SimInf_hip_batch **out, SimInf_hip_plan *const *plans,
                                        int n)
 {
@@ -1391,37 +1436,7 @@ extern "C" int SimInf_hip_batch_create(SimInf_hip_batch **out, SimInf_hip_plan *
         SimInf_hip_batch_destroy(B);
         return SIMINF_ERR_DEVICE;
     }
-    /* Every plan's run on its own branch: fork from the batch's stream
-     * into the plan's streams, the plan's kernels with the seed read from
-     * the batch's seed table, join back. */
-    int err = 0;
-    if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        SimInf_hip_batch_destroy(B);
-        return SIMINF_ERR_DEVICE;
-    }
-    for (int i = 0; i < n && !err; i++) {
-        SimInf_hip_plan *P = B->plans[i];
-        KParams p = P->p;
-        p.seed_dev = B->seeds + i;
-        bool filled = false;
-        if (hipEventRecord(B->fork[i], B->stream) != hipSuccess ||
-            hipStreamWaitEvent(P->stream, B->fork[i], 0) != hipSuccess) {
-            err = SIMINF_ERR_DEVICE;
-            break;
-        }
-        err = plan_enqueue(P, p, true, &filled);
-        B->filled[i] = filled;
-        if (!err && (hipEventRecord(B->join[i], P->stream) != hipSuccess ||
-                     hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess))
-            err = SIMINF_ERR_DEVICE;
-    }
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(B->stream, &g);
-    B->graph = g;
-    if (!err && (ec != hipSuccess || !g ||
-                 hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0) != hipSuccess))
-        err = SIMINF_ERR_DEVICE;
-    (void)hipGetLastError();
+    const int err = batch_capture(B);
     if (err) {
         SimInf_hip_batch_destroy(B);
         return err;
@@ -1512,6 +1527,89 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
             first = code;
     }
     return first;
+}
+
+/* Autotuner of a batch's launch (no reference counterpart): the plans
+ * share the device, so each one's lanes per wavefront is tuned in the
+ * batch rather than alone.  Coordinate descent, two passes: for each plan
+ * with an active pass in turn, every shift (-1 for the default rule, 0
+ * to 6) with the others held, keeping the least mean launch time over
+ * reps launches (seeds from seed); then the graph against the direct
+ * enqueue, the faster kept (the graph recaptured with the kept shapes).
+ * Results do not depend on any of it. */
+extern "C" int SimInf_hip_batch_autotune(SimInf_hip_batch *B, unsigned long int seed, int reps,
+                                         double *best_ms)
+{
+    if (!B || reps < 1)
+        return SIMINF_ERR_INVALID_MODEL;
+    const int n = (int)B->plans.size();
+    std::vector<unsigned long> sd((size_t)n);
+    unsigned long next = seed;
+    auto measure = [&](double *mean) -> int {
+        double sum = 0.0;
+        for (int r = 0; r < reps; r++) {
+            for (int i = 0; i < n; i++)
+                sd[i] = next++;
+            const int err = SimInf_hip_batch_run(B, sd.data());
+            if (err)
+                return err;
+            sum += B->ms;
+        }
+        *mean = sum / reps;
+        return 0;
+    };
+    B->use_graph = false;
+    double best = 0.0;
+    int err = measure(&best);
+    for (int pass = 0; pass < 2 && !err; pass++) {
+        for (int i = 0; i < n && !err; i++) {
+            SimInf_hip_plan *P = B->plans[i];
+            if ((P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES) || !P->p.frozen_from)
+                continue;
+            const int keep0 = P->lpw_fixed;
+            int keep = keep0;
+            for (int shift = -1; shift <= 6 && !err; shift++) {
+                if (shift == keep0)
+                    continue;
+                P->lpw_fixed = shift;
+                double t = 0.0;
+                err = measure(&t);
+                if (!err && t < best) {
+                    best = t;
+                    keep = shift;
+                }
+            }
+            P->lpw_fixed = keep;
+        }
+    }
+    if (!err)
+        err = batch_capture(B);
+    if (!err) {
+        B->use_graph = true;
+        double tg = 0.0;
+        err = measure(&tg);
+        if (!err && tg < best)
+            best = tg;
+        else
+            B->use_graph = false;
+    }
+    if (best_ms)
+        *best_ms = best;
+    return err;
+}
+
+/* The lanes-per-wavefront shift the batch's autotuner kept for plan i
+ * (-1: the default rule) and whether it launches the graph. */
+extern "C" int SimInf_hip_batch_get_shape(const SimInf_hip_batch *B, int i, int *shift,
+                                          int *use_graph)
+{
+    if (!B || i < 0 || i >= (int)B->plans.size())
+        return SIMINF_ERR_INVALID_MODEL;
+    if (shift)
+        *shift = B->plans[i]->lpw_fixed;
+    if (use_graph)
+        *use_graph = B->use_graph ? 1 : 0;
+    return 0;
 }
 
 extern "C" int SimInf_hip_batch_time(const SimInf_hip_batch *B, double *ms)

@@ -15,6 +15,7 @@
 #include "siminf_device.h"
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 namespace siminf {
 
@@ -958,24 +959,55 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     add_transitions(p, fired);
 }
 
-/* The columns frozen nodes left unwritten (node kernels, dense U/V):
- * one lane per (replicate, node), looping over the columns in order, so
- * that at any time the workgroups write neighbouring stretches of one
- * column, as a stream.  NC > 0 gives compile-time compartment counts
- * (the stores merge into wide ones); NC == 0 reads the count from nc.
- * Without V (HASV false) the loop holds stores only, so none of them
- * waits for another. */
-template <int NC, bool HASV>
-__global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int nc, int nd)
+/* Columns per workgroup of the frozen-node fill. */
+#define SIMINF_FILL_COLS 8
+
+/* Non-temporal stores of a node's NC ints (one vector store for 2 or 4
+ * compartments): the trajectory is written once and not read back by
+ * this run, so the lines need not stay in the caches. */
+template <int NC>
+__device__ __forceinline__ void store_nt(int *dst, const int (&u)[NC])
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    typedef int v2i __attribute__((ext_vector_type(2)));
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    if constexpr (NC == 4) {
+        const v4i x = {u[0], u[1], u[2], u[3]};
+        __builtin_nontemporal_store(x, (v4i *)dst);
+    } else if constexpr (NC == 2) {
+        const v2i x = {u[0], u[1]};
+        __builtin_nontemporal_store(x, (v2i *)dst);
+    } else {
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            __builtin_nontemporal_store(u[c], dst + c);
+        });
+    }
+}
+
+/* The columns frozen nodes left unwritten (node kernels, dense U/V).
+ * Workgroup b covers 256 lanes (replicate * Nn + node) of lane block
+ * b % nb and the SIMINF_FILL_COLS columns of column block b / nb, so the
+ * workgroups resident together write neighbouring stretches of the same
+ * few columns (U column-major, node-major within a column: one
+ * wavefront's store is 64 consecutive nodes' counts, contiguous).  NC > 0
+ * gives compile-time compartment counts; NC == 0 reads the count from
+ * nc.  HASV adds V (nd doubles per node-column). */
+template <int NC, bool HASV>
+__device__ __forceinline__ void fill_block(const KParams &p, int nc, int nd, long long b)
+{
+    const long long nb = (p.lanes + 255) / 256;
+    const long long lb = b % nb, cb = b / nb;
+    const long long g = lb * 256 + threadIdx.x;
     if (g >= p.lanes)
+        return;
+    const int from = p.frozen_from[g];
+    const int c0 = (int)(cb * SIMINF_FILL_COLS);
+    const int c1 = min(c0 + SIMINF_FILL_COLS, p.tlen);
+    const int first = max(from, c0);
+    if (first >= c1)
         return;
     const int rep = (int)(g / p.Nn);
     const int node = (int)(g - (long long)rep * p.Nn);
-    const int from = p.frozen_from[g];
-    if (from >= p.tlen)
-        return;
     const int Nc = NC > 0 ? NC : nc;
     int u[NC > 0 ? NC : 1];
     if constexpr (NC > 0) {
@@ -985,22 +1017,40 @@ __global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int nc, int nd)
         });
     }
     const long long stride = (long long)p.Nn;
-    long long q = ((long long)rep * p.tlen + from) * stride + node;
-    for (int it = from; it < p.tlen; it++, q += stride) {
+    long long q = ((long long)rep * p.tlen + first) * stride + node;
+    for (int it = first; it < c1; it++, q += stride) {
         if constexpr (NC > 0) {
-            static_for<NC>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                p.U[q * NC + c] = u[c];
-            });
+            store_nt<NC>(p.U + q * NC, u);
         } else {
             for (int c = 0; c < Nc; c++)
-                p.U[q * Nc + c] = p.frozen_u[g * Nc + c];
+                __builtin_nontemporal_store(p.frozen_u[g * Nc + c], p.U + q * Nc + c);
         }
         if constexpr (HASV) {
             for (int c = 0; c < nd; c++)
-                p.V[q * nd + c] = p.frozen_v[g * nd + c];
+                __builtin_nontemporal_store(p.frozen_v[g * nd + c], p.V + q * nd + c);
         }
     }
+}
+
+/* The fill's work is (lanes / 256) x (tlen / SIMINF_FILL_COLS) blocks;
+ * a launch of at most 2^20 workgroups takes them in order, each
+ * workgroup b, b + gridDim.x, ... */
+__host__ __device__ inline long long fill_blocks(long long lanes, int tlen)
+{
+    return ((lanes + 255) / 256) * ((tlen + SIMINF_FILL_COLS - 1) / SIMINF_FILL_COLS);
+}
+
+template <int NC, bool HASV>
+__global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int nc, int nd)
+{
+    const long long n = fill_blocks(p.lanes, p.tlen);
+    for (long long b = blockIdx.x; b < n; b += gridDim.x)
+        fill_block<NC, HASV>(p, nc, nd, b);
+}
+
+inline unsigned fill_grid(long long lanes, int tlen)
+{
+    return (unsigned)std::min<long long>(fill_blocks(lanes, tlen), 1LL << 20);
 }
 
 /* Layout version of SimInf_hip_model, checked when a plugin is used. */

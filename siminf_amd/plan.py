@@ -198,6 +198,23 @@ class Batch:
         if err:
             _raise(self.lib, err)
 
+    def autotune(self, seed=1, reps=2):
+        """Tune each plan's lanes per wavefront in the batch and the graph
+        against the direct enqueue; returns (mean launch ms of the kept
+        shape, [shift per plan], uses_graph)."""
+        ms = ctypes.c_double()
+        err = self.lib.SimInf_hip_batch_autotune(self._handle, int(seed), int(reps),
+                                                 ctypes.byref(ms))
+        if err:
+            _raise(self.lib, err)
+        shifts, g = [], ctypes.c_int()
+        for i in range(len(self.plans)):
+            sh = ctypes.c_int()
+            self.lib.SimInf_hip_batch_get_shape(self._handle, i, ctypes.byref(sh),
+                                                ctypes.byref(g))
+            shifts.append(sh.value)
+        return ms.value, shifts, bool(g.value)
+
     def run(self, seeds):
         """One launch, plan i with seeds[i]; returns the launch's device
         time in milliseconds."""

@@ -301,17 +301,27 @@ def pfilter_binomial_request(Nn=10_000, n_particles=2000, seed=23, n_obs=30, eve
     min_infected at every observation time, where particles near the
     truth hold several times the observed count."""
     from .model import run as run_model
+    from .particle_filter import pfilter_tspan
     rng = np.random.default_rng(seed)
     times = 1.0 + every * np.arange(1, n_obs + 1, dtype=float)
     u0 = {"S": np.full(Nn, 99), "I": np.full(Nn, 5), "R": np.zeros(Nn, int)}
+    intervals = pfilter_tspan(1.0, times)
     for _ in range(200):
         beta = float(rng.uniform(0.05, 0.4))
         gamma = float(rng.uniform(0.03, 0.2))
-        # The truth starts where the filter does (time 1); its first column
-        # is not observed.
-        truth = models.SIR_mparse(u0=u0, tspan=np.r_[1.0, times], beta=beta, gamma=gamma)
-        run_model(truth, seed=int(rng.integers(0, 2**62)), lib=lib)
-        I = truth.U.reshape(Nn, 3, -1)[:, 1, 1:].sum(axis=0)
+        # The truth runs the filter's intervals one after another, each
+        # from the previous one's final state (as a particle does), so
+        # that it follows the filter's model exactly.
+        state = models.SIR_mparse(u0=u0, tspan=[1.0], beta=beta, gamma=gamma).u0
+        I = []
+        for tb, te in intervals:
+            ts = [te] if np.isnan(tb) else [tb, te]
+            step = models.SIR_mparse(u0={"S": state[0], "I": state[1], "R": state[2]},
+                                     tspan=ts, beta=beta, gamma=gamma)
+            run_model(step, seed=int(rng.integers(0, 2**62)), lib=lib)
+            state = step.U[:, -1].reshape(Nn, 3).T
+            I.append(int(state[1].sum()))
+        I = np.asarray(I)
         if I.min() >= min_infected:
             break
     else:

@@ -120,3 +120,28 @@ def test_grid_coupled_plans_are_refused():
     with si.Plan(m2) as q:
         with pytest.raises(si.SimInfError, match="Invalid model."):
             si.Batch([q, q])
+
+
+def test_batch_autotune_keeps_results(oracle):
+    """The batch autotuner changes launch shapes only: after it, every
+    plan's trajectory equals its own run with the same seed, under the
+    kept launch (graph or direct enqueue)."""
+    models = [(W.seir_million(Nn=50_000, days=80, seed=5, seeded_frac=0.01), "ssm"),
+              (W.seir_million(Nn=50_000, days=80, seed=5, seeded_frac=0.01), "aem"),
+              (W.mparse_sir_readme(Nn=500, days=60), "ssm")]
+    plans = [si.Plan(m, solver=s) for m, s in models]
+    try:
+        with si.Batch(plans) as b:
+            ms, shifts, graph = b.autotune(seed=40, reps=1)
+            print("autotuned", ms, shifts, graph, flush=True)
+            assert ms > 0 and len(shifts) == 3
+            assert all(-1 <= x <= 6 for x in shifts)
+            for use_graph in (graph, not graph):
+                b.set_graph(use_graph)
+                b.run([21, 22, 23])
+                got = [_copy_u(p) + (_stats(p)[0],) for p in plans]
+                for a, c in zip(_separate(models, (21, 22, 23)), got):
+                    _same(a, c)
+    finally:
+        for p in plans:
+            p.close()
