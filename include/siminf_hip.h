@@ -227,6 +227,18 @@ int SimInf_hip_plan_stats(const SimInf_hip_plan *plan,
 int SimInf_hip_plan_kernel_times(const SimInf_hip_plan *plan, double *solver_ms,
                                  double *fill_ms, long long *fill_columns);
 
+/* Parameters per replicate (no reference counterpart; the reference's
+ * replicates share ldata and gdata): ldata, if not NULL, holds Nld x
+ * ldata_nodes values for each of the plan's Nrep replicates in turn:
+ * ldata_nodes Nn gives every node of every replicate its own column
+ * (column replicate * Nn + node), 1 gives all nodes of a replicate the
+ * same Nld values; gdata, if not NULL, holds the model's gdata values for
+ * each replicate in turn.  Later runs read them; a batch of proposals
+ * (ABC-SMC, PMCMC) then runs as one plan of Nrep replicates.  The first
+ * call allocates device memory, later ones copy into it. */
+int SimInf_hip_plan_set_replicate_data(SimInf_hip_plan *plan, const double *ldata,
+                                       int ldata_nodes, const double *gdata);
+
 /* Lanes per wavefront of the node kernels' active pass, 2^shift
  * (shift 0..6), or -1 for the default rule (about two wavefronts per
  * SIMD).  Results do not depend on it: every lane keeps its own stream

@@ -61,6 +61,8 @@ def load():
         lib.oracle_abc_weights.restype = ctypes.c_int
         lib.oracle_abc_weights.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int,
                                            ctypes.c_int, dp, dp, dp, dp, dp]
+        lib.oracle_set_replicate_data.restype = None
+        lib.oracle_set_replicate_data.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int]
         lib.oracle_set_threads.restype = None
         lib.oracle_set_threads.argtypes = [ctypes.c_int]
         lib.oracle_resample.restype = ctypes.c_int
@@ -99,14 +101,32 @@ def _set_plugin(lib, model, b):
         b.args.pts_fun = ctypes.c_void_p.in_dll(h, "siminf_pts_fun")
 
 
-def run(model, solver="ssm", seed=0, crn=False, threads=1):
+def run(model, solver="ssm", seed=0, crn=False, threads=1, rep_ldata=None,
+        rep_gdata=None):
     """Run the checker on the same SimInf_solver_args the GPU path gets;
     threads > 1 runs the node loops over that many OpenMP threads (node
-    blocks, the reference's partition; the same results)."""
+    blocks, the reference's partition; the same results).  rep_ldata
+    (Nld x Nn*Nrep) and rep_gdata (Ngdata x Nrep) give every replicate its
+    own parameters, as Plan.set_replicate_data does."""
     lib = load()
     b = _native.ArgsBundle(model, solver=0 if solver == "ssm" else 1,
                            crn=1 if crn else 0, seed=int(seed))
     _set_plugin(lib, model, b)
+    dp = ctypes.POINTER(ctypes.c_double)
+    keep = []
+
+    def arr(x, rows):
+        if x is None:
+            return None, 0
+        a = np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(rows, -1, order="F"))
+        keep.append(a)
+        return a.ctypes.data_as(dp), rows
+    lp, _ = arr(rep_ldata, model.Nld)
+    lnodes = 0
+    if rep_ldata is not None:
+        lnodes = model.Nn if np.size(rep_ldata) == model.Nld * model.Nn * model.replicates else 1
+    gp, gs = arr(rep_gdata, len(model.gdata))
+    lib.oracle_set_replicate_data(lp, lnodes, gp, gs)
     lib.oracle_set_threads(int(threads))
     try:
         t0 = time.perf_counter()
@@ -114,6 +134,7 @@ def run(model, solver="ssm", seed=0, crn=False, threads=1):
         seconds = time.perf_counter() - t0
     finally:
         lib.oracle_set_threads(1)
+        lib.oracle_set_replicate_data(None, 0, None, 0)
     r = OracleResult(err, b, lib.oracle_last_transitions(), lib.oracle_aem_ties())
     r.seconds = seconds      # the solver call alone
     return r

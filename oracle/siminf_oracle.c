@@ -74,6 +74,44 @@ void oracle_set_threads(int n)
     oracle_threads = n < 1 ? 1 : n;
 }
 
+/* Parameters per replicate, as the library's plans take them: ldata
+ * Nld x Nn for each replicate in turn, gdata gstride values for each
+ * replicate in turn (NULL: the arguments' shared ones). */
+static const double *oracle_rep_ldata, *oracle_rep_gdata;
+static int oracle_rep_gstride, oracle_rep_lnodes;
+static double *oracle_rep_lbuf;
+
+/* ldata_nodes: Nn for a column per node of every replicate, 1 for one
+ * column per replicate shared by its nodes. */
+void oracle_set_replicate_data(const double *ldata, int ldata_nodes, const double *gdata,
+                               int gstride)
+{
+    oracle_rep_ldata = ldata;
+    oracle_rep_lnodes = ldata_nodes;
+    oracle_rep_gdata = gdata;
+    oracle_rep_gstride = gstride;
+}
+
+/* The arguments a replicate runs with: its own parameters if set. */
+static SimInf_solver_args oracle_replicate_args(const SimInf_solver_args *a, int rep)
+{
+    SimInf_solver_args r = *a;
+    if (oracle_rep_ldata && oracle_rep_lnodes == 1) {
+        /* The replicate's values spread over its nodes. */
+        free(oracle_rep_lbuf);
+        oracle_rep_lbuf = malloc((size_t)a->Nn * a->Nld * sizeof(double) + 8);
+        for (int i = 0; i < a->Nn; i++)
+            memcpy(oracle_rep_lbuf + (ptrdiff_t)i * a->Nld,
+                   oracle_rep_ldata + (ptrdiff_t)rep * a->Nld, (size_t)a->Nld * sizeof(double));
+        r.ldata = oracle_rep_lbuf;
+    } else if (oracle_rep_ldata) {
+        r.ldata = oracle_rep_ldata + (ptrdiff_t)rep * a->Nn * a->Nld;
+    }
+    if (oracle_rep_gdata)
+        r.gdata = oracle_rep_gdata + (ptrdiff_t)rep * oracle_rep_gstride;
+    return r;
+}
+
 /* ---------------------------------------------------------------- */
 /* Sampling (SimInf_solver.c:62-324).                                */
 /* ---------------------------------------------------------------- */
@@ -595,7 +633,8 @@ static int oracle_pts_node(SimInf_solver_args *a, int node, int Nc, int Nt, int 
 
 static int oracle_direct_replicate(oracle_ctx *c, int rep, int is_mssm)
 {
-    SimInf_solver_args *a = c->a;
+    SimInf_solver_args args = oracle_replicate_args(c->a, rep);
+    SimInf_solver_args *a = &args;
     const int Nn = c->Nn, Nc = c->Nc, Nt = c->Nt, Nd = c->Nd, Nld = c->Nld;
     int *u = malloc((size_t)Nn * Nc * sizeof(int) + 1);
     double *v = malloc((size_t)Nn * Nd * sizeof(double) + 8);
@@ -924,7 +963,8 @@ static int oracle_aem_pts_node(SimInf_solver_args *a, int node, int Nc, int Nt, 
 
 static int oracle_aem(oracle_ctx *c)
 {
-    SimInf_solver_args *a = c->a;
+    SimInf_solver_args args = oracle_replicate_args(c->a, 0);
+    SimInf_solver_args *a = &args;
     const int Nn = c->Nn, Nc = c->Nc, Nt = c->Nt, Nd = c->Nd, Nld = c->Nld;
     int *u = malloc((size_t)Nn * Nc * sizeof(int) + 1);
     double *v = malloc((size_t)Nn * Nd * sizeof(double) + 8);

@@ -82,6 +82,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_download",
     "SimInf_hip_plan_stats",
     "SimInf_hip_plan_kernel_times",
+    "SimInf_hip_plan_set_replicate_data",
     "SimInf_hip_abc_weights_noalloc",
     "SimInf_hip_batch_create",
     "SimInf_hip_batch_run",
@@ -165,6 +166,8 @@ def load(path=None):
         f.restype = ctypes.c_int
         f.argtypes = [ctypes.c_char_p, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp,
                       dp]
+    lib.SimInf_hip_plan_set_replicate_data.restype = ctypes.c_int
+    lib.SimInf_hip_plan_set_replicate_data.argtypes = [vp, dp, ctypes.c_int, dp]
     lib.SimInf_hip_plan_set_lanes_per_wave.restype = ctypes.c_int
     lib.SimInf_hip_plan_set_lanes_per_wave.argtypes = [vp, ctypes.c_int]
     lib.SimInf_hip_plan_set_herd_map.restype = ctypes.c_int

@@ -81,6 +81,12 @@ struct KParams {
     const double *ldata;             /* Nld x Nn, or {INFINITY} */
     const double *gdata;
     const double *tspan;
+    /* Parameters per replicate (batches of proposals): ld_rep 1 when
+     * ldata holds Nld x Nn values for each replicate in turn, 2 when it
+     * holds Nld values per replicate for all its nodes (0: Nld x Nn shared
+     * by the replicates); gd_stride the gdata values per replicate (0: one
+     * gdata for all). */
+    int ld_rep, gd_stride;
 
     /* S and G: Smask[k] has bit c set when S[c,k] is stored, Sval[k*NC+c]
      * holds its value; Gmask[k] has bit j set when G[j,k] is stored. */
@@ -292,16 +298,19 @@ struct Ctx {
     int sval[CANON ? 1 : M::NT * M::NC];
     unsigned long long seed;
 
-    __device__ __forceinline__ void load(const KParams &p, int node)
+    __device__ __forceinline__ void load(const KParams &p, int node, int rep)
     {
-        const double *l = p.ldata + (long long)node * p.Nld;
+        const long long li = p.ld_rep == 2 ? (long long)rep
+                             : (p.ld_rep ? (long long)rep * p.Nn : 0LL) + node;
+        const double *l = p.ldata + li * (long long)p.Nld;
+        const double *gsrc = p.gdata + (long long)rep * p.gd_stride;
         static_for<(M::NLD > 0 ? M::NLD : 1)>([&](auto I) {
             constexpr int i = decltype(I)::value;
             ld[i] = i < p.Nld ? l[i] : INFINITY;
         });
         static_for<(M::NGD > 0 ? M::NGD : 1)>([&](auto I) {
             constexpr int i = decltype(I)::value;
-            gd[i] = i < M::NGD ? p.gdata[i] : 0.0;
+            gd[i] = i < M::NGD ? gsrc[i] : 0.0;
         });
         if constexpr (!CANON) {
             static_for<M::NT>([&](auto K) {

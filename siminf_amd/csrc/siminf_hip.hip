@@ -218,6 +218,14 @@ struct SimInf_hip_plan {
      * hold, and the replicates a launch runs under the current map. */
     int max_blocks = 0, reps_per_launch = 1;
 
+    /* Parameters per replicate (SimInf_hip_plan_set_replicate_data):
+     * device buffers allocated on first use, and a count of the calls
+     * that changed where the kernels read them (a batch recaptures its
+     * graph when it changes). */
+    double *rep_ldata = nullptr, *rep_gdata = nullptr;
+    unsigned param_version = 0;
+    int Ngdata = 0, Nld = 0;
+
     /* Last run. */
     bool has_run = false;       /* a run completed without error */
     unsigned long long transitions = 0;
@@ -359,6 +367,8 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     P->tlen = a->tlen;
     P->dense_U = a->U != nullptr;
     P->dense_V = a->V != nullptr;
+    P->Ngdata = model->Ngdata;
+    P->Nld = a->Nld;
     P->host_err_key = NO_ERROR_KEY;
 
     /* S, G as bit masks and a dense value table. */
@@ -1183,6 +1193,42 @@ extern "C" int SimInf_hip_plan_get_herd_map(const SimInf_hip_plan *P, int *n_sol
     return 0;
 }
 
+extern "C" int SimInf_hip_plan_set_replicate_data(SimInf_hip_plan *P, const double *ldata,
+                                                  int ldata_nodes, const double *gdata)
+{
+    if (!P || (ldata && ldata_nodes != P->Nn && ldata_nodes != 1))
+        return SIMINF_ERR_INVALID_MODEL;
+    const size_t nl = (size_t)P->Nld * (ldata ? ldata_nodes : 0) * P->Nrep;
+    const size_t ng = (size_t)P->Ngdata * P->Nrep;
+    if (ldata && nl == 0)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (gdata && ng == 0)
+        return SIMINF_ERR_INVALID_MODEL;
+    DeviceBuffers &B = P->buf;
+    if (ldata) {
+        const int mode = ldata_nodes == P->Nn ? 1 : 2;
+        if (!P->rep_ldata || mode != P->p.ld_rep) {
+            /* Room for the per-node layout, which also holds the other. */
+            if (!P->rep_ldata)
+                P->rep_ldata = B.alloc<double>((size_t)P->Nld * P->Nn * P->Nrep);
+            P->param_version++;
+        }
+        B.copy(P->rep_ldata, ldata, nl * sizeof(double), hipMemcpyHostToDevice);
+        P->p.ldata = P->rep_ldata;
+        P->p.ld_rep = mode;
+    }
+    if (gdata) {
+        if (!P->rep_gdata) {
+            P->rep_gdata = B.alloc<double>(ng);
+            P->param_version++;
+        }
+        B.copy(P->rep_gdata, gdata, ng * sizeof(double), hipMemcpyHostToDevice);
+        P->p.gdata = P->rep_gdata;
+        P->p.gd_stride = P->Ngdata;
+    }
+    return B.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
+}
+
 extern "C" int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *P, int shift)
 {
     if (!P || shift < -1 || shift > 6)
@@ -1335,6 +1381,8 @@ struct SimInf_hip_batch {
     unsigned long long *seeds = nullptr;   /* device, one per plan */
     double ms = 0.0;
     bool use_graph = true;
+    /* The plans' parameter versions the graph was captured with. */
+    std::vector<unsigned> versions;
 };
 
 extern "C" void SimInf_hip_batch_destroy(SimInf_hip_batch *B)
@@ -1380,6 +1428,9 @@ static int batch_capture(SimInf_hip_batch *B)
     }
     const int n = (int)B->plans.size();
     int err = 0;
+    B->versions.assign(n, 0);
+    for (int i = 0; i < n; i++)
+        B->versions[i] = B->plans[i]->param_version;
     if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return SIMINF_ERR_DEVICE;
     for (int i = 0; i < n && !err; i++) {
@@ -1508,6 +1559,16 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
     std::vector<unsigned long long> sd(seeds, seeds + n);
     for (SimInf_hip_plan *P : B->plans)
         P->has_run = false;
+    /* A plan whose kernels now read their parameters elsewhere: capture
+     * again. */
+    for (int i = 0; i < n; i++) {
+        if (B->plans[i]->param_version != B->versions[i]) {
+            const int err = batch_capture(B);
+            if (err)
+                return err;
+            break;
+        }
+    }
     if (hipMemcpyAsync(B->seeds, sd.data(), sizeof(sd[0]) * n, hipMemcpyHostToDevice,
                        B->stream) != hipSuccess ||
         hipEventRecord(B->ev0, B->stream) != hipSuccess ||

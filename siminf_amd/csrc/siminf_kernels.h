@@ -113,7 +113,7 @@ __device__ __forceinline__ void ssm_lane(const KParams &p, long long g)
     bool going = false;
 
     if (live) {
-        cx.load(p, node);
+        cx.load(p, node, rep);
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
             s.u[c] = p.u0[g * NC + c];
@@ -249,7 +249,7 @@ __device__ __forceinline__ void aem_lane(const KParams &p, long long g)
     if (g >= 0) {
         const int node = (int)g;
         Ctx<M, CANON> cx;
-        cx.load(p, node);
+        cx.load(p, node, 0);
         const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
         AemState<M> s;
         int err = 0;
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     };
 
     if (active) {
-        cx.load(p, node);
+        cx.load(p, node, rep);
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
             s.u[c] = p.u0[g * NC + c];

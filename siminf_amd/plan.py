@@ -61,6 +61,28 @@ class Plan:
                                        ctypes.byref(ms))
         return tr.value, nd.value, ms.value
 
+    def set_replicate_data(self, ldata=None, gdata=None):
+        """Parameters per replicate: ldata (Nld x Nn*Nrep, column
+        replicate*Nn + node, or Nld x Nrep for the same values on every
+        node of a replicate) and/or gdata (Ngdata x Nrep); later runs read
+        them."""
+        dp = ctypes.POINTER(ctypes.c_double)
+        keep = []
+
+        def arr(x, shape):
+            if x is None:
+                return None
+            a = np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(shape, order="F"))
+            keep.append(a)
+            return a.ctypes.data_as(dp)
+        m = self.model
+        nodes = 0
+        if ldata is not None:
+            nodes = m.Nn if np.size(ldata) == m.Nld * m.Nn * m.replicates else 1
+        lp = arr(ldata, (m.Nld, nodes * m.replicates))
+        gp = arr(gdata, (len(m.gdata), m.replicates))
+        self._check(self.lib.SimInf_hip_plan_set_replicate_data(self._handle, lp, nodes, gp))
+
     def set_lanes_per_wave(self, shift):
         """2^shift lanes per wavefront in the node kernels' active pass (-1:
         the default rule)."""

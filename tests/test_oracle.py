@@ -440,3 +440,36 @@ def test_threads_give_the_same_results(oracle, solver):
     m = si.SIR(u0={"S": [99] * 50, "I": 1, "R": 0}, tspan=np.arange(1, 11.0),
                beta=0.16, gamma=-0.077)
     assert oracle.run(m, solver=solver, seed=1, threads=4).err == -11
+
+
+def test_replicate_parameters(oracle):
+    """Parameters per replicate: the same values in every replicate give
+    the shared-parameter run bit for bit; a replicate given zero rates
+    keeps its initial state."""
+    m = W.mparse_sir_readme(Nn=20, days=40, replicates=5)
+    a = oracle.run(m, seed=3)
+    g = np.tile(m.gdata.reshape(-1, 1), (1, 5))
+    b = oracle.run(m, seed=3, rep_gdata=g)
+    np.testing.assert_array_equal(a.U, b.U)
+    g[:, 2] = 0.0
+    c = oracle.run(m, seed=3, rep_gdata=g)
+    # U column r * tlen + t.
+    U = c.U.reshape(60, 40, 5, order="F")
+    assert (U[:, :, 2] == m.u0[:, :20].ravel(order="F")[:, None]).all()
+    np.testing.assert_array_equal(U[:, :, 1], a.U.reshape(60, 40, 5, order="F")[:, :, 1])
+    s = si.SEIR(u0={"S": np.arange(100, 130), "E": 0, "I": 3, "R": 0},
+                tspan=np.arange(1, 31.0), beta=0.3, epsilon=0.2, gamma=0.1, replicates=4)
+    a = oracle.run(s, seed=8)
+    ld = np.tile(s.ldata, (1, 4))
+    b = oracle.run(s, seed=8, rep_ldata=ld)
+    np.testing.assert_array_equal(a.U, b.U)
+    ld[:, 30:60] = 0.0
+    c = oracle.run(s, seed=8, rep_ldata=ld)
+    U = c.U.reshape(30 * 4, 30, 4, order="F")
+    assert (U[:, :, 1] == s.u0[:, :30].ravel(order="F")[:, None]).all()
+    # One column per replicate for all its nodes: the same as spelling it
+    # out per node.
+    per = np.array([[0.3, 0.2, 0.1], [0.0, 0.0, 0.0], [0.5, 0.1, 0.2], [0.2, 0.3, 0.05]]).T
+    d = oracle.run(s, seed=8, rep_ldata=per)
+    e = oracle.run(s, seed=8, rep_ldata=np.repeat(per, 30, axis=1))
+    np.testing.assert_array_equal(d.U, e.U)
