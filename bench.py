@@ -663,3 +663,9 @@ def main():
 
 if __name__ == "__main__":
     main()
+    # Every plan is destroyed by now: end the device work and return the
+    # pooled memory before the interpreter's exit handlers run.
+    if "siminf_amd._native" in sys.modules:
+        from siminf_amd import _native
+        if _native._lib is not None:
+            _native._lib.SimInf_hip_release()

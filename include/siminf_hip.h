@@ -278,6 +278,12 @@ int SimInf_hip_plan_autotune(SimInf_hip_plan *plan, unsigned long int seed, int 
 
 void SimInf_hip_plan_destroy(SimInf_hip_plan *plan);
 
+/* Wait for all device work and return the memory that destroyed plans
+ * left in the stream-ordered pool to the device (no reference
+ * counterpart; for a caller about to exit or to hand the device to other
+ * code).  Call it after every plan and batch has been destroyed. */
+int SimInf_hip_release(void);
+
 /* Batches (no reference counterpart): the runs of several plans -- any
  * models, solvers, node counts and replicate counts -- captured once in a
  * graph whose branches run concurrently, then launched as one graph per

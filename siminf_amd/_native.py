@@ -84,6 +84,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_kernel_times",
     "SimInf_hip_plan_set_replicate_data",
     "SimInf_hip_abc_weights_noalloc",
+    "SimInf_hip_release",
     "SimInf_hip_batch_create",
     "SimInf_hip_batch_run",
     "SimInf_hip_batch_set_graph",
@@ -180,6 +181,8 @@ def load(path=None):
     lib.SimInf_hip_plan_autotune.argtypes = [vp, ctypes.c_ulong, ctypes.c_int, ip, dp]
     lib.SimInf_hip_plan_destroy.restype = None
     lib.SimInf_hip_plan_destroy.argtypes = [vp]
+    lib.SimInf_hip_release.restype = ctypes.c_int
+    lib.SimInf_hip_release.argtypes = []
     lib.SimInf_hip_batch_create.restype = ctypes.c_int
     lib.SimInf_hip_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp),
                                             ctypes.c_int]

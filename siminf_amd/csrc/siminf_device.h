@@ -810,6 +810,196 @@ __device__ __forceinline__ int aem_day(AemState<M> &s, const CT &cx,
     return err;
 }
 
+/* calcTimes (aem.c:61-86) for channel J with the logarithm of the
+ * channel's next uniform given (logu: siminf_log of the uniform at
+ * position s.j[J] of the channel's stream, used only if the channel
+ * draws). */
+template <class M, int J>
+__device__ __forceinline__ void calc_times_pre(AemState<M> &s, double logu, double tt,
+                                               double old_rate, double new_rate)
+{
+    const double oldtime = s.time[J];
+    if (isinf(oldtime)) {
+        if (s.inf[J] == 0.0) {
+            s.time[J] = -logu / new_rate + tt;
+            s.j[J] = s.j[J] + 1;
+        } else if (new_rate > 0.0) {
+            s.time[J] = tt + (s.inf[J] / new_rate);
+        }
+    } else if (new_rate >= 2.2250738585072014e-308) {
+        if (oldtime == tt) {
+            s.time[J] = -logu / new_rate + tt;
+            s.j[J] = s.j[J] + 1;
+        } else {
+            s.time[J] = ((old_rate / new_rate) * (oldtime - tt)) + tt;
+        }
+    } else {
+        s.inf[J] = (oldtime - tt) * old_rate;
+        s.time[J] = INFINITY;
+    }
+}
+
+/* aem_fire with every channel's next logarithm given (lv). */
+template <class M, int K, class CT>
+__device__ __forceinline__ int aem_fire_pre(AemState<M> &s, const CT &cx, double t,
+                                            const double (&lv)[M::NT])
+{
+    int err = 0;
+    static_for<M::NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        if (cx.template s_stored<K, c>()) {
+            s.u[c] += cx.template s_val<K, c>();
+            if (s.u[c] < 0)
+                err = SIMINF_ERR_NEGATIVE_STATE;
+        }
+    });
+    static_for<M::NT>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        if (j != K && cx.template g_stored<K, j>()) {
+            const double old = s.rate[j];
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, t);
+            s.rate[j] = r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+            calc_times_pre<M, j>(s, lv[j], t, old, r);
+        }
+    });
+    {
+        const double old = s.rate[K];
+        const double r = M::template rate<K>(s.u, s.v, cx.ld, cx.gd, t);
+        s.rate[K] = r;
+        if (!rate_ok(r))
+            err = SIMINF_ERR_INVALID_RATE;
+        calc_times_pre<M, K>(s, lv[K], t, old, r);
+    }
+    return err;
+}
+
+/* The logarithms of the channels' next uniforms, computed ahead by a
+ * wavefront for its live lanes (aem_day_wave).  With A live lanes and NT
+ * channels every live lane owns Q = 64 / (A NT) slots per channel: lane l
+ * holds slot l / (A NT) of channel (l / A) % NT of the live lane of rank
+ * l % A, the logarithm of the uniform at position base + slot of that
+ * channel's stream, base being the channel's position at the last
+ * refill. */
+template <class M>
+struct AemTable {
+    unsigned long long base[M::NT];
+    double h;
+    uint64_t live;
+    int A, rank, Q;
+    bool filled;
+};
+
+template <class M>
+__device__ __forceinline__ void aem_table_init(AemTable<M> &tb, bool live)
+{
+    tb.live = __ballot(live);
+    tb.A = __popcll(tb.live);
+    tb.rank = (int)__builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(tb.live >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tb.live, 0u));
+    tb.Q = tb.A > 0 ? 64 / (tb.A * M::NT) : 0;
+    tb.filled = false;
+    tb.h = 0.0;
+    static_for<M::NT>([&](auto K) { tb.base[decltype(K)::value] = 0; });
+}
+
+/* aem_day for a whole wavefront: every lane enters, run says whether its
+ * node simulates this day.  Channel draws take their logarithms from the
+ * table, refilled by all lanes when a running lane has used every slot of
+ * some channel; before each firing every lane pulls, for each channel, the
+ * value for its lane's next position (ds_bpermute).  Without slots (more
+ * than 64 / NT live lanes) each lane runs aem_day.  Values, stream
+ * positions and results are aem_day's. */
+template <class M, class CT>
+__device__ __forceinline__ int aem_day_wave(AemState<M> &s, const CT &cx, double next,
+                                            uint64_t chan_base, unsigned long long &fired,
+                                            bool run, AemTable<M> &tb)
+{
+    constexpr int NT = M::NT;
+    if (tb.Q == 0)
+        return run ? aem_day<M>(s, cx, next, chan_base, fired) : 0;
+    int err = 0;
+    const int lane = (int)__lane_id();
+    const bool live = (tb.live >> lane) & 1ull;
+    for (;;) {
+        /* Earliest channel; the first index wins among exact ties. */
+        int tr = 0;
+        double t = INFINITY;
+        if (run) {
+            t = s.time[0];
+            static_for<NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                if (k > 0 && s.time[k] < t) {
+                    t = s.time[k];
+                    tr = k;
+                }
+            });
+            if (isinf(t) || t >= next)
+                run = false;
+        }
+        if (__ballot(run) == 0)
+            break;
+        bool need = !tb.filled;
+        if (run) {
+            static_for<NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                need = need || s.j[k] - tb.base[k] >= (unsigned long long)tb.Q;
+            });
+        }
+        if (__ballot(need) != 0) {
+            /* Refill: every live lane's slots start at its channels'
+             * current positions. */
+            if (live) {
+                static_for<NT>([&](auto K) {
+                    constexpr int k = decltype(K)::value;
+                    tb.base[k] = s.j[k];
+                });
+            }
+            const int owner = lane % tb.A;
+            const int K = (lane / tb.A) % NT;
+            const int slot = lane / (tb.A * NT);
+            const int c = nth_set_bit(tb.live, owner);
+            const uint64_t cb = ((uint64_t)lane_pull(c, (uint32_t)(chan_base >> 32)) << 32) |
+                                lane_pull(c, (uint32_t)chan_base);
+            unsigned long long bk = 0;
+            static_for<NT>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const uint64_t bj =
+                    ((uint64_t)lane_pull(c, (uint32_t)(tb.base[j] >> 32)) << 32) |
+                    lane_pull(c, (uint32_t)tb.base[j]);
+                if (K == j)
+                    bk = bj;
+            });
+            const siminf_stream st =
+                siminf_stream_make(cx.seed, SIMINF_DOMAIN_AEM, cb + (uint64_t)K);
+            tb.h = slot < tb.Q ? siminf_log(siminf_uniform_at(&st, bk + (uint64_t)slot)) : 0.0;
+            tb.filled = true;
+        }
+        double lv[NT];
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            const int sl = run ? (int)(s.j[k] - tb.base[k]) : 0;
+            const int src = run ? tb.rank + tb.A * (k + NT * sl) : lane;
+            lv[k] = lane_pull(src, tb.h);
+        });
+        if (run) {
+            fired++;
+            static_for<NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                if (tr == k) {
+                    const int e = aem_fire_pre<M, k>(s, cx, t, lv);
+                    if (e)
+                        err = e;
+                }
+            });
+            if (err)
+                run = false;
+        }
+    }
+    return err;
+}
+
 /* ------------------------------------------------------------------ */
 /* Sampling individuals for an event (SimInf_solver.c:62-324).         */
 /* ------------------------------------------------------------------ */

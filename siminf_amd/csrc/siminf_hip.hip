@@ -1681,6 +1681,20 @@ extern "C" int SimInf_hip_batch_time(const SimInf_hip_batch *B, double *ms)
     return 0;
 }
 
+extern "C" int SimInf_hip_release(void)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    if (pool_allocation(dev)) {
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess ||
+            hipMemPoolTrimTo(pool, 0) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Blocking entry points.                                              */
 /* ------------------------------------------------------------------ */

@@ -245,16 +245,32 @@ __device__ __forceinline__ void aem_lane(const KParams &p, long long g)
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
     unsigned long long fired = 0, frozen_cols = 0;
     bool push = false;
+    const bool live = g >= 0;
+    const int node = live ? (int)g : 0;
+    Ctx<M, CANON> cx;
+    const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
+    AemState<M> s;
+    int err = 0;
+    double tt = p.tt0, next = p.next0;
+    const EvTab tab = global_tables(p);
+    E1Cursor ec;
+    /* Can the node's state no longer change (every rate zero)? */
+    auto frozen = [&]() -> bool {
+        if constexpr (pts_noop<M>::value) {
+            bool idle = !EVENTS || ec.cur >= ec.end;
+            static_for<NT>([&](auto J) {
+                idle = idle && s.rate[decltype(J)::value] == 0.0;
+            });
+            return idle;
+        } else {
+            return false;
+        }
+    };
+    /* The lane runs days (not the classify pass, no error at start). */
+    bool going = false;
 
-    if (g >= 0) {
-        const int node = (int)g;
-        Ctx<M, CANON> cx;
+    if (live) {
         cx.load(p, node, 0);
-        const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
-        AemState<M> s;
-        int err = 0;
-        double tt = p.tt0, next = p.next0;
-
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
             s.u[c] = p.u0[g * NC + c];
@@ -285,23 +301,8 @@ __device__ __forceinline__ void aem_lane(const KParams &p, long long g)
             s.inf[j] = 0.0;
             s.j[j] = 1;
         });
-
-        const EvTab tab = global_tables(p);
-        E1Cursor ec;
         if constexpr (EVENTS)
             ec.start(p, node);
-        /* Can the node's state no longer change (every rate zero)? */
-        auto frozen = [&]() -> bool {
-            if constexpr (pts_noop<M>::value) {
-                bool idle = !EVENTS || ec.cur >= ec.end;
-                static_for<NT>([&](auto J) {
-                    idle = idle && s.rate[decltype(J)::value] == 0.0;
-                });
-                return idle;
-            } else {
-                return false;
-            }
-        };
 
         if (err) {
             record_error(p, 0, 0, PH_INIT, node, err);
@@ -313,73 +314,90 @@ __device__ __forceinline__ void aem_lane(const KParams &p, long long g)
                 push = true;
             }
         } else {
-            int U_it = 0, day = 0;
-            double ts = p.tspan[0];
-            for (;;) {
-                bool upd = false;
-                int bad = 0;
-
-                if (p.mode == NODES_ACTIVE && frozen()) {
-                    store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
-                    break;
-                }
-
-                err = aem_day<M>(s, cx, next, chan_base, fired);
-                if (err) {
-                    record_error(p, 0, day, PH_SSA, node, err);
-                    break;
-                }
-                if constexpr (EVENTS) {
-                    err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
-                    if (err) {
-                        record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
-                        break;
-                    }
-                }
-
-                /* Post time step; refresh through calcTimes at the node's
-                 * time, which is the next unit of time (aem.c:329-384). */
-                const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
-                if (rc < 0) {
-                    record_error(p, 0, day, PH_PTS, node, rc);
-                    break;
-                }
-                if (rc > 0 || upd) {
-                    static_for<NT>([&](auto J) {
-                        constexpr int j = decltype(J)::value;
-                        const double old = s.rate[j];
-                        const double r = M::template rate<j>(s.u, s.vnew, cx.ld,
-                                                             cx.gd, tt);
-                        s.rate[j] = r;
-                        if (!rate_ok(r))
-                            err = SIMINF_ERR_INVALID_RATE;
-                        calc_times<M, j>(s, cx.seed, chan_base, next, old, r);
-                    });
-                    if (err) {
-                        record_error(p, 0, day, PH_PTS, node, err);
-                        break;
-                    }
-                }
-
-                tt = next;
-                next += 1.0;
-                store_columns<NC, ND>(p, 0, node, tt, U_it, ts, s.u, s.vnew);
-                if constexpr (ND > 0) {
-                    static_for<ND>([&](auto C) {
-                        constexpr int c = decltype(C)::value;
-                        const double x = s.v[c];
-                        s.v[c] = s.vnew[c];
-                        s.vnew[c] = x;
-                    });
-                }
-                if (U_it >= p.tlen)
-                    break;
-                day++;
-            }
+            going = true;
         }
-        if (p.mode == NODES_CLASSIFY && frozen_cols == 0)
-            p.frozen_from[g] = p.tlen;
     }
+
+    /* Day loop.  The wavefront runs it until its last lane is done; the
+     * lanes' channel draws come from the wavefront's table of logarithms
+     * (aem_day_wave), which lanes without a node or past their last day
+     * help to fill. */
+    if (p.mode != NODES_CLASSIFY) {
+        AemTable<M> tb;
+        aem_table_init<M>(tb, going);
+        int U_it = 0, day = 0;
+        double ts = p.tspan[0];
+        for (;;) {
+            if (__ballot(going) == 0)
+                break;
+            bool upd = false;
+            int bad = 0;
+
+            if (going && p.mode == NODES_ACTIVE && frozen()) {
+                store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                going = false;
+            }
+
+            err = aem_day_wave<M>(s, cx, next, chan_base, fired, going, tb);
+            if (!going)
+                continue;
+            if (err) {
+                record_error(p, 0, day, PH_SSA, node, err);
+                going = false;
+                continue;
+            }
+            if constexpr (EVENTS) {
+                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
+                if (err) {
+                    record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
+                    going = false;
+                    continue;
+                }
+            }
+
+            /* Post time step; refresh through calcTimes at the node's
+             * time, which is the next unit of time (aem.c:329-384). */
+            const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+            if (rc < 0) {
+                record_error(p, 0, day, PH_PTS, node, rc);
+                going = false;
+                continue;
+            }
+            if (rc > 0 || upd) {
+                static_for<NT>([&](auto J) {
+                    constexpr int j = decltype(J)::value;
+                    const double old = s.rate[j];
+                    const double r = M::template rate<j>(s.u, s.vnew, cx.ld, cx.gd, tt);
+                    s.rate[j] = r;
+                    if (!rate_ok(r))
+                        err = SIMINF_ERR_INVALID_RATE;
+                    calc_times<M, j>(s, cx.seed, chan_base, next, old, r);
+                });
+                if (err) {
+                    record_error(p, 0, day, PH_PTS, node, err);
+                    going = false;
+                    continue;
+                }
+            }
+
+            tt = next;
+            next += 1.0;
+            store_columns<NC, ND>(p, 0, node, tt, U_it, ts, s.u, s.vnew);
+            if constexpr (ND > 0) {
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    const double x = s.v[c];
+                    s.v[c] = s.vnew[c];
+                    s.vnew[c] = x;
+                });
+            }
+            if (U_it >= p.tlen)
+                going = false;
+            day++;
+        }
+    }
+    if (live && p.mode == NODES_CLASSIFY && frozen_cols == 0)
+        p.frozen_from[g] = p.tlen;
     if (p.mode == NODES_CLASSIFY) {
         push_active(p, g, push);
         add_count(p.frozen_cols, frozen_cols);
