@@ -259,12 +259,19 @@ def bench_batch(args):
     if not args.no_extras:
         for key, fn in (("pcie_inclusive", lambda: pcie_inclusive(model)),
                         ("sise3_herds", lambda: bench_sise3(args)),
+                        ("sise3_replicates", bench_sise3_replicates),
                         ("pfilter_request", lambda: bench_pfilter_request(args))):
             try:
                 out[key] = fn()
             except Exception as e:          # reported, not fatal to the line
                 out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             print("bench: %s done" % key, file=sys.stderr, flush=True)
+        pc = out["pcie_inclusive"]
+        if "error" not in pc:
+            # The plain path (one blocking run per solver, U to the host)
+            # against the batched launch with inputs resident.
+            pc["plain_over_batched"] = (sum(pc[s]["seconds"] for s in solvers)
+                                        / (out["ms_per_step"] / 1e3))
     if not args.no_cpu_baseline:
         T = host_cores()
         out["cpu_baseline"] = cpu_baseline(model, solvers, T)
@@ -320,6 +327,30 @@ def bench_sise3(args, steps=10):
             "ms_per_run": wall / steps * 1e3, "kernel_ms_mean": statistics.mean(ms),
             "herd_map": {"solo_herds": solo, "herds_per_wavefront": pack},
             "algorithmic_bytes_per_run": algorithmic_bytes(model)}
+
+
+def bench_sise3_replicates(reps=(1, 2, 4, 8, 16, 32), runs=2):
+    """SISe3, 1600 herds x 1460 days with scheduled events, R replicates
+    in one plan (mssm): the grid-wide coupled kernel runs as many
+    replicates per cooperative launch as the co-resident workgroups allow
+    (default herd map: 64 herds to a wavefront).  Node-days/s per R, mean
+    of `runs` resident runs."""
+    import siminf_amd as si
+    from siminf_amd import workloads as W
+    rows = []
+    for R in reps:
+        model = W.sise3_herds(replicates=R)
+        with si.Plan(model) as p:
+            p.run(100)
+            nd = 0
+            t0 = time.perf_counter()
+            for i in range(runs):
+                nd += p.run(1 + i)[1]
+            wall = time.perf_counter() - t0
+        rows.append({"replicates": R, "ms_per_run": wall / runs * 1e3,
+                     "node_days_per_s": nd / wall})
+    return {"workload": "SISe3 1600 herds x 1460 days with events, R replicates "
+                        "(default herd map)", "rows": rows}
 
 
 def bench_pfilter_request(args):
