@@ -29,6 +29,22 @@ FLAGS = [
 ]
 
 
+# The node kernels' translation units are scheduled for instruction-level
+# parallelism: their wavefronts run mostly one per SIMD, each a long
+# dependent chain (the SSA step) beside independent work (the next
+# attempt's Philox block and logarithm), so interleaving the two matters
+# more than the register pressure the default scheduler minimises.
+ILP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+
+
+def unit_flags(src):
+    """Compile flags of one translation unit."""
+    base = os.path.basename(src)
+    if base.startswith("builtin_") and not base.endswith("_coupled.hip"):
+        return FLAGS + ILP_FLAGS
+    return FLAGS
+
+
 def sources():
     """The library's translation units, host side first."""
     return [SRC] + sorted(glob.glob(os.path.join(CSRC, "builtin_*.hip")))
@@ -45,7 +61,7 @@ def _compile(src, verbose, incremental=False):
         newest = max(os.path.getmtime(f) for f in [src] + _headers())
         if os.path.getmtime(obj) > newest:
             return obj
-    cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
+    cmd = [HIPCC] + unit_flags(src) + ["-c", "-o", obj, src]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)

@@ -208,10 +208,14 @@ struct KParams {
      * SIMD of the device's simds, at most 64 lanes).  The active pass
      * reads the count on the device: its grid is fixed, and each
      * wavefront takes the wave slots w, w + (wavefronts in the grid),
-     * ... below the slots the count needs. */
+     * ... below the slots the count needs.  free_lanes: after the
+     * classify pass each lane steps through its own days (one attempt or
+     * one day's end per loop trip) instead of the wavefront stepping day
+     * by day. */
     int mode;
     int lpw_shift;
     int simds;
+    int free_lanes;
     int *active;
     int *active_count;
 
@@ -246,6 +250,30 @@ __device__ __forceinline__ T sel(const T (&a)[N], int i)
         constexpr int k = decltype(K)::value;
         if (k > 0 && i == k)
             r = a[k];
+    });
+    return r;
+}
+
+/* Holds x in a register at this point of the program: the work that
+ * computes x stays before the branches that follow (the compiler would
+ * otherwise sink it past them, next to its use, where it no longer runs
+ * beside the independent work of the same block). */
+__device__ __forceinline__ void pin(double &x)
+{
+    __asm__ volatile("" : "+v"(x));
+}
+
+/* a[i] for a run-time i as a sum of masked elements: a chain of selects
+ * between elements can be turned back into one indexed load, which moves
+ * the array to scratch memory. */
+template <int N>
+__device__ __forceinline__ unsigned long long pick_u64(const unsigned long long (&a)[N],
+                                                       int i)
+{
+    unsigned long long r = 0;
+    static_for<N>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        r += a[k] & (0ull - (unsigned long long)(i == k));
     });
     return r;
 }
@@ -321,6 +349,23 @@ struct Ctx {
                     constexpr int c = decltype(C)::value;
                     sval[k * M::NC + c] = p.Sval[k * M::NC + c];
                 });
+            });
+        }
+        seed = p.seed;
+    }
+
+    /* Defined values for a lane without a node, which runs a wavefront's
+     * straight-line code with results it discards. */
+    __device__ __forceinline__ void clear(const KParams &p)
+    {
+        static_for<(M::NLD > 0 ? M::NLD : 1)>([&](auto I) { ld[decltype(I)::value] = 0.0; });
+        static_for<(M::NGD > 0 ? M::NGD : 1)>([&](auto I) { gd[decltype(I)::value] = 0.0; });
+        if constexpr (!CANON) {
+            static_for<M::NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                smask[k] = 0;
+                gmask[k] = 0;
+                static_for<M::NC>([&](auto C) { sval[k * M::NC + decltype(C)::value] = 0; });
             });
         }
         seed = p.seed;
@@ -437,6 +482,50 @@ __device__ __forceinline__ int fire_any(NodeState<M> &s, const CT &cx, int tr,
     });
     s.a0 += delta;
     return err;
+}
+
+/* fire_any computed on copies and kept only where on: every lane runs
+ * the same straight-line code whether it fires or not, so that the
+ * compiler can interleave it with independent work (the next attempt's
+ * Philox block and logarithm).  Returns an error code (0 where !on). */
+template <class M, class CT>
+__device__ __forceinline__ int fire_sel(NodeState<M> &s, const CT &cx, int tr,
+                                        double t, bool on)
+{
+    int err = 0;
+    int un[M::NC];
+    static_for<M::NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        int d = 0;
+        static_for<M::NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if constexpr (M::S[k][c] != 0)
+                d += tr == k ? M::S[k][c] : 0;
+        });
+        un[c] = s.u[c] + d;
+        if (un[c] < 0)
+            err = SIMINF_ERR_NEGATIVE_STATE;
+    });
+    double delta = 0.0;
+    static_for<M::NT>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr unsigned gm = g_row_mask<M, j>();
+        if constexpr (gm != 0) {
+            const bool in = (gm >> tr) & 1u;
+            const double old = s.rate[j];
+            const double r = M::template rate<j>(un, s.v, cx.ld, cx.gd, t);
+            s.rate[j] = on && in ? r : old;
+            delta += in ? r - old : 0.0;
+            if (in && !rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+        }
+    });
+    static_for<M::NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        s.u[c] = on ? un[c] : s.u[c];
+    });
+    s.a0 = on ? s.a0 + delta : s.a0;
+    return on ? err : 0;
 }
 
 /* One SSA firing at time t: the transition u_tr picks by a linear scan
@@ -837,6 +926,35 @@ __device__ __forceinline__ void calc_times_pre(AemState<M> &s, double logu, doub
         s.inf[J] = (oldtime - tt) * old_rate;
         s.time[J] = INFINITY;
     }
+}
+
+/* calc_times_pre for channel J without branches, applied only where on:
+ * the three cases that divide (draw, wake-up, rescale) share one
+ * division, num / new_rate with num = -logu, reactInf or the old rate,
+ * and the results are selected; tt + q and q + tt are the same IEEE sum.
+ * Returns whether the channel drew (used logu: its stream position then
+ * advances by one, which the caller applies). */
+template <class M, int J>
+__device__ __forceinline__ bool calc_times_sel(AemState<M> &s, double logu, double tt,
+                                               double old_rate, double new_rate, bool on)
+{
+    const double oldtime = s.time[J];
+    const double rinf = s.inf[J];
+    const bool oi = isinf(oldtime);
+    const bool big = new_rate >= 2.2250738585072014e-308;
+    const bool draw = oi ? rinf == 0.0 : (big && oldtime == tt);
+    const bool wake = oi && rinf != 0.0 && new_rate > 0.0;
+    const bool resc = !oi && big && oldtime != tt;
+    const bool sleep = !oi && !big;
+    const double num = draw ? -logu : (wake ? rinf : old_rate);
+    const double q = num / new_rate;
+    const double tn = resc ? ((q * (oldtime - tt)) + tt) : (q + tt);
+    double tnew = oldtime;
+    tnew = (draw || wake || resc) ? tn : tnew;
+    tnew = sleep ? INFINITY : tnew;
+    s.time[J] = on ? tnew : oldtime;
+    s.inf[J] = on && sleep ? (oldtime - tt) * old_rate : rinf;
+    return on && draw;
 }
 
 /* aem_fire with every channel's next logarithm given (lv). */
@@ -1605,11 +1723,15 @@ __device__ __forceinline__ void store_sparse(const int *ir, const int *jc,
     const int end = jc[col + 1];
     for (int j = lo; j < end && ir[j] < first + NROW; j++) {
         const int r = ir[j] - first;
+        /* The row's value by selects (an if per row lets the compiler
+         * index x at run time, which moves x to scratch memory). */
+        T val = x[0];
         static_for<NROW>([&](auto C) {
             constexpr int c = decltype(C)::value;
-            if (r == c)
-                pr[j] = (double)x[c];
+            if constexpr (c > 0)
+                val = r == c ? x[c] : val;
         });
+        pr[j] = (double)val;
     }
 }
 

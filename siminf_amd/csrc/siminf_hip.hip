@@ -865,6 +865,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                 hipSuccess && cus > 0)
             P->simds = 4 * cus;
         p.simds = P->simds;
+        p.free_lanes = 1;
         /* The active pass's fixed grid: eight wavefronts per SIMD (what
          * the node kernels' register use lets the device hold, so that a
          * packing that needs up to that many gives every wave slot a

@@ -229,6 +229,224 @@ __device__ __forceinline__ void ssm_lane(const KParams &p, long long g)
     add_transitions(p, fired);
 }
 
+/* ssm_lane after the classify pass (modes NODES_ALL and NODES_ACTIVE)
+ * with free-running lanes: each lane steps through its own days, one SSA
+ * attempt or one day's end per trip round the loop, so a wavefront lasts
+ * as long as its busiest node's run, where a loop over days lasts the
+ * sum over days of each day's busiest lane.  Every attempt consumes one
+ * Philox block (ssa_day's schedule), so the next attempt's block and
+ * waiting-time logarithm are computed during the current one, in the
+ * same straight-line code as the attempt (fire_sel for models whose S and
+ * G fold into the code).  Stream positions and every value are those of
+ * ssa_day. */
+template <class M, bool EVENTS, bool CANON>
+__device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    const bool live = g >= 0;
+    unsigned long long fired = 0;
+
+    const int rep = live ? (int)(g / p.Nn) : 0;
+    const int node = live ? (int)(g - (long long)rep * p.Nn) : 0;
+    Ctx<M, CANON> cx;
+    NodeState<M> s;
+    s.a0 = 0.0;
+    int err = 0;
+    double tt = p.tt0, next = p.next0;
+    const EvTab tab = global_tables(p);
+    E1Cursor ec;
+    auto frozen = [&]() __attribute__((always_inline)) -> bool {
+        if constexpr (pts_noop<M>::value)
+            return s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end);
+        else
+            return false;
+    };
+    bool going = false;
+    if (live) {
+        cx.load(p, node, rep);
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            s.u[c] = p.u0[g * NC + c];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.v[c] = p.v0[g * ND + c];
+                s.vnew[c] = s.v[c];
+            });
+        }
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+        static_for<NT>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+            s.rate[j] = r;
+            s.a0 += r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+        });
+        if constexpr (EVENTS)
+            ec.start(p, node);
+        if (err)
+            record_error(p, rep, 0, PH_INIT, node, err);
+        else
+            going = true;
+    } else {
+        cx.clear(p);
+        static_for<NC>([&](auto C) { s.u[decltype(C)::value] = 0; });
+        static_for<(ND > 0 ? ND : 1)>([&](auto C) {
+            s.v[decltype(C)::value] = 0.0;
+            s.vnew[decltype(C)::value] = 0.0;
+        });
+        static_for<NT>([&](auto J) { s.rate[decltype(J)::value] = 0.0; });
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, 0);
+    }
+
+    int U_it = 0, day = 0;
+    double ts = p.tspan[0];
+    double t = tt;
+    /* b: the next attempt's block (the stream's position is 2b once an
+     * attempt of the day has consumed one); bday: b at the day's start;
+     * wE, wU: block b's waiting-time numerator -log(u0) and u1. */
+    uint64_t b = 0, bday = 0;
+    double wE = 1.0, wU = 0.5;
+    auto load_block = [&]() __attribute__((always_inline)) {
+        double u0, u1;
+        siminf_uniform_pair(&s.rng, b, &u0, &u1);
+        wE = -siminf_log_u01(u0);
+        wU = u1;
+    };
+    if (going)
+        load_block();
+    if (going && p.mode == NODES_ACTIVE && frozen()) {
+        store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+        going = false;
+    }
+
+    for (;;) {
+        if (__ballot(going) == 0)
+            break;
+        /* The attempt with block b; block b + 1 beside it. */
+        const bool att = going && s.a0 > 0.0;
+        double n0, n1;
+        siminf_uniform_pair(&s.rng, b + 1u, &n0, &n1);
+        const double nE = -siminf_log_u01(n0);
+        const double tau = wE / s.a0;
+        bool fires = att && (tau + t) < next;
+        t = fires ? t + tau : t;
+        /* Transition by the scan's stopping point (ssa_fire). */
+        const double rnd = wU * s.a0;
+        int tr = 0;
+        double cum = s.rate[0];
+        static_for<NT - 1>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            tr += rnd > cum ? 1 : 0;
+            if constexpr (k + 1 < NT)
+                cum += s.rate[k + 1];
+        });
+        double rt = s.rate[0];
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if constexpr (k > 0)
+                rt = tr == k ? s.rate[k] : rt;
+        });
+        /* Floating point fix (ssm.c:170-189) by selects: a zero rate
+         * walks back to the nearest non-zero one below it; none left
+         * stops the node for the day with a0 = 0. */
+        static_for<NT>([&](auto K) {
+            constexpr int k = NT - 1 - decltype(K)::value;
+            if constexpr (k > 0) {
+                const bool back = tr == k && s.rate[k] == 0.0;
+                tr = back ? k - 1 : tr;
+                rt = back ? s.rate[k - 1] : rt;
+            }
+        });
+        const bool none = fires && rt == 0.0;
+        s.a0 = none ? 0.0 : s.a0;
+        fires = fires && !none;
+        int e = 0;
+        if constexpr (CANON && NT <= 8) {
+            e = fire_sel<M>(s, cx, tr, t, fires);
+        } else {
+            if (fires) {
+                static_for<NT>([&](auto K) {
+                    constexpr int k = decltype(K)::value;
+                    if (tr == k)
+                        e = fire<M, k>(s, cx, t);
+                });
+            }
+        }
+        fired += fires ? 1u : 0u;
+        if (att) {
+            b++;
+            wE = nE;
+            wU = n1;
+        }
+        pin(wE);
+        pin(wU);
+        if (e) {
+            record_error(p, rep, day, PH_SSA, node, e);
+            going = false;
+        }
+        /* The day ends here unless the lane fired and keeps a positive
+         * total rate (ssa_day_wave). */
+        const bool dend = going && !(fires && s.a0 > 0.0);
+        if (__ballot(dend) == 0 || !dend)
+            continue;
+        bool upd = false;
+        if constexpr (EVENTS) {
+            int bad = 0;
+            if (b != bday)
+                s.rng.j = 2u * b;
+            const uint64_t j0 = s.rng.j;
+            err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
+            if (err) {
+                record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
+                going = false;
+                continue;
+            }
+            if (s.rng.j != j0) {
+                b = (s.rng.j + 1u) >> 1;
+                load_block();
+            }
+        }
+        const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+        if (rc < 0) {
+            record_error(p, rep, day, PH_PTS, node, rc);
+            going = false;
+            continue;
+        }
+        if (rc > 0 || upd) {
+            err = refresh_rates<M>(s, cx, tt);
+            if (err) {
+                record_error(p, rep, day, PH_PTS, node, err);
+                going = false;
+                continue;
+            }
+        }
+        tt = next;
+        next += 1.0;
+        t = tt;
+        bday = b;
+        store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                const double x = s.v[c];
+                s.v[c] = s.vnew[c];
+                s.vnew[c] = x;
+            });
+        }
+        if (U_it >= p.tlen)
+            going = false;
+        day++;
+        if (going && p.mode == NODES_ACTIVE && frozen()) {
+            store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+            going = false;
+        }
+    }
+    add_transitions(p, fired);
+}
+
 /* ssm / mssm over nodes that never interact: each wavefront runs its
  * wave slots (one, its own lanes, except in the active pass). */
 template <class M, bool EVENTS, bool CANON>
@@ -237,6 +455,16 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
     p.seed = run_seed(p);
     for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
         ssm_lane<M, EVENTS, CANON>(p, node_lane(p, a));
+}
+
+/* The same after the classify pass with free-running lanes: a kernel of
+ * its own, so that its registers are allocated for its own code. */
+template <class M, bool EVENTS, bool CANON>
+__global__ __launch_bounds__(256) void k_ssm_nodes_free(KParams p)
+{
+    p.seed = run_seed(p);
+    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
+        ssm_lane_free<M, EVENTS, CANON>(p, node_lane(p, a));
 }
 
 template <class M, bool EVENTS, bool CANON>
@@ -405,6 +633,314 @@ __device__ __forceinline__ void aem_lane(const KParams &p, long long g)
     add_transitions(p, fired);
 }
 
+/* aem_lane after the classify pass with free-running lanes, as
+ * ssm_lane_free: one firing or one day's end per trip round the loop.
+ * Each channel keeps the logarithms of the uniforms at its next two
+ * stream positions (l0, l1): a firing that draws from a channel takes l0
+ * and shifts l1 down, and the next trip computes the channel's new l1
+ * (one Philox block and one logarithm) in the same straight-line code as
+ * its firing, which needs only l0.  A firing recomputes every rate G
+ * marks and runs calcTimes for those channels and the fired one
+ * (calc_times_sel, one division per channel), so lanes that fired
+ * different transitions run the same instructions.  Values, stream
+ * positions and results are aem_day's. */
+template <class M, bool EVENTS, bool CANON>
+__device__ __forceinline__ void aem_lane_free(const KParams &p, long long g)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    unsigned long long fired = 0;
+    const bool live = g >= 0;
+    const int node = live ? (int)g : 0;
+    Ctx<M, CANON> cx;
+    const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
+    AemState<M> s;
+    int err = 0;
+    double tt = p.tt0, next = p.next0;
+    const EvTab tab = global_tables(p);
+    E1Cursor ec;
+    auto frozen = [&]() __attribute__((always_inline)) -> bool {
+        if constexpr (pts_noop<M>::value) {
+            bool idle = !EVENTS || ec.cur >= ec.end;
+            static_for<NT>([&](auto J) {
+                idle = idle && s.rate[decltype(J)::value] == 0.0;
+            });
+            return idle;
+        } else {
+            return false;
+        }
+    };
+    /* l0[k], l1[k]: siminf_log of channel k's uniforms at positions
+     * s.j[k] and s.j[k] + 1; pend: channels whose l1 is still to be
+     * computed (bit k). */
+    double l0[NT], l1[NT];
+    unsigned pend = 0;
+    auto chan_log = [&](int k, unsigned long long j)
+        __attribute__((always_inline)) -> double {
+        const siminf_stream c = siminf_stream_make(cx.seed, SIMINF_DOMAIN_AEM,
+                                                   chan_base + (uint64_t)k);
+        return siminf_log_u01(siminf_uniform_at(&c, j));
+    };
+    bool going = false;
+    if (live) {
+        cx.load(p, node, 0);
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            s.u[c] = p.u0[g * NC + c];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.v[c] = p.v0[g * ND + c];
+                s.vnew[c] = s.v[c];
+            });
+        }
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
+        /* Initial rates and reaction times (aem.c:117-153). */
+        static_for<NT>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
+            s.rate[j] = r;
+            if (!rate_ok(r))
+                err = SIMINF_ERR_INVALID_RATE;
+            const siminf_stream c = siminf_stream_make(
+                p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
+            double u0, u1, u2, u3;
+            siminf_uniform_pair(&c, 0, &u0, &u1);
+            siminf_uniform_pair(&c, 1, &u2, &u3);
+            double tj = -siminf_log(u0) / r + tt;
+            if (tj <= 0.0)
+                tj = INFINITY;
+            s.time[j] = tj;
+            s.inf[j] = 0.0;
+            s.j[j] = 1;
+            l0[j] = siminf_log_u01(u1);
+            l1[j] = siminf_log_u01(u2);
+        });
+        if constexpr (EVENTS)
+            ec.start(p, node);
+        if (err)
+            record_error(p, 0, 0, PH_INIT, node, err);
+        else
+            going = true;
+    } else {
+        cx.clear(p);
+        static_for<NC>([&](auto C) { s.u[decltype(C)::value] = 0; });
+        static_for<(ND > 0 ? ND : 1)>([&](auto C) {
+            s.v[decltype(C)::value] = 0.0;
+            s.vnew[decltype(C)::value] = 0.0;
+        });
+        static_for<NT>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            s.rate[j] = 0.0;
+            s.time[j] = INFINITY;
+            s.inf[j] = 0.0;
+            s.j[j] = 0;
+            l0[j] = 0.0;
+            l1[j] = 0.0;
+        });
+        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, 0);
+    }
+    int U_it = 0, day = 0;
+    double ts = p.tspan[0];
+    if (going && p.mode == NODES_ACTIVE && frozen()) {
+        store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+        going = false;
+    }
+
+    /* A draw from channel k: l1 moves down, the new l1 is pending. */
+    auto took = [&](int k, bool drew) __attribute__((always_inline)) {
+        static_for<NT>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            const bool d = drew && k == c;
+            s.j[c] += d ? 1u : 0u;
+            l0[c] = d ? l1[c] : l0[c];
+            pend |= d ? 1u << c : 0u;
+        });
+    };
+    /* The new l1 of the lowest pending channel (every lane computes one,
+     * used where a channel is pending). */
+    auto refill = [&]() __attribute__((always_inline)) {
+        const int k = pend ? __builtin_ctz(pend) : 0;
+        const unsigned long long jk = pick_u64(s.j, k);
+        const double h = chan_log(k, jk + 1u);
+        static_for<NT>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            l1[c] = pend && k == c ? h : l1[c];
+        });
+        pend &= pend - 1u;
+    };
+
+    for (;;) {
+        if (__ballot(going) == 0)
+            break;
+        /* At most one channel is pending here: its new l1, computed
+         * beside the firing. */
+        const unsigned pend0 = pend;
+        const int kp = pend0 ? __builtin_ctz(pend0) : 0;
+        const unsigned long long jp = pick_u64(s.j, kp);
+        const double hp = chan_log(kp, jp + 1u);
+
+        /* Earliest channel; the first index wins among exact ties. */
+        int tr = 0;
+        double t = s.time[0];
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            if (k > 0 && s.time[k] < t) {
+                t = s.time[k];
+                tr = k;
+            }
+        });
+        const bool fires = going && !(isinf(t) || t >= next);
+        int e = 0;
+        unsigned drew = 0;
+        if constexpr (CANON && NT <= 8) {
+            int un[NC];
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                int d = 0;
+                static_for<NT>([&](auto K) {
+                    constexpr int k = decltype(K)::value;
+                    if constexpr (M::S[k][c] != 0)
+                        d += tr == k ? M::S[k][c] : 0;
+                });
+                un[c] = s.u[c] + d;
+                if (un[c] < 0)
+                    e = SIMINF_ERR_NEGATIVE_STATE;
+            });
+            static_for<NT>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                constexpr unsigned gm = g_row_mask<M, j>() | (1u << j);
+                const bool in = fires && ((gm >> tr) & 1u);
+                const double old = s.rate[j];
+                const double r = M::template rate<j>(un, s.v, cx.ld, cx.gd, t);
+                s.rate[j] = in ? r : old;
+                if (in && !rate_ok(r))
+                    e = SIMINF_ERR_INVALID_RATE;
+                drew |= calc_times_sel<M, j>(s, l0[j], t, old, r, in) ? 1u << j : 0u;
+            });
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                s.u[c] = fires ? un[c] : s.u[c];
+            });
+            e = fires ? e : 0;
+        } else if (fires) {
+            static_for<NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                if (tr == k) {
+                    static_for<NC>([&](auto C) {
+                        constexpr int c = decltype(C)::value;
+                        if (cx.template s_stored<k, c>()) {
+                            s.u[c] += cx.template s_val<k, c>();
+                            if (s.u[c] < 0)
+                                e = SIMINF_ERR_NEGATIVE_STATE;
+                        }
+                    });
+                    static_for<NT>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        if (j == k || cx.template g_stored<k, j>()) {
+                            const double old = s.rate[j];
+                            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, t);
+                            s.rate[j] = r;
+                            if (!rate_ok(r))
+                                e = SIMINF_ERR_INVALID_RATE;
+                            drew |= calc_times_sel<M, j>(s, l0[j], t, old, r, true)
+                                    ? 1u << j : 0u;
+                        }
+                    });
+                }
+            });
+        }
+        fired += fires ? 1u : 0u;
+        /* The pending l1 lands before this firing's draws move l1 down. */
+        static_for<NT>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            l1[c] = pend0 && kp == c ? hp : l1[c];
+        });
+        pend &= pend - 1u;
+        static_for<NT>([&](auto K) {
+            constexpr int c = decltype(K)::value;
+            took(c, (drew >> c) & 1u);
+            pin(l0[c]);
+            pin(l1[c]);
+        });
+        if (e) {
+            record_error(p, 0, day, PH_SSA, node, e);
+            going = false;
+        }
+        const bool dend = going && !fires;
+        if (__ballot(dend) != 0 && dend) {
+            bool upd = false;
+            bool ok = true;
+            if constexpr (EVENTS) {
+                int bad = 0;
+                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
+                if (err) {
+                    record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
+                    going = false;
+                    ok = false;
+                }
+            }
+            /* Post time step; refresh through calcTimes at the next unit
+             * of time (aem.c:329-384).  A refresh may draw from several
+             * channels: their l1 are computed below. */
+            if (ok) {
+                const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+                if (rc < 0) {
+                    record_error(p, 0, day, PH_PTS, node, rc);
+                    going = false;
+                    ok = false;
+                } else if (rc > 0 || upd) {
+                    unsigned dr = 0;
+                    static_for<NT>([&](auto J) {
+                        constexpr int j = decltype(J)::value;
+                        const double old = s.rate[j];
+                        const double r = M::template rate<j>(s.u, s.vnew, cx.ld, cx.gd, tt);
+                        s.rate[j] = r;
+                        if (!rate_ok(r))
+                            err = SIMINF_ERR_INVALID_RATE;
+                        dr |= calc_times_sel<M, j>(s, l0[j], next, old, r, true) ? 1u << j : 0u;
+                    });
+                    static_for<NT>([&](auto K) {
+                        constexpr int c = decltype(K)::value;
+                        took(c, (dr >> c) & 1u);
+                    });
+                    if (err) {
+                        record_error(p, 0, day, PH_PTS, node, err);
+                        going = false;
+                        ok = false;
+                    }
+                }
+            }
+            if (ok) {
+                tt = next;
+                next += 1.0;
+                store_columns<NC, ND>(p, 0, node, tt, U_it, ts, s.u, s.vnew);
+                if constexpr (ND > 0) {
+                    static_for<ND>([&](auto C) {
+                        constexpr int c = decltype(C)::value;
+                        const double x = s.v[c];
+                        s.v[c] = s.vnew[c];
+                        s.vnew[c] = x;
+                    });
+                }
+                if (U_it >= p.tlen)
+                    going = false;
+                day++;
+                if (going && p.mode == NODES_ACTIVE && frozen()) {
+                    store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
+                    going = false;
+                }
+            }
+        }
+        /* Leave at most one pending channel for the next trip. */
+        while (__ballot(going && (pend & (pend - 1u)) != 0) != 0) {
+            if (going && (pend & (pend - 1u)) != 0)
+                refill();
+        }
+    }
+    add_transitions(p, fired);
+}
+
 /* aem over nodes that never interact, by wave slots as k_ssm_nodes. */
 template <class M, bool EVENTS, bool CANON>
 __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
@@ -412,6 +948,14 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
     p.seed = run_seed(p);
     for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
         aem_lane<M, EVENTS, CANON>(p, node_lane(p, a));
+}
+
+template <class M, bool EVENTS, bool CANON>
+__global__ __launch_bounds__(256) void k_aem_nodes_free(KParams p)
+{
+    p.seed = run_seed(p);
+    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
+        aem_lane_free<M, EVENTS, CANON>(p, node_lane(p, a));
 }
 
 /* ------------------------------------------------------------------ */
@@ -1110,6 +1654,17 @@ template <class M>
 void launch_ssm_nodes(const KParams &p, bool events, bool canon,
                       unsigned grid, unsigned block, hipStream_t st)
 {
+    if (p.mode != NODES_CLASSIFY && p.free_lanes) {
+        if (events && canon)
+            hipLaunchKernelGGL((k_ssm_nodes_free<M, true, true>), dim3(grid), dim3(block), 0, st, p);
+        else if (events)
+            hipLaunchKernelGGL((k_ssm_nodes_free<M, true, false>), dim3(grid), dim3(block), 0, st, p);
+        else if (canon)
+            hipLaunchKernelGGL((k_ssm_nodes_free<M, false, true>), dim3(grid), dim3(block), 0, st, p);
+        else
+            hipLaunchKernelGGL((k_ssm_nodes_free<M, false, false>), dim3(grid), dim3(block), 0, st, p);
+        return;
+    }
     if (events && canon)
         hipLaunchKernelGGL((k_ssm_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
     else if (events)
@@ -1124,6 +1679,17 @@ template <class M>
 void launch_aem_nodes(const KParams &p, bool events, bool canon,
                       unsigned grid, unsigned block, hipStream_t st)
 {
+    if (p.mode != NODES_CLASSIFY && p.free_lanes) {
+        if (events && canon)
+            hipLaunchKernelGGL((k_aem_nodes_free<M, true, true>), dim3(grid), dim3(block), 0, st, p);
+        else if (events)
+            hipLaunchKernelGGL((k_aem_nodes_free<M, true, false>), dim3(grid), dim3(block), 0, st, p);
+        else if (canon)
+            hipLaunchKernelGGL((k_aem_nodes_free<M, false, true>), dim3(grid), dim3(block), 0, st, p);
+        else
+            hipLaunchKernelGGL((k_aem_nodes_free<M, false, false>), dim3(grid), dim3(block), 0, st, p);
+        return;
+    }
     if (events && canon)
         hipLaunchKernelGGL((k_aem_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
     else if (events)

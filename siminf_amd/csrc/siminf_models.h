@@ -42,9 +42,13 @@ SIMINF_FN double SIR_S_to_I(const int *u, const double *v, const double *ldata,
     const double I_n = u[1];
     const double n = S_n + I_n + u[2];
     (void)v; (void)gdata; (void)t;
-    if (n > 0.0)
-        return (ldata[0] * S_n * I_n) / n;
-    return 0.0;
+    {
+        /* The quotient is formed unconditionally and selected, so that
+         * the rate is straight-line code; for n > 0 it is the value of
+         * the guarded division. */
+        const double r = (ldata[0] * S_n * I_n) / n;
+        return n > 0.0 ? r : 0.0;
+    }
 }
 
 SIMINF_FN double SIR_I_to_R(const int *u, const double *v, const double *ldata,
@@ -74,9 +78,11 @@ SIMINF_FN double SEIR_S_to_E(const int *u, const double *v, const double *ldata,
     const double I_n = u[2];
     const double n = S_n + u[1] + I_n + u[3];
     (void)v; (void)gdata; (void)t;
-    if (n > 0.0)
-        return ldata[0] * S_n * I_n / n;
-    return 0.0;
+    {
+        /* Formed unconditionally and selected (as SIR_S_to_I). */
+        const double r = ldata[0] * S_n * I_n / n;
+        return n > 0.0 ? r : 0.0;
+    }
 }
 
 SIMINF_FN double SEIR_E_to_I(const int *u, const double *v, const double *ldata,
@@ -102,9 +108,13 @@ SIMINF_FN double SIS_S_to_I(const int *u, const double *v, const double *ldata,
     const double I_n = u[1];
     const double n = S_n + I_n;
     (void)v; (void)gdata; (void)t;
-    if (n > 0.0)
-        return (ldata[0] * S_n * I_n) / n;
-    return 0.0;
+    {
+        /* The quotient is formed unconditionally and selected, so that
+         * the rate is straight-line code; for n > 0 it is the value of
+         * the guarded division. */
+        const double r = (ldata[0] * S_n * I_n) / n;
+        return n > 0.0 ? r : 0.0;
+    }
 }
 
 SIMINF_FN double SIS_I_to_S(const int *u, const double *v, const double *ldata,

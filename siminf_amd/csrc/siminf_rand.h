@@ -80,6 +80,9 @@ SIMINF_FN void siminf_philox4x32_10(
 {
     const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    /* Unrolled: ten rounds of straight-line code that the compiler can
+     * interleave with independent work around the call. */
+    SIMINF_UNROLL
     for (int round = 0; round < 10; round++) {
         /* One 32x32->64 product per multiplier gives both halves (a
          * single v_mad_u64_u32 on the device). */
@@ -205,11 +208,12 @@ SIMINF_FN double siminf_log_core(uint64_t b, int e)
     const double LN2_LO = 0x1.473de6af278edp-34;
     e += (int)(b >> 52) - 1023;
     b &= 0x000FFFFFFFFFFFFFull;
-    if (b >= 0x6A09E667F3BCDull) {         /* m >= sqrt(2): use m/2 */
-        b |= 0x3FE0000000000000ull;
-        e += 1;
-    } else {
-        b |= 0x3FF0000000000000ull;
+    {
+        /* m >= sqrt(2): use m/2.  Selects, not a branch, so that the
+         * series is straight-line code. */
+        const int half = b >= 0x6A09E667F3BCDull;
+        b |= half ? 0x3FE0000000000000ull : 0x3FF0000000000000ull;
+        e += half;
     }
 
     {
