@@ -1770,6 +1770,106 @@ __device__ __forceinline__ void store_columns(
     }
 }
 
+/* Largest tspan the free-running node kernels copy into LDS (64 KiB). */
+#define SIMINF_TS_LDS_MAX 8192
+
+/* tspan[i] from the workgroup's copy in LDS (TSL) or from global memory.
+ * The copy matters beyond latency: vector-memory loads and stores share
+ * one completion counter, so waiting for a global tspan load also waits
+ * for every trajectory store the wavefront has in flight, while an LDS
+ * read completes on its own counter. */
+template <bool TSL>
+__device__ __forceinline__ double tspan_at(const KParams &p, const double *ts_sh, int i)
+{
+    if constexpr (TSL)
+        return ts_sh[i];
+    else
+        return p.tspan[i];
+}
+
+/* Copy tspan into the workgroup's LDS (every thread of the workgroup
+ * calls this once, before any of them diverges). */
+template <bool TSL>
+__device__ __forceinline__ void tspan_to_lds(const KParams &p, double *ts_sh)
+{
+    if constexpr (TSL) {
+        for (int i = (int)threadIdx.x; i < p.tlen; i += (int)blockDim.x)
+            ts_sh[i] = p.tspan[i];
+        __syncthreads();
+    }
+}
+
+/* store_columns with the time point after ts also held (ts1): a day that
+ * writes one column compares tt with values loaded a day or more before,
+ * so the load of the next point never sits on the day's end (a lane
+ * stepping through its own days reaches its next day's end within a few
+ * hundred cycles, far less than a load's latency). */
+template <int NC, int ND, bool TSL>
+__device__ __forceinline__ void store_columns_ahead(
+    const KParams &p, int rep, int node, double tt, int &it, double &ts, double &ts1,
+    const int (&u)[NC], const double (&vnew)[ND > 0 ? ND : 1], const double *ts_sh)
+{
+    while (it < p.tlen && tt > ts) {
+        const long long col = (long long)rep * p.tlen + it;
+        if (p.U) {
+            int *dst = p.U + (col * p.Nn + node) * NC;
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                dst[c] = u[c];
+            });
+        } else {
+            store_sparse<NC>(p.irU, p.jcU, p.prU, col, node, u);
+        }
+        if constexpr (ND > 0) {
+            if (p.V) {
+                double *dst = p.V + (col * p.Nn + node) * ND;
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    dst[c] = vnew[c];
+                });
+            } else {
+                store_sparse<ND>(p.irV, p.jcV, p.prV, col, node, vnew);
+            }
+        }
+        it++;
+        ts = ts1;
+        ts1 = it + 1 < p.tlen ? tspan_at<TSL>(p, ts_sh, it + 1) : INFINITY;
+    }
+}
+
+/* The end of a day for a free-running lane of a model without events,
+ * post-time-step work or continuous state, dense U: where the day writes
+ * at most one column, the node is not frozen (alive) and the column is
+ * not the last, the column is stored and tt, next and the time points
+ * advance; returns whether it did (the caller's general day end takes the
+ * others).  Straight-line code: the store is masked, the rest selects. */
+template <int NC, bool TSL>
+__device__ __forceinline__ bool day_end_fast(const KParams &p, int rep, int node, bool dend,
+                                             bool alive, const int (&u)[NC], double &tt,
+                                             double &next, int &it, double &ts, double &ts1,
+                                             const double *ts_sh)
+{
+    const double tn = next;
+    const bool col = tn > ts;
+    const bool fast = dend && alive && !(tn > ts1) && it + 1 < p.tlen;
+    if (fast && col) {
+        int *dst = p.U + (((long long)rep * p.tlen + it) * p.Nn + node) * NC;
+        static_for<NC>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            dst[c] = u[c];
+        });
+    }
+    const bool adv = fast && col;
+    it += adv ? 1 : 0;
+    ts = adv ? ts1 : ts;
+    const int i1 = it + 1 < p.tlen ? it + 1 : p.tlen - 1;
+    const double t1 = tspan_at<TSL>(p, ts_sh, i1);
+    ts1 = adv ? (it + 1 < p.tlen ? t1 : INFINITY) : ts1;
+    tt = fast ? tn : tt;
+    next = fast ? tn + 1.0 : next;
+    return fast;
+}
+
 /* Does the model declare that its post-time-step function never changes
  * anything (PTS_NOOP)?  Models that do not say are assumed to. */
 template <class M, class = void>

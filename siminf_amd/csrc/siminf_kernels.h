@@ -239,8 +239,9 @@ __device__ __forceinline__ void ssm_lane(const KParams &p, long long g)
  * same straight-line code as the attempt (fire_sel for models whose S and
  * G fold into the code).  Stream positions and every value are those of
  * ssa_day. */
-template <class M, bool EVENTS, bool CANON>
-__device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g)
+template <class M, bool EVENTS, bool CANON, bool HELP, bool TSL>
+__device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g, int shift,
+                                              const double *ts_sh)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
     const bool live = g >= 0;
@@ -302,7 +303,8 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g)
     }
 
     int U_it = 0, day = 0;
-    double ts = p.tspan[0];
+    double ts = tspan_at<TSL>(p, ts_sh, 0);
+    double ts1 = p.tlen > 1 ? tspan_at<TSL>(p, ts_sh, 1) : INFINITY;
     double t = tt;
     /* b: the next attempt's block (the stream's position is 2b once an
      * attempt of the day has consumed one); bday: b at the day's start;
@@ -321,15 +323,48 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g)
         store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
         going = false;
     }
+    /* With HELP the wavefront's A = 2^shift lanes hold nodes and all 64
+     * lanes compute their attempts' values ahead into a table: lane l
+     * holds block tb + l / A of the node on lane l % A (Q = 64 / A blocks
+     * per node, refilled by the whole wavefront when some node's next
+     * block is past its share), and each node fetches its next block's
+     * values from the lane that computed them (ds_bpermute), one trip
+     * ahead of their use. */
+    const int lane = (int)__lane_id();
+    const int A = 1 << shift, Q = 64 >> shift;
+    const int own = lane & (A - 1), slot = lane >> shift;
+    uint64_t tb = 0;
+    double hE = 0.0, hU = 0.0;
+    bool filled = false;
 
     for (;;) {
         if (__ballot(going) == 0)
             break;
-        /* The attempt with block b; block b + 1 beside it. */
+        /* The attempt with block b; block b + 1's values beside it. */
         const bool att = going && s.a0 > 0.0;
-        double n0, n1;
-        siminf_uniform_pair(&s.rng, b + 1u, &n0, &n1);
-        const double nE = -siminf_log_u01(n0);
+        double nE, n1;
+        if constexpr (HELP) {
+            if (__ballot(going && (!filled || b + 1u - tb >= (uint64_t)Q)) != 0) {
+                tb = b + 1u;
+                siminf_stream st = s.rng;
+                st.id0 = lane_pull(own, s.rng.id0);
+                st.id1 = lane_pull(own, s.rng.id1);
+                const uint64_t t0 = ((uint64_t)lane_pull(own, (uint32_t)(tb >> 32)) << 32) |
+                                    lane_pull(own, (uint32_t)tb);
+                double h0, h1;
+                siminf_uniform_pair(&st, t0 + (uint64_t)slot, &h0, &h1);
+                hE = -siminf_log_u01(h0);
+                hU = h1;
+                filled = true;
+            }
+            const int src = going ? (int)(b + 1u - tb) * A + own : lane;
+            nE = lane_pull(src, hE);
+            n1 = lane_pull(src, hU);
+        } else {
+            double n0;
+            siminf_uniform_pair(&s.rng, b + 1u, &n0, &n1);
+            nE = -siminf_log_u01(n0);
+        }
         const double tau = wE / s.a0;
         bool fires = att && (tau + t) < next;
         t = fires ? t + tau : t;
@@ -390,7 +425,22 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g)
         /* The day ends here unless the lane fired and keeps a positive
          * total rate (ssa_day_wave). */
         const bool dend = going && !(fires && s.a0 > 0.0);
-        if (__ballot(dend) == 0 || !dend)
+        bool slow = dend;
+        if constexpr (!EVENTS && pts_noop<M>::value && ND == 0) {
+            /* Most days end with at most one dense column written and
+             * nothing else to do: that runs as selects and one masked
+             * store, beside the next trip's work; the rest (a node that
+             * freezes or finishes, several columns in one day, sparse
+             * output) takes the general code below. */
+            if (p.U)
+                slow = dend && !day_end_fast<NC, TSL>(p, rep, node, dend, s.a0 > 0.0, s.u,
+                                                       tt, next, U_it, ts, ts1, ts_sh);
+            const bool ended = dend && !slow;
+            t = ended ? tt : t;
+            bday = ended ? b : bday;
+            day += ended ? 1 : 0;
+        }
+        if (__ballot(slow) == 0 || !slow)
             continue;
         bool upd = false;
         if constexpr (EVENTS) {
@@ -427,7 +477,7 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g)
         next += 1.0;
         t = tt;
         bday = b;
-        store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
+        store_columns_ahead<NC, ND, TSL>(p, rep, node, tt, U_it, ts, ts1, s.u, s.vnew, ts_sh);
         if constexpr (ND > 0) {
             static_for<ND>([&](auto C) {
                 constexpr int c = decltype(C)::value;
@@ -459,12 +509,20 @@ __global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
 
 /* The same after the classify pass with free-running lanes: a kernel of
  * its own, so that its registers are allocated for its own code. */
-template <class M, bool EVENTS, bool CANON>
+template <class M, bool EVENTS, bool CANON, bool TSL>
 __global__ __launch_bounds__(256) void k_ssm_nodes_free(KParams p)
 {
+    extern __shared__ double ts_sh[];
+    tspan_to_lds<TSL>(p, ts_sh);
     p.seed = run_seed(p);
-    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
-        ssm_lane_free<M, EVENTS, CANON>(p, node_lane(p, a));
+    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride) {
+        /* Lanes without a node help when the active pass packs fewer
+         * than 64 nodes to a wavefront. */
+        if (p.mode == NODES_ACTIVE && a.shift < 6)
+            ssm_lane_free<M, EVENTS, CANON, true, TSL>(p, node_lane(p, a), a.shift, ts_sh);
+        else
+            ssm_lane_free<M, EVENTS, CANON, false, TSL>(p, node_lane(p, a), 6, ts_sh);
+    }
 }
 
 template <class M, bool EVENTS, bool CANON>
@@ -644,8 +702,9 @@ __device__ __forceinline__ void aem_lane(const KParams &p, long long g)
  * (calc_times_sel, one division per channel), so lanes that fired
  * different transitions run the same instructions.  Values, stream
  * positions and results are aem_day's. */
-template <class M, bool EVENTS, bool CANON>
-__device__ __forceinline__ void aem_lane_free(const KParams &p, long long g)
+template <class M, bool EVENTS, bool CANON, bool HELP, bool TSL>
+__device__ __forceinline__ void aem_lane_free(const KParams &p, long long g, int shift,
+                                              const double *ts_sh)
 {
     constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
     unsigned long long fired = 0;
@@ -741,7 +800,8 @@ __device__ __forceinline__ void aem_lane_free(const KParams &p, long long g)
         s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, 0);
     }
     int U_it = 0, day = 0;
-    double ts = p.tspan[0];
+    double ts = tspan_at<TSL>(p, ts_sh, 0);
+    double ts1 = p.tlen > 1 ? tspan_at<TSL>(p, ts_sh, 1) : INFINITY;
     if (going && p.mode == NODES_ACTIVE && frozen()) {
         store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
         going = false;
@@ -770,15 +830,60 @@ __device__ __forceinline__ void aem_lane_free(const KParams &p, long long g)
         pend &= pend - 1u;
     };
 
+    /* With HELP the wavefront's A = 2^shift lanes hold nodes and all 64
+     * lanes compute logarithms ahead into a table: lane l holds position
+     * tb[c] + k of channel c of the node on lane l % A, with c and k the
+     * remainder and quotient of (l / A) / NT (Q = 64 / (A NT) positions
+     * per channel).  A pending l1 is then fetched from the table
+     * (ds_bpermute) instead of computed; the wavefront refills the table
+     * when some node's pending position is past its share, from each
+     * channel's next needed position. */
+    const int lane = (int)__lane_id();
+    const int A = 1 << shift, Q = 64 / (A * NT);
+    const int own = lane & (A - 1);
+    const int tch = (lane >> shift) % NT, tk = (lane >> shift) / NT;
+    unsigned long long tb[NT];
+    static_for<NT>([&](auto K) { tb[decltype(K)::value] = 0; });
+    double th = 0.0;
+    bool filled = false;
+
     for (;;) {
         if (__ballot(going) == 0)
             break;
-        /* At most one channel is pending here: its new l1, computed
-         * beside the firing. */
+        /* At most one channel is pending here: its new l1, computed (or
+         * fetched) beside the firing. */
         const unsigned pend0 = pend;
         const int kp = pend0 ? __builtin_ctz(pend0) : 0;
         const unsigned long long jp = pick_u64(s.j, kp);
-        const double hp = chan_log(kp, jp + 1u);
+        double hp;
+        if constexpr (HELP) {
+            const bool need = going && pend0 &&
+                              jp + 1u - pick_u64(tb, kp) >= (unsigned long long)Q;
+            if (__ballot(going && (!filled || need)) != 0) {
+                static_for<NT>([&](auto K) {
+                    constexpr int c = decltype(K)::value;
+                    tb[c] = s.j[c] + (((pend0 >> c) & 1u) ? 1u : 2u);
+                });
+                const uint64_t cb =
+                    ((uint64_t)lane_pull(own, (uint32_t)(chan_base >> 32)) << 32) |
+                    lane_pull(own, (uint32_t)chan_base);
+                unsigned long long to[NT];
+                static_for<NT>([&](auto K) {
+                    constexpr int c = decltype(K)::value;
+                    to[c] = ((unsigned long long)lane_pull(own, (uint32_t)(tb[c] >> 32)) << 32) |
+                            lane_pull(own, (uint32_t)tb[c]);
+                });
+                const siminf_stream c = siminf_stream_make(cx.seed, SIMINF_DOMAIN_AEM,
+                                                           cb + (uint64_t)tch);
+                th = siminf_log_u01(siminf_uniform_at(&c, pick_u64(to, tch) + (uint64_t)tk));
+                filled = true;
+            }
+            const int src = going && pend0
+                ? ((int)(jp + 1u - pick_u64(tb, kp)) * NT + kp) * A + own : lane;
+            hp = lane_pull(src, th);
+        } else {
+            hp = chan_log(kp, jp + 1u);
+        }
 
         /* Earliest channel; the first index wins among exact ties. */
         int tr = 0;
@@ -868,7 +973,20 @@ __device__ __forceinline__ void aem_lane_free(const KParams &p, long long g)
             going = false;
         }
         const bool dend = going && !fires;
-        if (__ballot(dend) != 0 && dend) {
+        bool slow = dend;
+        if constexpr (!EVENTS && pts_noop<M>::value && ND == 0) {
+            /* The fast day end of ssm_lane_free. */
+            if (p.U) {
+                bool live_rates = false;
+                static_for<NT>([&](auto J) {
+                    live_rates = live_rates || s.rate[decltype(J)::value] != 0.0;
+                });
+                slow = dend && !day_end_fast<NC, TSL>(p, 0, node, dend, live_rates, s.u, tt,
+                                                       next, U_it, ts, ts1, ts_sh);
+                day += dend && !slow ? 1 : 0;
+            }
+        }
+        if (__ballot(slow) != 0 && slow) {
             bool upd = false;
             bool ok = true;
             if constexpr (EVENTS) {
@@ -914,7 +1032,7 @@ __device__ __forceinline__ void aem_lane_free(const KParams &p, long long g)
             if (ok) {
                 tt = next;
                 next += 1.0;
-                store_columns<NC, ND>(p, 0, node, tt, U_it, ts, s.u, s.vnew);
+                store_columns_ahead<NC, ND, TSL>(p, 0, node, tt, U_it, ts, ts1, s.u, s.vnew, ts_sh);
                 if constexpr (ND > 0) {
                     static_for<ND>([&](auto C) {
                         constexpr int c = decltype(C)::value;
@@ -950,12 +1068,20 @@ __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
         aem_lane<M, EVENTS, CANON>(p, node_lane(p, a));
 }
 
-template <class M, bool EVENTS, bool CANON>
+template <class M, bool EVENTS, bool CANON, bool TSL>
 __global__ __launch_bounds__(256) void k_aem_nodes_free(KParams p)
 {
+    extern __shared__ double ts_sh[];
+    tspan_to_lds<TSL>(p, ts_sh);
     p.seed = run_seed(p);
-    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
-        aem_lane_free<M, EVENTS, CANON>(p, node_lane(p, a));
+    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride) {
+        /* Lanes without a node help when the active pass packs few
+         * enough nodes to a wavefront for two positions per channel. */
+        if (p.mode == NODES_ACTIVE && (64 >> a.shift) / M::NT >= 2)
+            aem_lane_free<M, EVENTS, CANON, true, TSL>(p, node_lane(p, a), a.shift, ts_sh);
+        else
+            aem_lane_free<M, EVENTS, CANON, false, TSL>(p, node_lane(p, a), 6, ts_sh);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1655,14 +1781,27 @@ void launch_ssm_nodes(const KParams &p, bool events, bool canon,
                       unsigned grid, unsigned block, hipStream_t st)
 {
     if (p.mode != NODES_CLASSIFY && p.free_lanes) {
+        /* tspan in LDS up to SIMINF_TS_LDS_MAX points. */
+        const bool tsl = p.tlen <= SIMINF_TS_LDS_MAX;
+        const size_t lds = tsl ? (size_t)p.tlen * sizeof(double) : 0;
+#define SIMINF_FREE(E, C)                                                                \
+    do {                                                                                 \
+        if (tsl)                                                                         \
+            hipLaunchKernelGGL((k_ssm_nodes_free<M, E, C, true>), dim3(grid), dim3(block),  \
+                               lds, st, p);                                              \
+        else                                                                             \
+            hipLaunchKernelGGL((k_ssm_nodes_free<M, E, C, false>), dim3(grid), dim3(block), \
+                               0, st, p);                                                \
+    } while (0)
         if (events && canon)
-            hipLaunchKernelGGL((k_ssm_nodes_free<M, true, true>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(true, true);
         else if (events)
-            hipLaunchKernelGGL((k_ssm_nodes_free<M, true, false>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(true, false);
         else if (canon)
-            hipLaunchKernelGGL((k_ssm_nodes_free<M, false, true>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(false, true);
         else
-            hipLaunchKernelGGL((k_ssm_nodes_free<M, false, false>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(false, false);
+#undef SIMINF_FREE
         return;
     }
     if (events && canon)
@@ -1680,14 +1819,27 @@ void launch_aem_nodes(const KParams &p, bool events, bool canon,
                       unsigned grid, unsigned block, hipStream_t st)
 {
     if (p.mode != NODES_CLASSIFY && p.free_lanes) {
+        /* tspan in LDS up to SIMINF_TS_LDS_MAX points. */
+        const bool tsl = p.tlen <= SIMINF_TS_LDS_MAX;
+        const size_t lds = tsl ? (size_t)p.tlen * sizeof(double) : 0;
+#define SIMINF_FREE(E, C)                                                                \
+    do {                                                                                 \
+        if (tsl)                                                                         \
+            hipLaunchKernelGGL((k_aem_nodes_free<M, E, C, true>), dim3(grid), dim3(block),  \
+                               lds, st, p);                                              \
+        else                                                                             \
+            hipLaunchKernelGGL((k_aem_nodes_free<M, E, C, false>), dim3(grid), dim3(block), \
+                               0, st, p);                                                \
+    } while (0)
         if (events && canon)
-            hipLaunchKernelGGL((k_aem_nodes_free<M, true, true>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(true, true);
         else if (events)
-            hipLaunchKernelGGL((k_aem_nodes_free<M, true, false>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(true, false);
         else if (canon)
-            hipLaunchKernelGGL((k_aem_nodes_free<M, false, true>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(false, true);
         else
-            hipLaunchKernelGGL((k_aem_nodes_free<M, false, false>), dim3(grid), dim3(block), 0, st, p);
+            SIMINF_FREE(false, false);
+#undef SIMINF_FREE
         return;
     }
     if (events && canon)
