@@ -183,7 +183,7 @@ def bench_batch(args):
     elif not args.no_autotune:
         # The launch shape is tuned for the plans together: each plan's
         # lanes per wavefront and the graph against the direct enqueue.
-        ms, shifts, graph = batch.autotune(seed=200, reps=2)
+        ms, shifts, graph = batch.autotune(seed=200, reps=5)
         tuned = {"lanes_per_wave_shift": dict(zip(solvers, shifts)),
                  "launch": "graph" if graph else "streams", "tuned_ms": ms}
     for i in range(args.warmup):

@@ -645,6 +645,54 @@ __device__ __forceinline__ double lane_pull(int src, double v)
     return siminf_from_bits(((uint64_t)hi << 32) | lo);
 }
 
+/* Values of the lane l ^ 1 or l ^ 2 of the same quad (DPP quad_perm
+ * moves; every lane of the quad must be active). */
+template <int X>
+__device__ __forceinline__ uint32_t quad_xor(uint32_t v)
+{
+    static_assert(X == 1 || X == 2, "quad lanes");
+    constexpr int ctrl = X == 1 ? 0xB1 : 0x4E;   /* quad_perm(1,0,3,2), (2,3,0,1) */
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, ctrl, 0xF, 0xF, false);
+}
+
+template <int X>
+__device__ __forceinline__ double quad_xor(double v)
+{
+    const uint64_t b = siminf_bits(v);
+    const uint32_t lo = quad_xor<X>((uint32_t)b);
+    const uint32_t hi = quad_xor<X>((uint32_t)(b >> 32));
+    return siminf_from_bits(((uint64_t)hi << 32) | lo);
+}
+
+/* The earliest (t, k) of the quad: smaller t, the smaller k among exact
+ * ties (the first channel wins, as aem_day's scan). */
+__device__ __forceinline__ void quad_argmin(double &t, int &k)
+{
+    {
+        const double t1 = quad_xor<1>(t);
+        const int k1 = (int)quad_xor<1>((uint32_t)k);
+        const bool take = t1 < t || (t1 == t && k1 < k);
+        t = take ? t1 : t;
+        k = take ? k1 : k;
+    }
+    {
+        const double t1 = quad_xor<2>(t);
+        const int k1 = (int)quad_xor<2>((uint32_t)k);
+        const bool take = t1 < t || (t1 == t && k1 < k);
+        t = take ? t1 : t;
+        k = take ? k1 : k;
+    }
+}
+
+/* OR of a flag over the quad. */
+__device__ __forceinline__ bool quad_any(bool b)
+{
+    uint32_t x = b ? 1u : 0u;
+    x |= quad_xor<1>(x);
+    x |= quad_xor<2>(x);
+    return x != 0u;
+}
+
 /* Position of the (x+1)-th set bit of m (x < popcount(m)). */
 __device__ __forceinline__ int nth_set_bit(uint64_t m, int x)
 {
@@ -934,12 +982,11 @@ __device__ __forceinline__ void calc_times_pre(AemState<M> &s, double logu, doub
  * and the results are selected; tt + q and q + tt are the same IEEE sum.
  * Returns whether the channel drew (used logu: its stream position then
  * advances by one, which the caller applies). */
-template <class M, int J>
-__device__ __forceinline__ bool calc_times_sel(AemState<M> &s, double logu, double tt,
-                                               double old_rate, double new_rate, bool on)
+__device__ __forceinline__ bool calc_times_one(double &time, double &rinf, double logu,
+                                               double tt, double old_rate, double new_rate,
+                                               bool on)
 {
-    const double oldtime = s.time[J];
-    const double rinf = s.inf[J];
+    const double oldtime = time;
     const bool oi = isinf(oldtime);
     const bool big = new_rate >= 2.2250738585072014e-308;
     const bool draw = oi ? rinf == 0.0 : (big && oldtime == tt);
@@ -952,9 +999,16 @@ __device__ __forceinline__ bool calc_times_sel(AemState<M> &s, double logu, doub
     double tnew = oldtime;
     tnew = (draw || wake || resc) ? tn : tnew;
     tnew = sleep ? INFINITY : tnew;
-    s.time[J] = on ? tnew : oldtime;
-    s.inf[J] = on && sleep ? (oldtime - tt) * old_rate : rinf;
+    time = on ? tnew : oldtime;
+    rinf = on && sleep ? (oldtime - tt) * old_rate : rinf;
     return on && draw;
+}
+
+template <class M, int J>
+__device__ __forceinline__ bool calc_times_sel(AemState<M> &s, double logu, double tt,
+                                               double old_rate, double new_rate, bool on)
+{
+    return calc_times_one(s.time[J], s.inf[J], logu, tt, old_rate, new_rate, on);
 }
 
 /* aem_fire with every channel's next logarithm given (lv). */
@@ -1807,11 +1861,14 @@ __device__ __forceinline__ void tspan_to_lds(const KParams &p, double *ts_sh)
 template <int NC, int ND, bool TSL>
 __device__ __forceinline__ void store_columns_ahead(
     const KParams &p, int rep, int node, double tt, int &it, double &ts, double &ts1,
-    const int (&u)[NC], const double (&vnew)[ND > 0 ? ND : 1], const double *ts_sh)
+    const int (&u)[NC], const double (&vnew)[ND > 0 ? ND : 1], const double *ts_sh,
+    bool writer = true)
 {
     while (it < p.tlen && tt > ts) {
         const long long col = (long long)rep * p.tlen + it;
-        if (p.U) {
+        if (!writer) {
+            /* another lane of the node writes */
+        } else if (p.U) {
             int *dst = p.U + (col * p.Nn + node) * NC;
             static_for<NC>([&](auto C) {
                 constexpr int c = decltype(C)::value;
@@ -1821,7 +1878,8 @@ __device__ __forceinline__ void store_columns_ahead(
             store_sparse<NC>(p.irU, p.jcU, p.prU, col, node, u);
         }
         if constexpr (ND > 0) {
-            if (p.V) {
+            if (!writer) {
+            } else if (p.V) {
                 double *dst = p.V + (col * p.Nn + node) * ND;
                 static_for<ND>([&](auto C) {
                     constexpr int c = decltype(C)::value;
@@ -1847,12 +1905,12 @@ template <int NC, bool TSL>
 __device__ __forceinline__ bool day_end_fast(const KParams &p, int rep, int node, bool dend,
                                              bool alive, const int (&u)[NC], double &tt,
                                              double &next, int &it, double &ts, double &ts1,
-                                             const double *ts_sh)
+                                             const double *ts_sh, bool writer = true)
 {
     const double tn = next;
     const bool col = tn > ts;
     const bool fast = dend && alive && !(tn > ts1) && it + 1 < p.tlen;
-    if (fast && col) {
+    if (fast && col && writer) {
         int *dst = p.U + (((long long)rep * p.tlen + it) * p.Nn + node) * NC;
         static_for<NC>([&](auto C) {
             constexpr int c = decltype(C)::value;
@@ -1947,6 +2005,17 @@ __device__ __forceinline__ long long node_lane(const KParams &p, const WaveSlots
     if (l >= (1 << a.shift))
         return -1;
     const long long i = (a.w << a.shift) + l;
+    return i < a.n ? (long long)p.active[i] : -1;
+}
+
+/* node_lane for kernels that give each node a quad of lanes (the
+ * channel-parallel aem): lane l runs node l / 4 of its wave slot. */
+__device__ __forceinline__ long long node_lane_quad(const KParams &p, const WaveSlots &a)
+{
+    const int q = (int)((threadIdx.x & 63) >> 2);
+    if (q >= (1 << a.shift))
+        return -1;
+    const long long i = (a.w << a.shift) + q;
     return i < a.n ? (long long)p.active[i] : -1;
 }
 

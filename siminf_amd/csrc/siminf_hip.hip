@@ -923,55 +923,86 @@ __global__ void k_run_init(unsigned long long *err_key, unsigned long long *tran
             rep_err[i] = ~0ull;
 }
 
-/* Enqueue one run of the plan on its streams: kernels, timing events and
- * stream-event dependencies only, nothing that waits for the device, so
- * the sequence can be captured in a graph (where the timing events become
- * event-record nodes; a grid-wide coupled plan, whose kernel needs a
- * cooperative launch, is refused).  *filled tells whether the
- * frozen-node fill kernel ran. */
-static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, bool *filled)
+/* Enqueue one run of a plan on its streams in three phases: (0) the
+ * run's initialisation and, for node plans with a fill kernel, the
+ * classify pass; (1) the solver: the active pass or the single kernel;
+ * (2) the fill kernel on the second stream, joined back.  A batch runs
+ * each phase for every plan before the next phase, so that the small
+ * kernels of all plans reach the device before any fill kernel, whose
+ * grid holds the dispatcher until it drains.  A single run takes the
+ * phases in turn.  filled: whether the fill kernel ran (set in phase 0).
+ * With capture the timing events are recorded as graph nodes. */
+static int plan_enqueue_phase(SimInf_hip_plan *P, const KParams &base, bool capture,
+                              int phase, bool *filled, hipEvent_t after_fill = nullptr)
 {
     KParams p = base;
     hipStream_t st = P->stream;
-    *filled = false;
     if (capture && P->kernel == K_COUPLED_GRID)
         return SIMINF_ERR_INVALID_MODEL;
     auto mark = [&](hipEvent_t ev, hipStream_t s) {
         (void)hipEventRecordWithFlags(ev, s, capture ? hipEventRecordExternal
                                                      : hipEventRecordDefault);
     };
-    mark(P->ev_start, st);
-    hipLaunchKernelGGL(k_run_init, dim3(1), dim3(256), 0, st, p.err_key, p.transitions,
-                       p.frozen_cols, p.active_count, p.gsync, p.rep_err, P->Nrep);
-    switch (P->kernel) {
-    case K_SSM_NODES:
-    case K_AEM_NODES: {
-        auto launch = [&](const KParams &q, unsigned grid) {
-            if (P->kernel == K_SSM_NODES)
-                P->model->ssm_nodes(q, P->has_e1, P->canon, grid, P->block, st);
-            else
-                P->model->aem_nodes(q, P->has_e1, P->canon, grid, P->block, st);
-        };
-        if (!p.frozen_from) {
-            p.mode = NODES_ALL;
+    const bool nodes = P->kernel == K_SSM_NODES || P->kernel == K_AEM_NODES;
+    auto launch = [&](const KParams &q, unsigned grid) {
+        if (P->kernel == K_SSM_NODES)
+            P->model->ssm_nodes(q, P->has_e1, P->canon, grid, P->block, st);
+        else
+            P->model->aem_nodes(q, P->has_e1, P->canon, grid, P->block, st);
+    };
+    if (phase == 0) {
+        *filled = nodes && p.frozen_from;
+        mark(P->ev_start, st);
+        hipLaunchKernelGGL(k_run_init, dim3(1), dim3(256), 0, st, p.err_key, p.transitions,
+                           p.frozen_cols, p.active_count, p.gsync, p.rep_err, P->Nrep);
+        if (*filled) {
+            /* Classify every lane; nodes frozen from the start go to the
+             * fill kernel on the second stream while the active pass runs
+             * the others, packed so that they spread over the SIMDs (the
+             * pass reads how many there are on the device). */
+            p.mode = NODES_CLASSIFY;
             launch(p, P->grid);
+            mark(P->ev_classified, st);
+            if (hipEventRecord(P->dep_fork, st) != hipSuccess)
+                return SIMINF_ERR_DEVICE;
+        }
+    } else if (phase == 1) {
+        switch (P->kernel) {
+        case K_SSM_NODES:
+        case K_AEM_NODES:
+            if (!p.frozen_from) {
+                p.mode = NODES_ALL;
+                launch(p, P->grid);
+                mark(P->ev_mid, st);
+            } else {
+                p.mode = NODES_ACTIVE;
+                p.lpw_shift = P->lpw_fixed;
+                mark(P->ev_mid, st);
+                launch(p, P->grid_active);
+                mark(P->ev_active1, st);
+            }
+            break;
+        case K_COUPLED_GRID:
+            /* Cooperative launches of reps_per_launch replicates each. */
+            if (launch_grid_reps(P, p, st) != hipSuccess)
+                return SIMINF_ERR_DEVICE;
+            mark(P->ev_mid, st);
+            break;
+        case K_COUPLED_REPS:
+            if (P->model->coupled(p, P->canon, false, P->grid, P->block, P->lds, st,
+                                  nullptr) != hipSuccess)
+                return SIMINF_ERR_DEVICE;
             mark(P->ev_mid, st);
             break;
         }
-        /* Classify every lane; nodes frozen from the start go to the fill
-         * kernel on the second stream while the active pass runs the
-         * others, packed so that they spread over the SIMDs (the pass
-         * reads how many there are on the device). */
-        p.mode = NODES_CLASSIFY;
-        launch(p, P->grid);
-        mark(P->ev_classified, st);
-        if (hipEventRecord(P->dep_fork, st) != hipSuccess ||
-            hipStreamWaitEvent(P->fill_stream, P->dep_fork, 0) != hipSuccess)
-            return SIMINF_ERR_DEVICE;
-        mark(P->ev_fill0, P->fill_stream);
-        {
-            const dim3 fg(fill_grid(P->p.lanes, P->tlen)), fb(256);
+    } else {
+        if (nodes && p.frozen_from) {
             hipStream_t fs = P->fill_stream;
+            if (hipStreamWaitEvent(fs, P->dep_fork, 0) != hipSuccess ||
+                (after_fill && hipStreamWaitEvent(fs, after_fill, 0) != hipSuccess))
+                return SIMINF_ERR_DEVICE;
+            mark(P->ev_fill0, fs);
+            const dim3 fg(fill_grid(P->p.lanes, P->tlen)), fb(256);
             const bool v = P->Nd > 0;
 #define SIMINF_FILL(NCV)                                                          \
     do {                                                                          \
@@ -990,36 +1021,26 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
             default: SIMINF_FILL(0); break;
             }
 #undef SIMINF_FILL
+            mark(P->ev_fill1, fs);
+            if (hipEventRecord(P->dep_join, fs) != hipSuccess ||
+                hipStreamWaitEvent(st, P->dep_join, 0) != hipSuccess)
+                return SIMINF_ERR_DEVICE;
         }
-        mark(P->ev_fill1, P->fill_stream);
-        if (hipEventRecord(P->dep_join, P->fill_stream) != hipSuccess)
+        mark(P->ev_stop, st);
+        if (hipGetLastError() != hipSuccess)
             return SIMINF_ERR_DEVICE;
-        p.mode = NODES_ACTIVE;
-        p.lpw_shift = P->lpw_fixed;
-        mark(P->ev_mid, st);
-        launch(p, P->grid_active);
-        mark(P->ev_active1, st);
-        if (hipStreamWaitEvent(st, P->dep_join, 0) != hipSuccess)
-            return SIMINF_ERR_DEVICE;
-        *filled = true;
-        break;
     }
-    case K_COUPLED_GRID:
-        /* Cooperative launches of reps_per_launch replicates each. */
-        if (launch_grid_reps(P, p, st) != hipSuccess)
-            return SIMINF_ERR_DEVICE;
-        break;
-    case K_COUPLED_REPS:
-        if (P->model->coupled(p, P->canon, false, P->grid, P->block, P->lds, st,
-                              nullptr) != hipSuccess)
-            return SIMINF_ERR_DEVICE;
-        break;
+    return 0;
+}
+
+/* All three phases of one run. */
+static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, bool *filled)
+{
+    for (int ph = 0; ph < 3; ph++) {
+        const int err = plan_enqueue_phase(P, base, capture, ph, filled);
+        if (err)
+            return err;
     }
-    if (P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES)
-        mark(P->ev_mid, st);
-    mark(P->ev_stop, st);
-    if (hipGetLastError() != hipSuccess)
-        return SIMINF_ERR_DEVICE;
     return 0;
 }
 
@@ -1435,19 +1456,26 @@ static int batch_capture(SimInf_hip_batch *B)
     if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return SIMINF_ERR_DEVICE;
     for (int i = 0; i < n && !err; i++) {
-        SimInf_hip_plan *P = B->plans[i];
-        KParams p = P->p;
-        p.seed_dev = B->seeds + i;
-        bool filled = false;
         if (hipEventRecord(B->fork[i], B->stream) != hipSuccess ||
-            hipStreamWaitEvent(P->stream, B->fork[i], 0) != hipSuccess) {
+            hipStreamWaitEvent(B->plans[i]->stream, B->fork[i], 0) != hipSuccess)
             err = SIMINF_ERR_DEVICE;
-            break;
+    }
+    for (int ph = 0; ph < 3 && !err; ph++) {
+        for (int i = 0; i < n && !err; i++) {
+            SimInf_hip_plan *P = B->plans[i];
+            KParams p = P->p;
+            p.seed_dev = B->seeds + i;
+            bool filled = B->filled[i] != 0;
+            /* The captured graph leaves the fills side by side (no wait of
+             * one plan's fill stream on another's, a form not captured
+             * here). */
+            err = plan_enqueue_phase(P, p, true, ph, &filled);
+            B->filled[i] = filled;
         }
-        err = plan_enqueue(P, p, true, &filled);
-        B->filled[i] = filled;
-        if (!err && (hipEventRecord(B->join[i], P->stream) != hipSuccess ||
-                     hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess))
+    }
+    for (int i = 0; i < n && !err; i++) {
+        if (hipEventRecord(B->join[i], B->plans[i]->stream) != hipSuccess ||
+            hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess)
             err = SIMINF_ERR_DEVICE;
     }
     hipGraph_t g = nullptr;
@@ -1511,21 +1539,42 @@ extern "C" int SimInf_hip_batch_set_graph(SimInf_hip_batch *B, int use_graph)
 static int batch_run_streams(SimInf_hip_batch *B, const unsigned long *seeds)
 {
     const int n = (int)B->plans.size();
+    /* The plans go in by their last solver time, longest first: the
+     * host's enqueue of each plan delays the next one's start, which is
+     * best spent on plans that have time to spare. */
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; i++)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return B->plans[a]->solver_ms > B->plans[b]->solver_ms;
+    });
     if (hipEventRecord(B->ev0, B->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
-    for (int i = 0; i < n; i++) {
-        SimInf_hip_plan *P = B->plans[i];
-        KParams p = P->p;
-        p.seed = (unsigned long long)seeds[i];
-        p.seed_dev = nullptr;
-        bool filled = false;
-        if (hipStreamWaitEvent(P->stream, B->ev0, 0) != hipSuccess)
+    for (int i : order)
+        if (hipStreamWaitEvent(B->plans[i]->stream, B->ev0, 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
-        const int err = plan_enqueue(P, p, false, &filled);
-        if (err)
-            return err;
-        B->filled[i] = filled;
-        if (hipEventRecord(B->join[i], P->stream) != hipSuccess ||
+    /* The fill kernels run one after another (each after the previous
+     * plan's fill): they are bound by HBM writes, so running them at once
+     * takes as long as in turn, and in turn each one's timing is its
+     * own. */
+    for (int ph = 0; ph < 3; ph++) {
+        hipEvent_t prev = nullptr;
+        for (int i : order) {
+            SimInf_hip_plan *P = B->plans[i];
+            KParams p = P->p;
+            p.seed = (unsigned long long)seeds[i];
+            p.seed_dev = nullptr;
+            bool filled = B->filled[i] != 0;
+            const int err = plan_enqueue_phase(P, p, false, ph, &filled, ph == 2 ? prev : nullptr);
+            if (err)
+                return err;
+            B->filled[i] = filled;
+            if (ph == 2 && filled)
+                prev = P->dep_join;
+        }
+    }
+    for (int i : order) {
+        if (hipEventRecord(B->join[i], B->plans[i]->stream) != hipSuccess ||
             hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
     }

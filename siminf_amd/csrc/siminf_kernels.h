@@ -497,6 +497,115 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g, int
     add_transitions(p, fired);
 }
 
+/* The classify pass of the node kernels on its own: one lane per
+ * (replicate, node) computes the initial rates (an invalid one is the
+ * run's PH_INIT error), hands a node that can never change (every rate
+ * zero, or a0 <= 0 for ssm; no events; a post-time-step function that
+ * does nothing) to the fill kernel and lists the others for the active
+ * pass -- what ssm_lane and aem_lane decide in classify mode, in a kernel
+ * small enough to fill the device (the solvers' loops hold a wavefront
+ * to one or two per SIMD). */
+template <class M, bool EVENTS, bool CANON, bool AEM>
+__global__ __launch_bounds__(1024) void k_classify(KParams p)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    /* The active list and the frozen-column count take one atomic per
+     * workgroup (the listed lanes are written at offsets from a prefix
+     * over the workgroup's wavefronts): one per wavefront on two counters
+     * serialised at the cache and cost as much as the pass itself. */
+    __shared__ unsigned s_cnt[16];
+    __shared__ unsigned long long s_frz[16];
+    __shared__ int s_base;
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int nw = (int)(blockDim.x >> 6);
+    unsigned long long frozen_cols = 0;
+    for (long long g0 = (long long)blockIdx.x * blockDim.x; g0 < p.lanes;
+         g0 += (long long)gridDim.x * blockDim.x) {
+        const long long g = g0 + threadIdx.x;
+        const bool live = g < p.lanes;
+        bool push = false;
+        if (live) {
+            const int rep = (int)(g / p.Nn);
+            const int node = (int)(g - (long long)rep * p.Nn);
+            Ctx<M, CANON> cx;
+            cx.load(p, node, rep);
+            int u[NC];
+            double v[ND > 0 ? ND : 1];
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                u[c] = p.u0[g * NC + c];
+            });
+            if constexpr (ND > 0) {
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    v[c] = p.v0[g * ND + c];
+                });
+            } else {
+                v[0] = 0.0;
+            }
+            int err = 0;
+            double a0 = 0.0;
+            bool zero = true;
+            static_for<NT>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                const double r = M::template rate<j>(u, v, cx.ld, cx.gd, p.tt0);
+                a0 += r;
+                zero = zero && r == 0.0;
+                if (!rate_ok(r))
+                    err = SIMINF_ERR_INVALID_RATE;
+            });
+            bool idle = AEM ? zero : a0 <= 0.0;
+            if constexpr (EVENTS) {
+                E1Cursor ec;
+                ec.start(p, node);
+                idle = idle && ec.cur >= ec.end;
+            }
+            bool frz = false;
+            if (err) {
+                record_error(p, rep, 0, PH_INIT, node, err);
+            } else if (pts_noop<M>::value && idle) {
+                record_frozen<NC, ND>(p, g, 0, u, v);
+                frozen_cols += (unsigned long long)p.tlen;
+                frz = true;
+            } else {
+                push = true;
+            }
+            if (!frz)
+                p.frozen_from[g] = p.tlen;
+        }
+        const unsigned long long m = __ballot(push);
+        if (lane == 0)
+            s_cnt[wave] = (unsigned)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned tot = 0;
+            for (int w = 0; w < nw; w++) {
+                const unsigned c = s_cnt[w];
+                s_cnt[w] = tot;
+                tot += c;
+            }
+            s_base = tot ? atomicAdd(p.active_count, (int)tot) : 0;
+        }
+        __syncthreads();
+        if (push)
+            p.active[s_base + (int)s_cnt[wave] +
+                     __popcll(m & ((1ull << lane) - 1ull))] = (int)g;
+        __syncthreads();
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        frozen_cols += __shfl_xor(frozen_cols, off);
+    if (lane == 0)
+        s_frz[wave] = frozen_cols;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < nw; w++)
+            tot += s_frz[w];
+        if (tot)
+            atomicAdd(p.frozen_cols, tot);
+    }
+}
+
 /* ssm / mssm over nodes that never interact: each wavefront runs its
  * wave slots (one, its own lanes, except in the active pass). */
 template <class M, bool EVENTS, bool CANON>
@@ -1059,6 +1168,293 @@ __device__ __forceinline__ void aem_lane_free(const KParams &p, long long g, int
     add_transitions(p, fired);
 }
 
+/* aem with each node's channels on lanes of their own, for models whose S
+ * and G fold into the code and that have at most four transitions: a
+ * node holds a quad of lanes, lane c of the quad runs channel c (its
+ * rate, next firing time, reactInf and stream) and every lane of the quad
+ * keeps the node's state.  A firing is the quad's earliest channel (DPP
+ * moves, the first channel winning exact ties as in aem_day); each lane
+ * then applies the transition, recomputes its own channel's rate if G
+ * marks it and runs calcTimes for it, so the channels' divisions run side
+ * by side instead of one after another.  A channel's next logarithms (l0,
+ * l1) are kept as in aem_lane_free; with HELP the wavefront's lanes
+ * compute l1 values ahead into a table: for A = 2^shift nodes a channel
+ * lane s = 4 q + c owns positions tb .. tb + Q - 1 (Q = 16 / A) held by
+ * lanes s, s + 4A, ...  Values, stream positions and results are
+ * aem_day's. */
+template <class M, bool EVENTS, bool TSL, bool HELP>
+__device__ __forceinline__ void aem_lane_quad(const KParams &p, long long g, int shift,
+                                              const double *ts_sh)
+{
+    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
+    static_assert(NT <= 4, "a quad of lanes per node");
+    const int lane = (int)__lane_id();
+    const int c = lane & 3;
+    const bool chan = c < NT;
+    const bool lead = c == 0;
+    unsigned long long fired = 0;
+    const bool live = g >= 0;
+    const int node = live ? (int)g : 0;
+    Ctx<M, true> cx;
+    const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
+    int u[NC];
+    double v[ND > 0 ? ND : 1], vnew[ND > 0 ? ND : 1];
+    /* This lane's channel: rate, next firing time, reactInf, stream
+     * position and the logarithms of the uniforms at j and j + 1. */
+    double rate = 0.0, ctime = INFINITY, rinf = 0.0, l0 = 0.0, l1 = 0.0;
+    unsigned long long j = 0;
+    bool pend = false;
+    siminf_stream rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, live ? (uint64_t)g : 0);
+    int err = 0;
+    double tt = p.tt0, next = p.next0;
+    const EvTab tab = global_tables(p);
+    E1Cursor ec;
+    /* The rate of this lane's channel at the given state. */
+    auto own_rate = [&](const int (&uu)[NC], const double *vv, double t)
+        __attribute__((always_inline)) -> double {
+        double r = 0.0;
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            const double rk = M::template rate<k>(uu, vv, cx.ld, cx.gd, t);
+            r = c == k ? rk : r;
+        });
+        return r;
+    };
+    auto chan_log = [&](unsigned long long pos) __attribute__((always_inline)) -> double {
+        const siminf_stream cs = siminf_stream_make(cx.seed, SIMINF_DOMAIN_AEM,
+                                                    chan_base + (uint64_t)(chan ? c : 0));
+        return siminf_log_u01(siminf_uniform_at(&cs, pos));
+    };
+    auto frozen = [&]() __attribute__((always_inline)) -> bool {
+        if constexpr (pts_noop<M>::value)
+            return (!EVENTS || ec.cur >= ec.end) && !quad_any(chan && rate != 0.0);
+        else
+            return false;
+    };
+    /* A draw by this lane's channel: l1 moves down, the new l1 pends. */
+    auto took = [&](bool drew) __attribute__((always_inline)) {
+        j += drew ? 1u : 0u;
+        l0 = drew ? l1 : l0;
+        pend = pend || drew;
+    };
+
+    bool going = false;
+    if (live) {
+        cx.load(p, node, 0);
+        static_for<NC>([&](auto C) {
+            constexpr int cc = decltype(C)::value;
+            u[cc] = p.u0[g * NC + cc];
+        });
+        if constexpr (ND > 0) {
+            static_for<ND>([&](auto C) {
+                constexpr int cc = decltype(C)::value;
+                v[cc] = p.v0[g * ND + cc];
+                vnew[cc] = v[cc];
+            });
+        }
+    } else {
+        cx.clear(p);
+        static_for<NC>([&](auto C) { u[decltype(C)::value] = 0; });
+        static_for<(ND > 0 ? ND : 1)>([&](auto C) {
+            v[decltype(C)::value] = 0.0;
+            vnew[decltype(C)::value] = 0.0;
+        });
+    }
+    {
+        /* Initial rates and reaction times (aem.c:117-153). */
+        bool bad = false;
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            bad = bad || !rate_ok(M::template rate<k>(u, v, cx.ld, cx.gd, tt));
+        });
+        rate = chan ? own_rate(u, v, tt) : 0.0;
+        const siminf_stream cs = siminf_stream_make(p.seed, SIMINF_DOMAIN_AEM,
+                                                    chan_base + (uint64_t)(chan ? c : 0));
+        double u0, u1, u2, u3;
+        siminf_uniform_pair(&cs, 0, &u0, &u1);
+        siminf_uniform_pair(&cs, 1, &u2, &u3);
+        double tj = -siminf_log(u0) / rate + tt;
+        tj = tj <= 0.0 ? INFINITY : tj;
+        ctime = chan ? tj : INFINITY;
+        j = 1;
+        l0 = siminf_log_u01(u1);
+        l1 = siminf_log_u01(u2);
+        if (live) {
+            if constexpr (EVENTS)
+                ec.start(p, node);
+            if (bad) {
+                if (lead)
+                    record_error(p, 0, 0, PH_INIT, node, SIMINF_ERR_INVALID_RATE);
+            } else {
+                going = true;
+            }
+        }
+    }
+    int U_it = 0, day = 0;
+    double ts = tspan_at<TSL>(p, ts_sh, 0);
+    double ts1 = p.tlen > 1 ? tspan_at<TSL>(p, ts_sh, 1) : INFINITY;
+    if (going && p.mode == NODES_ACTIVE && frozen()) {
+        if (lead)
+            store_frozen<NC, ND>(p, g, U_it, u, vnew);
+        going = false;
+    }
+
+    const int A = 1 << shift, L = 4 << shift, Q = 16 >> shift;
+    const int tslot = lane / L, towner = lane % L;
+    unsigned long long tb = 0;
+    double th = 0.0;
+    bool filled = false;
+    (void)A;
+
+    for (;;) {
+        if (__ballot(going) == 0)
+            break;
+        /* The pending l1 of this lane's channel, fetched or computed
+         * beside the firing. */
+        const bool pend0 = pend && going;
+        double hp;
+        if constexpr (HELP) {
+            if (__ballot(pend0 && (!filled || j + 1u - tb >= (unsigned long long)Q)) != 0) {
+                tb = j + (pend ? 1u : 2u);
+                const uint64_t cb = ((uint64_t)lane_pull(towner, (uint32_t)(chan_base >> 32)) << 32) |
+                                    lane_pull(towner, (uint32_t)chan_base);
+                const uint64_t to = ((uint64_t)lane_pull(towner, (uint32_t)(tb >> 32)) << 32) |
+                                    lane_pull(towner, (uint32_t)tb);
+                const int oc = towner & 3;
+                const siminf_stream cs = siminf_stream_make(cx.seed, SIMINF_DOMAIN_AEM,
+                                                            cb + (uint64_t)(oc < NT ? oc : 0));
+                th = siminf_log_u01(siminf_uniform_at(&cs, to + (uint64_t)tslot));
+                filled = true;
+            }
+            const int src = pend0 ? lane + (int)(j + 1u - tb) * L : lane;
+            hp = lane_pull(src, th);
+        } else {
+            hp = chan_log(j + 1u);
+        }
+
+        /* The quad's earliest channel. */
+        double t = ctime;
+        int tr = chan ? c : 4;
+        quad_argmin(t, tr);
+        const bool fires = going && !(isinf(t) || t >= next);
+        /* The transition's state change and this lane's channel. */
+        int un[NC];
+        bool neg = false;
+        static_for<NC>([&](auto C) {
+            constexpr int cc = decltype(C)::value;
+            int d = 0;
+            static_for<NT>([&](auto K) {
+                constexpr int k = decltype(K)::value;
+                if constexpr (M::S[k][cc] != 0)
+                    d += tr == k ? M::S[k][cc] : 0;
+            });
+            un[cc] = u[cc] + d;
+            neg = neg || un[cc] < 0;
+        });
+        unsigned gm = 0;
+        static_for<NT>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            gm = c == k ? (g_row_mask<M, k>() | (1u << k)) : gm;
+        });
+        const bool in = fires && chan && ((gm >> tr) & 1u);
+        const double r = own_rate(un, v, t);
+        const bool drew = calc_times_one(ctime, rinf, l0, t, rate, r, in);
+        const bool badr = quad_any(in && !rate_ok(r));
+        rate = in ? r : rate;
+        static_for<NC>([&](auto C) {
+            constexpr int cc = decltype(C)::value;
+            u[cc] = fires ? un[cc] : u[cc];
+        });
+        fired += fires && lead ? 1u : 0u;
+        /* The pending l1 lands before this firing's draw moves it down. */
+        l1 = pend0 ? hp : l1;
+        pend = pend && !pend0;
+        took(drew);
+        pin(l0);
+        pin(l1);
+        const int e = fires ? (badr ? SIMINF_ERR_INVALID_RATE
+                                    : (neg ? SIMINF_ERR_NEGATIVE_STATE : 0)) : 0;
+        if (e) {
+            if (lead)
+                record_error(p, 0, day, PH_SSA, node, e);
+            going = false;
+        }
+        const bool dend = going && !fires;
+        bool slow = dend;
+        if constexpr (!EVENTS && pts_noop<M>::value && ND == 0) {
+            if (p.U) {
+                const bool alive = quad_any(chan && rate != 0.0);
+                slow = dend && !day_end_fast<NC, TSL>(p, 0, node, dend, alive, u, tt, next,
+                                                       U_it, ts, ts1, ts_sh, lead);
+                day += dend && !slow ? 1 : 0;
+            }
+        }
+        if (__ballot(slow) != 0 && slow) {
+            bool upd = false;
+            bool ok = true;
+            if constexpr (EVENTS) {
+                int bad = 0;
+                err = e1_day<NC>(p, tab, ec, tt, u, &rng, upd, bad);
+                if (err) {
+                    if (lead)
+                        record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
+                    going = false;
+                    ok = false;
+                }
+            }
+            if (ok) {
+                const int rc = M::pts(vnew, u, v, cx.ld, cx.gd, node, tt);
+                if (rc < 0) {
+                    if (lead)
+                        record_error(p, 0, day, PH_PTS, node, rc);
+                    going = false;
+                    ok = false;
+                } else if (rc > 0 || upd) {
+                    bool bad = false;
+                    static_for<NT>([&](auto K) {
+                        constexpr int k = decltype(K)::value;
+                        bad = bad || !rate_ok(M::template rate<k>(u, vnew, cx.ld, cx.gd, tt));
+                    });
+                    const double rr = own_rate(u, vnew, tt);
+                    took(calc_times_one(ctime, rinf, l0, next, rate, rr, chan));
+                    rate = chan ? rr : rate;
+                    if (bad) {
+                        if (lead)
+                            record_error(p, 0, day, PH_PTS, node, SIMINF_ERR_INVALID_RATE);
+                        going = false;
+                        ok = false;
+                    }
+                }
+            }
+            if (ok) {
+                tt = next;
+                next += 1.0;
+                store_columns_ahead<NC, ND, TSL>(p, 0, node, tt, U_it, ts, ts1, u, vnew, ts_sh,
+                                                 lead);
+                if constexpr (ND > 0) {
+                    static_for<ND>([&](auto C) {
+                        constexpr int cc = decltype(C)::value;
+                        const double x = v[cc];
+                        v[cc] = vnew[cc];
+                        vnew[cc] = x;
+                    });
+                }
+                if (U_it >= p.tlen)
+                    going = false;
+                day++;
+                if (going && p.mode == NODES_ACTIVE && frozen()) {
+                    if (lead)
+                        store_frozen<NC, ND>(p, g, U_it, u, vnew);
+                    going = false;
+                }
+            }
+        }
+        /* A refresh's draw may leave a second l1 pending: none is, as
+         * each lane runs one channel and its pending l1 landed above. */
+    }
+    add_transitions(p, fired);
+}
+
 /* aem over nodes that never interact, by wave slots as k_ssm_nodes. */
 template <class M, bool EVENTS, bool CANON>
 __global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
@@ -1075,6 +1471,17 @@ __global__ __launch_bounds__(256) void k_aem_nodes_free(KParams p)
     tspan_to_lds<TSL>(p, ts_sh);
     p.seed = run_seed(p);
     for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride) {
+        if constexpr (CANON && M::NT <= 4) {
+            /* A quad of lanes per node when the active pass packs at most
+             * 16 nodes to a wavefront. */
+            if (p.mode == NODES_ACTIVE && a.shift <= 4) {
+                if (a.shift <= 3)
+                    aem_lane_quad<M, EVENTS, TSL, true>(p, node_lane_quad(p, a), a.shift, ts_sh);
+                else
+                    aem_lane_quad<M, EVENTS, TSL, false>(p, node_lane_quad(p, a), a.shift, ts_sh);
+                continue;
+            }
+        }
         /* Lanes without a node help when the active pass packs few
          * enough nodes to a wavefront for two positions per channel. */
         if (p.mode == NODES_ACTIVE && (64 >> a.shift) / M::NT >= 2)
@@ -1780,6 +2187,20 @@ template <class M>
 void launch_ssm_nodes(const KParams &p, bool events, bool canon,
                       unsigned grid, unsigned block, hipStream_t st)
 {
+    if (p.mode == NODES_CLASSIFY) {
+        /* 1024-thread workgroups, each lane at most a few nodes. */
+        const long long wgs = std::min<long long>((p.lanes + 1023) / 1024, 2048);
+        const dim3 cg((unsigned)std::max<long long>(wgs, 1)), cb(1024);
+        if (events && canon)
+            hipLaunchKernelGGL((k_classify<M, true, true, false>), cg, cb, 0, st, p);
+        else if (events)
+            hipLaunchKernelGGL((k_classify<M, true, false, false>), cg, cb, 0, st, p);
+        else if (canon)
+            hipLaunchKernelGGL((k_classify<M, false, true, false>), cg, cb, 0, st, p);
+        else
+            hipLaunchKernelGGL((k_classify<M, false, false, false>), cg, cb, 0, st, p);
+        return;
+    }
     if (p.mode != NODES_CLASSIFY && p.free_lanes) {
         /* tspan in LDS up to SIMINF_TS_LDS_MAX points. */
         const bool tsl = p.tlen <= SIMINF_TS_LDS_MAX;
@@ -1818,6 +2239,20 @@ template <class M>
 void launch_aem_nodes(const KParams &p, bool events, bool canon,
                       unsigned grid, unsigned block, hipStream_t st)
 {
+    if (p.mode == NODES_CLASSIFY) {
+        /* 1024-thread workgroups, each lane at most a few nodes. */
+        const long long wgs = std::min<long long>((p.lanes + 1023) / 1024, 2048);
+        const dim3 cg((unsigned)std::max<long long>(wgs, 1)), cb(1024);
+        if (events && canon)
+            hipLaunchKernelGGL((k_classify<M, true, true, true>), cg, cb, 0, st, p);
+        else if (events)
+            hipLaunchKernelGGL((k_classify<M, true, false, true>), cg, cb, 0, st, p);
+        else if (canon)
+            hipLaunchKernelGGL((k_classify<M, false, true, true>), cg, cb, 0, st, p);
+        else
+            hipLaunchKernelGGL((k_classify<M, false, false, true>), cg, cb, 0, st, p);
+        return;
+    }
     if (p.mode != NODES_CLASSIFY && p.free_lanes) {
         /* tspan in LDS up to SIMINF_TS_LDS_MAX points. */
         const bool tsl = p.tlen <= SIMINF_TS_LDS_MAX;
