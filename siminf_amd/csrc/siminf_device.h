@@ -208,14 +208,10 @@ struct KParams {
      * SIMD of the device's simds, at most 64 lanes).  The active pass
      * reads the count on the device: its grid is fixed, and each
      * wavefront takes the wave slots w, w + (wavefronts in the grid),
-     * ... below the slots the count needs.  free_lanes: after the
-     * classify pass each lane steps through its own days (one attempt or
-     * one day's end per loop trip) instead of the wavefront stepping day
-     * by day. */
+     * ... below the slots the count needs. */
     int mode;
     int lpw_shift;
     int simds;
-    int free_lanes;
     int *active;
     int *active_count;
 
@@ -727,7 +723,7 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int x)
  * block and the logarithm, which are most of an attempt's instructions,
  * leave the busiest lane's chain.  Stream positions and every value are
  * those of ssa_day. */
-template <class M, bool DENSE = false, class CT>
+template <class M, class CT>
 __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
                                             double tt, double next,
                                             unsigned long long &fired,
@@ -745,20 +741,7 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
         const uint64_t running = __ballot(run);
         if (running == 0)
             break;
-        bool own = false;
-        if (DENSE && running == ~0ull) {
-            /* Every lane running: each computes its own next attempt
-             * (compiled in for kernels whose wavefronts are often full). */
-            double u0, u1;
-            siminf_uniform_pair(&s.rng, b, &u0, &u1);
-            hE = -siminf_log_u01(u0);
-            hU = u1;
-            A = 64;
-            rank = lane;
-            have = 1;
-            pos = 0;
-            own = true;
-        } else if (__ballot(run && pos >= have) != 0) {
+        if (__ballot(run && pos >= have) != 0) {
             A = __popcll(running);
             /* l / A and l % A for l < 64 by a float reciprocal: (l + 0.5) / A
              * is at least 0.5 / A away from an integer, far beyond the
@@ -788,8 +771,8 @@ __device__ __forceinline__ int ssa_day_wave(NodeState<M> &s, const CT &cx,
         /* Every lane takes part in the pulls; only running lanes use
          * them. */
         const int src = run ? pos * A + rank : lane;
-        const double E = DENSE && own ? hE : lane_pull(src, hE);
-        const double u_tr = DENSE && own ? hU : lane_pull(src, hU);
+        const double E = lane_pull(src, hE);
+        const double u_tr = lane_pull(src, hU);
         if (run) {
             b++;
             pos++;
@@ -1043,131 +1026,6 @@ __device__ __forceinline__ int aem_fire_pre(AemState<M> &s, const CT &cx, double
         if (!rate_ok(r))
             err = SIMINF_ERR_INVALID_RATE;
         calc_times_pre<M, K>(s, lv[K], t, old, r);
-    }
-    return err;
-}
-
-/* The logarithms of the channels' next uniforms, computed ahead by a
- * wavefront for its live lanes (aem_day_wave).  With A live lanes and NT
- * channels every live lane owns Q = 64 / (A NT) slots per channel: lane l
- * holds slot l / (A NT) of channel (l / A) % NT of the live lane of rank
- * l % A, the logarithm of the uniform at position base + slot of that
- * channel's stream, base being the channel's position at the last
- * refill. */
-template <class M>
-struct AemTable {
-    unsigned long long base[M::NT];
-    double h;
-    uint64_t live;
-    int A, rank, Q;
-    bool filled;
-};
-
-template <class M>
-__device__ __forceinline__ void aem_table_init(AemTable<M> &tb, bool live)
-{
-    tb.live = __ballot(live);
-    tb.A = __popcll(tb.live);
-    tb.rank = (int)__builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(tb.live >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tb.live, 0u));
-    tb.Q = tb.A > 0 ? 64 / (tb.A * M::NT) : 0;
-    tb.filled = false;
-    tb.h = 0.0;
-    static_for<M::NT>([&](auto K) { tb.base[decltype(K)::value] = 0; });
-}
-
-/* aem_day for a whole wavefront: every lane enters, run says whether its
- * node simulates this day.  Channel draws take their logarithms from the
- * table, refilled by all lanes when a running lane has used every slot of
- * some channel; before each firing every lane pulls, for each channel, the
- * value for its lane's next position (ds_bpermute).  Without slots (more
- * than 64 / NT live lanes) each lane runs aem_day.  Values, stream
- * positions and results are aem_day's. */
-template <class M, class CT>
-__device__ __forceinline__ int aem_day_wave(AemState<M> &s, const CT &cx, double next,
-                                            uint64_t chan_base, unsigned long long &fired,
-                                            bool run, AemTable<M> &tb)
-{
-    constexpr int NT = M::NT;
-    if (tb.Q == 0)
-        return run ? aem_day<M>(s, cx, next, chan_base, fired) : 0;
-    int err = 0;
-    const int lane = (int)__lane_id();
-    const bool live = (tb.live >> lane) & 1ull;
-    for (;;) {
-        /* Earliest channel; the first index wins among exact ties. */
-        int tr = 0;
-        double t = INFINITY;
-        if (run) {
-            t = s.time[0];
-            static_for<NT>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                if (k > 0 && s.time[k] < t) {
-                    t = s.time[k];
-                    tr = k;
-                }
-            });
-            if (isinf(t) || t >= next)
-                run = false;
-        }
-        if (__ballot(run) == 0)
-            break;
-        bool need = !tb.filled;
-        if (run) {
-            static_for<NT>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                need = need || s.j[k] - tb.base[k] >= (unsigned long long)tb.Q;
-            });
-        }
-        if (__ballot(need) != 0) {
-            /* Refill: every live lane's slots start at its channels'
-             * current positions. */
-            if (live) {
-                static_for<NT>([&](auto K) {
-                    constexpr int k = decltype(K)::value;
-                    tb.base[k] = s.j[k];
-                });
-            }
-            const int owner = lane % tb.A;
-            const int K = (lane / tb.A) % NT;
-            const int slot = lane / (tb.A * NT);
-            const int c = nth_set_bit(tb.live, owner);
-            const uint64_t cb = ((uint64_t)lane_pull(c, (uint32_t)(chan_base >> 32)) << 32) |
-                                lane_pull(c, (uint32_t)chan_base);
-            unsigned long long bk = 0;
-            static_for<NT>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                const uint64_t bj =
-                    ((uint64_t)lane_pull(c, (uint32_t)(tb.base[j] >> 32)) << 32) |
-                    lane_pull(c, (uint32_t)tb.base[j]);
-                if (K == j)
-                    bk = bj;
-            });
-            const siminf_stream st =
-                siminf_stream_make(cx.seed, SIMINF_DOMAIN_AEM, cb + (uint64_t)K);
-            tb.h = slot < tb.Q ? siminf_log(siminf_uniform_at(&st, bk + (uint64_t)slot)) : 0.0;
-            tb.filled = true;
-        }
-        double lv[NT];
-        static_for<NT>([&](auto K) {
-            constexpr int k = decltype(K)::value;
-            const int sl = run ? (int)(s.j[k] - tb.base[k]) : 0;
-            const int src = run ? tb.rank + tb.A * (k + NT * sl) : lane;
-            lv[k] = lane_pull(src, tb.h);
-        });
-        if (run) {
-            fired++;
-            static_for<NT>([&](auto K) {
-                constexpr int k = decltype(K)::value;
-                if (tr == k) {
-                    const int e = aem_fire_pre<M, k>(s, cx, t, lv);
-                    if (e)
-                        err = e;
-                }
-            });
-            if (err)
-                run = false;
-        }
     }
     return err;
 }

@@ -3,12 +3,19 @@
  * solver core (declarations in include/siminf_hip.h).
  *
  * Kernels
- *   k_ssm_nodes<M>  one lane per (replicate, node), all days in one launch:
+ *   k_classify<M>   one lane per (replicate, node): initial rates, nodes
+ *                   frozen from the start to the fill kernel, the others
+ *                   to the active list.
+ *   k_ssm_nodes_free<M>
  *                   ssm/mssm when no external-transfer (E2) event couples
- *                   the nodes (SimInf_solver_ssm.c:74-373,
+ *                   the nodes, all days in one launch, each lane stepping
+ *                   through its own days (SimInf_solver_ssm.c:74-373,
  *                   SimInf_solver_mssm.c:81-402).
- *   k_aem_nodes<M>  the same for the all-events method
+ *   k_aem_nodes_free<M>
+ *                   the same for the all-events method, a quad of lanes
+ *                   per node where the model allows
  *                   (SimInf_solver_aem.c:93-459).
+ *   k_fill_frozen   the columns of frozen nodes.
  *   k_coupled<M>    one lane per node, barrier-separated days, for models
  *                   with E2 events: one replicate over a cooperative grid
  *                   or one workgroup per replicate; each day's E2 events
@@ -865,7 +872,6 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                 hipSuccess && cus > 0)
             P->simds = 4 * cus;
         p.simds = P->simds;
-        p.free_lanes = 1;
         /* The active pass's fixed grid: eight wavefronts per SIMD (what
          * the node kernels' register use lets the device hold, so that a
          * packing that needs up to that many gives every wave slot a

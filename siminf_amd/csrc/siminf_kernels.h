@@ -83,154 +83,10 @@ __device__ __forceinline__ int e1_day(const KParams &p, const EvTab &tab,
     return 0;
 }
 
-/* One lane's run in a node kernel (g: replicate * Nn + node, or -1 for a
- * lane without a node, which still helps its wavefront's SSA). */
-template <class M, bool EVENTS, bool CANON>
-__device__ __forceinline__ void ssm_lane(const KParams &p, long long g)
-{
-    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    const bool live = g >= 0;
-    unsigned long long fired = 0, frozen_cols = 0;
-    bool push = false;
-
-    const int rep = live ? (int)(g / p.Nn) : 0;
-    const int node = live ? (int)(g - (long long)rep * p.Nn) : 0;
-    Ctx<M, CANON> cx;
-    NodeState<M> s;
-    s.a0 = 0.0;
-    int err = 0;
-    double tt = p.tt0, next = p.next0;
-    const EvTab tab = global_tables(p);
-    E1Cursor ec;
-    /* Can the node's state no longer change? */
-    auto frozen = [&]() -> bool {
-        if constexpr (pts_noop<M>::value)
-            return s.a0 <= 0.0 && (!EVENTS || ec.cur >= ec.end);
-        else
-            return false;
-    };
-    /* The lane runs days (not the classify pass, no error at start). */
-    bool going = false;
-
-    if (live) {
-        cx.load(p, node, rep);
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            s.u[c] = p.u0[g * NC + c];
-        });
-        if constexpr (ND > 0) {
-            static_for<ND>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.v[c] = p.v0[g * ND + c];
-                s.vnew[c] = s.v[c];
-            });
-        }
-        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
-        s.nb = ~0ull;
-
-        /* Initial rates (ssm.c:96-117). */
-        static_for<NT>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
-            s.rate[j] = r;
-            s.a0 += r;
-            if (!rate_ok(r))
-                err = SIMINF_ERR_INVALID_RATE;
-        });
-        if constexpr (EVENTS)
-            ec.start(p, node);
-
-        if (err) {
-            record_error(p, rep, 0, PH_INIT, node, err);
-        } else if (p.mode == NODES_CLASSIFY) {
-            if (frozen()) {
-                record_frozen<NC, ND>(p, g, 0, s.u, s.vnew);
-                frozen_cols = (unsigned long long)p.tlen;
-            } else {
-                push = true;
-            }
-        } else {
-            going = true;
-        }
-    }
-
-    /* Day loop.  The wavefront runs it until its last lane is done: the
-     * SSA takes every lane (ssa_day_wave), and lanes without a node or
-     * past their last day only help the others draw their attempts. */
-    if (p.mode != NODES_CLASSIFY) {
-        int U_it = 0, day = 0;
-        double ts = p.tspan[0];
-        for (;;) {
-            if (__ballot(going) == 0)
-                break;
-            bool upd = false;
-            int bad = 0;
-
-            if (going && p.mode == NODES_ACTIVE && frozen()) {
-                store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
-                going = false;
-            }
-
-            err = ssa_day_wave<M, true>(s, cx, tt, next, fired, going);
-            if (!going)
-                continue;
-            if (err) {
-                record_error(p, rep, day, PH_SSA, node, err);
-                going = false;
-                continue;
-            }
-            if constexpr (EVENTS) {
-                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
-                if (err) {
-                    record_error(p, rep, day, PH_E1, p.ev_rank[bad], err);
-                    going = false;
-                    continue;
-                }
-            }
-
-            /* Post time step and refresh (ssm.c:262-302). */
-            const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
-            if (rc < 0) {
-                record_error(p, rep, day, PH_PTS, node, rc);
-                going = false;
-                continue;
-            }
-            if (rc > 0 || upd) {
-                err = refresh_rates<M>(s, cx, tt);
-                if (err) {
-                    record_error(p, rep, day, PH_PTS, node, err);
-                    going = false;
-                    continue;
-                }
-            }
-
-            tt = next;
-            next += 1.0;
-            store_columns<NC, ND>(p, rep, node, tt, U_it, ts, s.u, s.vnew);
-            if constexpr (ND > 0) {
-                static_for<ND>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    const double x = s.v[c];
-                    s.v[c] = s.vnew[c];
-                    s.vnew[c] = x;
-                });
-            }
-            if (U_it >= p.tlen)
-                going = false;
-            day++;
-        }
-    }
-    if (live && p.mode == NODES_CLASSIFY && frozen_cols == 0)
-        p.frozen_from[g] = p.tlen;
-    if (p.mode == NODES_CLASSIFY) {
-        push_active(p, g, push);
-        add_count(p.frozen_cols, frozen_cols);
-    }
-    add_transitions(p, fired);
-}
-
-/* ssm_lane after the classify pass (modes NODES_ALL and NODES_ACTIVE)
- * with free-running lanes: each lane steps through its own days, one SSA
+/* One lane's ssm / mssm run in a node kernel after the classify pass
+ * (modes NODES_ALL and NODES_ACTIVE; g: replicate * Nn + node, or -1 for
+ * a lane without a node), with free-running lanes: each lane steps
+ * through its own days, one SSA
  * attempt or one day's end per trip round the loop, so a wavefront lasts
  * as long as its busiest node's run, where a loop over days lasts the
  * sum over days of each day's busiest lane.  Every attempt consumes one
@@ -497,14 +353,12 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g, int
     add_transitions(p, fired);
 }
 
-/* The classify pass of the node kernels on its own: one lane per
- * (replicate, node) computes the initial rates (an invalid one is the
- * run's PH_INIT error), hands a node that can never change (every rate
- * zero, or a0 <= 0 for ssm; no events; a post-time-step function that
- * does nothing) to the fill kernel and lists the others for the active
- * pass -- what ssm_lane and aem_lane decide in classify mode, in a kernel
- * small enough to fill the device (the solvers' loops hold a wavefront
- * to one or two per SIMD). */
+/* The classify pass of the node kernels: one lane per (replicate, node)
+ * computes the initial rates (an invalid one is the run's PH_INIT error),
+ * hands a node that can never change (every rate zero, or a0 <= 0 for
+ * ssm; no events; a post-time-step function that does nothing) to the
+ * fill kernel and lists the others for the active pass.  A kernel of its
+ * own, small enough to fill the device. */
 template <class M, bool EVENTS, bool CANON, bool AEM>
 __global__ __launch_bounds__(1024) void k_classify(KParams p)
 {
@@ -606,18 +460,8 @@ __global__ __launch_bounds__(1024) void k_classify(KParams p)
     }
 }
 
-/* ssm / mssm over nodes that never interact: each wavefront runs its
- * wave slots (one, its own lanes, except in the active pass). */
-template <class M, bool EVENTS, bool CANON>
-__global__ __launch_bounds__(256) void k_ssm_nodes(KParams p)
-{
-    p.seed = run_seed(p);
-    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
-        ssm_lane<M, EVENTS, CANON>(p, node_lane(p, a));
-}
-
-/* The same after the classify pass with free-running lanes: a kernel of
- * its own, so that its registers are allocated for its own code. */
+/* ssm / mssm over nodes that never interact after the classify pass
+ * (modes NODES_ALL and NODES_ACTIVE): free-running lanes. */
 template <class M, bool EVENTS, bool CANON, bool TSL>
 __global__ __launch_bounds__(256) void k_ssm_nodes_free(KParams p)
 {
@@ -634,174 +478,9 @@ __global__ __launch_bounds__(256) void k_ssm_nodes_free(KParams p)
     }
 }
 
-template <class M, bool EVENTS, bool CANON>
-__device__ __forceinline__ void aem_lane(const KParams &p, long long g)
-{
-    constexpr int NC = M::NC, NT = M::NT, ND = M::ND;
-    unsigned long long fired = 0, frozen_cols = 0;
-    bool push = false;
-    const bool live = g >= 0;
-    const int node = live ? (int)g : 0;
-    Ctx<M, CANON> cx;
-    const uint64_t chan_base = (uint64_t)node * (uint64_t)NT;
-    AemState<M> s;
-    int err = 0;
-    double tt = p.tt0, next = p.next0;
-    const EvTab tab = global_tables(p);
-    E1Cursor ec;
-    /* Can the node's state no longer change (every rate zero)? */
-    auto frozen = [&]() -> bool {
-        if constexpr (pts_noop<M>::value) {
-            bool idle = !EVENTS || ec.cur >= ec.end;
-            static_for<NT>([&](auto J) {
-                idle = idle && s.rate[decltype(J)::value] == 0.0;
-            });
-            return idle;
-        } else {
-            return false;
-        }
-    };
-    /* The lane runs days (not the classify pass, no error at start). */
-    bool going = false;
-
-    if (live) {
-        cx.load(p, node, 0);
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            s.u[c] = p.u0[g * NC + c];
-        });
-        if constexpr (ND > 0) {
-            static_for<ND>([&](auto C) {
-                constexpr int c = decltype(C)::value;
-                s.v[c] = p.v0[g * ND + c];
-                s.vnew[c] = s.v[c];
-            });
-        }
-        s.rng = siminf_stream_make(p.seed, SIMINF_DOMAIN_NODE, (uint64_t)g);
-
-        /* Initial rates and reaction times (aem.c:117-153). */
-        static_for<NT>([&](auto J) {
-            constexpr int j = decltype(J)::value;
-            const double r = M::template rate<j>(s.u, s.v, cx.ld, cx.gd, tt);
-            s.rate[j] = r;
-            if (!rate_ok(r))
-                err = SIMINF_ERR_INVALID_RATE;
-            const siminf_stream c = siminf_stream_make(
-                p.seed, SIMINF_DOMAIN_AEM, chan_base + (uint64_t)j);
-            s.cj[j] = ~0ull;
-            double tj = -siminf_log(aem_uniform<M, j>(s, c, 0)) / r + tt;
-            if (tj <= 0.0)
-                tj = INFINITY;
-            s.time[j] = tj;
-            s.inf[j] = 0.0;
-            s.j[j] = 1;
-        });
-        if constexpr (EVENTS)
-            ec.start(p, node);
-
-        if (err) {
-            record_error(p, 0, 0, PH_INIT, node, err);
-        } else if (p.mode == NODES_CLASSIFY) {
-            if (frozen()) {
-                record_frozen<NC, ND>(p, g, 0, s.u, s.vnew);
-                frozen_cols = (unsigned long long)p.tlen;
-            } else {
-                push = true;
-            }
-        } else {
-            going = true;
-        }
-    }
-
-    /* Day loop.  The wavefront runs it until its last lane is done; the
-     * lanes' channel draws come from the wavefront's table of logarithms
-     * (aem_day_wave), which lanes without a node or past their last day
-     * help to fill. */
-    if (p.mode != NODES_CLASSIFY) {
-        AemTable<M> tb;
-        aem_table_init<M>(tb, going);
-        int U_it = 0, day = 0;
-        double ts = p.tspan[0];
-        for (;;) {
-            if (__ballot(going) == 0)
-                break;
-            bool upd = false;
-            int bad = 0;
-
-            if (going && p.mode == NODES_ACTIVE && frozen()) {
-                store_frozen<NC, ND>(p, g, U_it, s.u, s.vnew);
-                going = false;
-            }
-
-            err = aem_day_wave<M>(s, cx, next, chan_base, fired, going, tb);
-            if (!going)
-                continue;
-            if (err) {
-                record_error(p, 0, day, PH_SSA, node, err);
-                going = false;
-                continue;
-            }
-            if constexpr (EVENTS) {
-                err = e1_day<NC>(p, tab, ec, tt, s.u, &s.rng, upd, bad);
-                if (err) {
-                    record_error(p, 0, day, PH_E1, p.ev_rank[bad], err);
-                    going = false;
-                    continue;
-                }
-            }
-
-            /* Post time step; refresh through calcTimes at the node's
-             * time, which is the next unit of time (aem.c:329-384). */
-            const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
-            if (rc < 0) {
-                record_error(p, 0, day, PH_PTS, node, rc);
-                going = false;
-                continue;
-            }
-            if (rc > 0 || upd) {
-                static_for<NT>([&](auto J) {
-                    constexpr int j = decltype(J)::value;
-                    const double old = s.rate[j];
-                    const double r = M::template rate<j>(s.u, s.vnew, cx.ld, cx.gd, tt);
-                    s.rate[j] = r;
-                    if (!rate_ok(r))
-                        err = SIMINF_ERR_INVALID_RATE;
-                    calc_times<M, j>(s, cx.seed, chan_base, next, old, r);
-                });
-                if (err) {
-                    record_error(p, 0, day, PH_PTS, node, err);
-                    going = false;
-                    continue;
-                }
-            }
-
-            tt = next;
-            next += 1.0;
-            store_columns<NC, ND>(p, 0, node, tt, U_it, ts, s.u, s.vnew);
-            if constexpr (ND > 0) {
-                static_for<ND>([&](auto C) {
-                    constexpr int c = decltype(C)::value;
-                    const double x = s.v[c];
-                    s.v[c] = s.vnew[c];
-                    s.vnew[c] = x;
-                });
-            }
-            if (U_it >= p.tlen)
-                going = false;
-            day++;
-        }
-    }
-    if (live && p.mode == NODES_CLASSIFY && frozen_cols == 0)
-        p.frozen_from[g] = p.tlen;
-    if (p.mode == NODES_CLASSIFY) {
-        push_active(p, g, push);
-        add_count(p.frozen_cols, frozen_cols);
-    }
-    add_transitions(p, fired);
-}
-
-/* aem_lane after the classify pass with free-running lanes, as
- * ssm_lane_free: one firing or one day's end per trip round the loop.
+/* One lane's aem run in a node kernel after the classify pass, with
+ * free-running lanes as ssm_lane_free: one firing or one day's end per
+ * trip round the loop (aem.c:181-300, the day's end :329-419).
  * Each channel keeps the logarithms of the uniforms at its next two
  * stream positions (l0, l1): a firing that draws from a channel takes l0
  * and shifts l1 down, and the next trip computes the channel's new l1
@@ -1455,15 +1134,8 @@ __device__ __forceinline__ void aem_lane_quad(const KParams &p, long long g, int
     add_transitions(p, fired);
 }
 
-/* aem over nodes that never interact, by wave slots as k_ssm_nodes. */
-template <class M, bool EVENTS, bool CANON>
-__global__ __launch_bounds__(256) void k_aem_nodes(KParams p)
-{
-    p.seed = run_seed(p);
-    for (WaveSlots a = wave_slots(p); a.w < a.waves; a.w += a.stride)
-        aem_lane<M, EVENTS, CANON>(p, node_lane(p, a));
-}
-
+/* aem over nodes that never interact after the classify pass, by wave
+ * slots as k_ssm_nodes_free. */
 template <class M, bool EVENTS, bool CANON, bool TSL>
 __global__ __launch_bounds__(256) void k_aem_nodes_free(KParams p)
 {
@@ -2201,7 +1873,7 @@ void launch_ssm_nodes(const KParams &p, bool events, bool canon,
             hipLaunchKernelGGL((k_classify<M, false, false, false>), cg, cb, 0, st, p);
         return;
     }
-    if (p.mode != NODES_CLASSIFY && p.free_lanes) {
+    {
         /* tspan in LDS up to SIMINF_TS_LDS_MAX points. */
         const bool tsl = p.tlen <= SIMINF_TS_LDS_MAX;
         const size_t lds = tsl ? (size_t)p.tlen * sizeof(double) : 0;
@@ -2223,16 +1895,7 @@ void launch_ssm_nodes(const KParams &p, bool events, bool canon,
         else
             SIMINF_FREE(false, false);
 #undef SIMINF_FREE
-        return;
     }
-    if (events && canon)
-        hipLaunchKernelGGL((k_ssm_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
-    else if (events)
-        hipLaunchKernelGGL((k_ssm_nodes<M, true, false>), dim3(grid), dim3(block), 0, st, p);
-    else if (canon)
-        hipLaunchKernelGGL((k_ssm_nodes<M, false, true>), dim3(grid), dim3(block), 0, st, p);
-    else
-        hipLaunchKernelGGL((k_ssm_nodes<M, false, false>), dim3(grid), dim3(block), 0, st, p);
 }
 
 template <class M>
@@ -2253,7 +1916,7 @@ void launch_aem_nodes(const KParams &p, bool events, bool canon,
             hipLaunchKernelGGL((k_classify<M, false, false, true>), cg, cb, 0, st, p);
         return;
     }
-    if (p.mode != NODES_CLASSIFY && p.free_lanes) {
+    {
         /* tspan in LDS up to SIMINF_TS_LDS_MAX points. */
         const bool tsl = p.tlen <= SIMINF_TS_LDS_MAX;
         const size_t lds = tsl ? (size_t)p.tlen * sizeof(double) : 0;
@@ -2275,16 +1938,7 @@ void launch_aem_nodes(const KParams &p, bool events, bool canon,
         else
             SIMINF_FREE(false, false);
 #undef SIMINF_FREE
-        return;
     }
-    if (events && canon)
-        hipLaunchKernelGGL((k_aem_nodes<M, true, true>), dim3(grid), dim3(block), 0, st, p);
-    else if (events)
-        hipLaunchKernelGGL((k_aem_nodes<M, true, false>), dim3(grid), dim3(block), 0, st, p);
-    else if (canon)
-        hipLaunchKernelGGL((k_aem_nodes<M, false, true>), dim3(grid), dim3(block), 0, st, p);
-    else
-        hipLaunchKernelGGL((k_aem_nodes<M, false, false>), dim3(grid), dim3(block), 0, st, p);
 }
 
 /* Clears a grid launch's barrier words (a kernel, not a memset, so that
