@@ -501,7 +501,8 @@ def bench_abc_stream(args, generations=8, proposals=10_000, n_particles=1000, ch
     from siminf_amd import abc_stream as S
     problems = S.config_problems()
     res = S.run_stream(problems, generations=generations, proposals=proposals,
-                       n_particles=n_particles, chunk=chunk, seed=7)
+                       n_particles=n_particles, chunk=chunk, seed=7,
+                       fuse=not args.no_fuse)
     lat = np.asarray(res.latencies)
     return {"workload": "ABC-SMC stream: SIR 1000 nodes x 150 days, SEIR 20000 nodes x 365 "
                         "days, SISe3 500 herds x 365 days with events; %d generations x %d "
@@ -515,6 +516,7 @@ def bench_abc_stream(args, generations=8, proposals=10_000, n_particles=1000, ch
             "launch_ms_mean": float(np.mean(res.launch_s) * 1e3),
             "split_s": res.split_s,
             "shape": res.shape,
+            "fused_prevalence": dict(zip([pr.name for pr in problems], res.fused)),
             "tolerances": {pr.name: [g[pr.name][0] for g in res.generations]
                            for pr in problems},
             "posterior_means": {pr.name: dict(zip([q.parameter for q in pr.priors],
@@ -540,6 +542,8 @@ def main():
                          "measurements")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default lanes-per-wavefront rule of the node kernels")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="abc_stream: plans keep U and prevalence reads it back")
     ap.add_argument("--requests", type=int, default=24,
                     help="pfilter_stream: requests in the timed stream")
     ap.add_argument("--particles", type=int, default=2000)

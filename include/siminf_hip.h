@@ -387,6 +387,20 @@ int SimInf_hip_plan_prevalence(SimInf_hip_plan *plan, unsigned int cases_mask,
                                const double *cond_args, int cond_len, int level,
                                const int *index, int n_index, double *out);
 
+/* Fused level-1 prevalence (no counterpart in the reference; for callers
+ * that need only prevalence(model, cases ~ population, level = 1) of every
+ * run, such as ABC distances, R/abc.R:585-611): later runs of the plan
+ * keep, instead of U, exact integer sums of cases_mask and pop_mask per
+ * replicate and U column, and SimInf_hip_plan_prevalence with level 1,
+ * the same masks, no condition and index NULL returns
+ * sum(cases) / sum(population) from them -- the values the unfused plan
+ * gives.  Frees the plan's dense U and V: download, trajectories and other
+ * summaries then return SIMINF_ERR_INVALID_SUMMARY.  Calling it again sets
+ * other masks.  Returns 0, 1 when the plan cannot fuse (not a node-kernel
+ * plan, or U not dense: the plan is unchanged) or an error code. */
+int SimInf_hip_plan_fuse_prevalence(SimInf_hip_plan *plan, unsigned int cases_mask,
+                                    unsigned int pop_mask);
+
 /* Trajectory of the compartments in comp_mask (in model order) for the
  * nodes in index (zero-based, or NULL for all) from the resident U, in
  * the layout of trajectory(..., format = "matrix") (trajectory_as_is,

@@ -94,6 +94,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_batch_destroy",
     "SimInf_hip_pfilter",
     "SimInf_hip_plan_prevalence",
+    "SimInf_hip_plan_fuse_prevalence",
     "SimInf_hip_plan_trajectory",
     "SimInf_hip_plan_trajectory_v",
     "SimInf_hip_abc_weights",
@@ -157,6 +158,8 @@ def load(path=None):
     lib.SimInf_hip_plan_prevalence.argtypes = [
         vp, ctypes.c_uint, ctypes.c_uint, ip, dp, ctypes.c_int, ctypes.c_int, ip,
         ctypes.c_int, dp]
+    lib.SimInf_hip_plan_fuse_prevalence.restype = ctypes.c_int
+    lib.SimInf_hip_plan_fuse_prevalence.argtypes = [vp, ctypes.c_uint, ctypes.c_uint]
     lib.SimInf_hip_plan_trajectory.restype = ctypes.c_int
     lib.SimInf_hip_plan_trajectory.argtypes = [vp, ctypes.c_uint, ip, ctypes.c_int, ip]
     lib.SimInf_hip_plan_trajectory_v.restype = ctypes.c_int

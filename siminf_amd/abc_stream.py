@@ -12,6 +12,9 @@ proposals runs in one batched launch (one Batch holding the problems'
 plans), and each replicate's daily prevalence comes from the plan's
 resident trajectory on the device (prevalence level 1 per column); the
 distance is the root mean square difference to the observed series.
+Plans whose solver runs node kernels keep only the sums that prevalence
+needs instead of U (Plan.fuse_prevalence), so the trajectory is neither
+written nor read back.
 
 The tolerance of a generation is its n_particles-th smallest distance, so
 that n_particles proposals are accepted from every generation (a
@@ -95,6 +98,7 @@ class StreamResult:
         self.node_days = 0
         self.wall_s = 0.0
         self.shape = None
+        self.fused = []              # per problem: whether its plan fused
         # Seconds spent setting parameters, in the batched launches and in
         # the summaries (prevalence and distances).
         self.split_s = {"parameters": 0.0, "launch": 0.0, "summaries": 0.0}
@@ -109,10 +113,12 @@ def observe(problem, seed, lib=None):
 
 
 def run_stream(problems, generations=8, proposals=10_000, n_particles=1000, chunk=1000,
-               seed=1, lib=None, autotune=True):
+               seed=1, lib=None, autotune=True, fuse=True):
     """ABC-SMC of every problem, generation by generation; a generation's
     proposals of all problems run chunk by chunk, each chunk of all
-    problems in one batched launch.  Returns a StreamResult."""
+    problems in one batched launch.  fuse: node-kernel plans keep only
+    their prevalence sums (Plan.fuse_prevalence).  Returns a
+    StreamResult."""
     rng = np.random.default_rng(seed)
     for pr in problems:
         if pr.observed is None:
@@ -120,6 +126,10 @@ def run_stream(problems, generations=8, proposals=10_000, n_particles=1000, chun
     plans = [Plan(pr.replicated(chunk), solver=pr.solver, lib=lib) for pr in problems]
     res = StreamResult()
     try:
+        # Plans that can keep the prevalence sums instead of U do (the
+        # distances need nothing else).
+        res.fused = [pl.fuse_prevalence(pr.formula) if fuse else False
+                     for pr, pl in zip(problems, plans)]
         # Parameters per replicate before the batch is captured, so that
         # the graph reads them where they stay.
         for pr, pl in zip(problems, plans):

@@ -218,6 +218,16 @@ struct KParams {
     /* Results. */
     unsigned long long *err_key;
     unsigned long long *transitions;
+
+    /* Fused prevalence (node plans; null when off): instead of writing
+     * U, every column's sums of the cases and population compartments
+     * (fp_cases, fp_pop masks) go to fp_sum[2 (rep tlen + it) + 0/1], and a
+     * node whose state stops changing at column it adds its counts once to
+     * the difference array fp_delta[2 (rep (tlen + 1) + it) + 0/1], summed
+     * over the columns when the prevalence is read. */
+    long long *fp_sum;
+    long long *fp_delta;
+    unsigned fp_cases, fp_pop;
 };
 
 __device__ __forceinline__ void record_error(const KParams &p, int rep, int day,
@@ -1682,6 +1692,46 @@ __device__ __forceinline__ void store_columns(
     }
 }
 
+/* A node's (cases, population) counts for the fused prevalence. */
+template <int NC>
+__device__ __forceinline__ void fused_counts(const KParams &p, const int (&u)[NC],
+                                             long long &x, long long &y)
+{
+    x = 0;
+    y = 0;
+    static_for<NC>([&](auto C) {
+        constexpr int c = decltype(C)::value;
+        x += ((p.fp_cases >> c) & 1u) ? (long long)u[c] : 0LL;
+        y += ((p.fp_pop >> c) & 1u) ? (long long)u[c] : 0LL;
+    });
+}
+
+__device__ __forceinline__ void fused_add(long long *dst, long long x, long long y)
+{
+    atomicAdd((unsigned long long *)dst, (unsigned long long)x);
+    atomicAdd((unsigned long long *)dst + 1, (unsigned long long)y);
+}
+
+/* Column it of replicate rep in the fused sums. */
+template <int NC>
+__device__ __forceinline__ void fused_column(const KParams &p, int rep, int it,
+                                             const int (&u)[NC])
+{
+    long long x, y;
+    fused_counts<NC>(p, u, x, y);
+    fused_add(p.fp_sum + 2 * ((long long)rep * p.tlen + it), x, y);
+}
+
+/* Columns it .. tlen - 1 of replicate rep, all equal to u. */
+template <int NC>
+__device__ __forceinline__ void fused_from(const KParams &p, int rep, int it,
+                                           const int (&u)[NC])
+{
+    long long x, y;
+    fused_counts<NC>(p, u, x, y);
+    fused_add(p.fp_delta + 2 * ((long long)rep * (p.tlen + 1) + it), x, y);
+}
+
 /* Largest tspan the free-running node kernels copy into LDS (64 KiB). */
 #define SIMINF_TS_LDS_MAX 8192
 
@@ -1726,6 +1776,8 @@ __device__ __forceinline__ void store_columns_ahead(
         const long long col = (long long)rep * p.tlen + it;
         if (!writer) {
             /* another lane of the node writes */
+        } else if (p.fp_sum) {
+            fused_column<NC>(p, rep, it, u);
         } else if (p.U) {
             int *dst = p.U + (col * p.Nn + node) * NC;
             static_for<NC>([&](auto C) {
@@ -1736,7 +1788,7 @@ __device__ __forceinline__ void store_columns_ahead(
             store_sparse<NC>(p.irU, p.jcU, p.prU, col, node, u);
         }
         if constexpr (ND > 0) {
-            if (!writer) {
+            if (!writer || p.fp_sum) {
             } else if (p.V) {
                 double *dst = p.V + (col * p.Nn + node) * ND;
                 static_for<ND>([&](auto C) {
@@ -1769,11 +1821,15 @@ __device__ __forceinline__ bool day_end_fast(const KParams &p, int rep, int node
     const bool col = tn > ts;
     const bool fast = dend && alive && !(tn > ts1) && it + 1 < p.tlen;
     if (fast && col && writer) {
-        int *dst = p.U + (((long long)rep * p.tlen + it) * p.Nn + node) * NC;
-        static_for<NC>([&](auto C) {
-            constexpr int c = decltype(C)::value;
-            dst[c] = u[c];
-        });
+        if (p.fp_sum) {
+            fused_column<NC>(p, rep, it, u);
+        } else {
+            int *dst = p.U + (((long long)rep * p.tlen + it) * p.Nn + node) * NC;
+            static_for<NC>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                dst[c] = u[c];
+            });
+        }
     }
     const bool adv = fast && col;
     it += adv ? 1 : 0;
@@ -1899,6 +1955,11 @@ __device__ __forceinline__ void store_frozen(const KParams &p, long long g, int 
                                              const int (&u)[NC],
                                              const double (&vnew)[ND > 0 ? ND : 1])
 {
+    if (p.fp_sum) {
+        if (U_it < p.tlen)
+            fused_from<NC>(p, (int)(g / p.Nn), U_it, u);
+        return;
+    }
     const int rep = (int)(g / p.Nn);
     const int node = (int)(g - (long long)rep * p.Nn);
     for (int it = U_it; it < p.tlen; it++) {

@@ -165,6 +165,18 @@ struct DeviceBuffers {
         return d;
     }
 
+    /* Frees one buffer of these. */
+    void free_one(void *p)
+    {
+        for (size_t k = 0; k < ptrs.size(); k++) {
+            if (ptrs[k] == p) {
+                (void)(pooled ? hipFreeAsync(p, stream) : hipFree(p));
+                ptrs.erase(ptrs.begin() + (long)k);
+                return;
+            }
+        }
+    }
+
     /* Frees everything; the stream must still exist. */
     void release()
     {
@@ -210,6 +222,10 @@ struct SimInf_hip_plan {
     /* Host copies needed after runs. */
     int Nn, Nc, Nd, Nrep, tlen;
     bool dense_U, dense_V;
+    /* Fused prevalence (SimInf_hip_plan_fuse_prevalence): the sums and
+     * difference array, and whether V was dense before U and V were freed. */
+    long long *fp_sum = nullptr, *fp_delta = nullptr;
+    bool fp_had_V = false;
     size_t nnzU, nnzV;
     int ndays;
     uint64_t host_err_key;     /* errors found while validating events */
@@ -910,6 +926,35 @@ static hipError_t launch_grid_reps(SimInf_hip_plan *P, KParams q, hipStream_t st
 /* Clears a run's result words and counters on the device (a kernel, not
  * memcpys or memsets, so that a run's launch sequence is kernels only and
  * can be captured in a graph). */
+/* Zero of a fused-prevalence array (a kernel, so that runs can sit in a
+ * graph). */
+__global__ void k_clear_i64(long long *a, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+/* Level-1 prevalence from the fused sums: per replicate the frozen
+ * nodes' counts are summed over the columns of the difference array and
+ * added to each column's sums, then cases / population as
+ * k_prevalence_cols divides. */
+__global__ void k_fused_finalize(const long long *sum, const long long *delta, int nrep,
+                                 int tlen, double *out)
+{
+    const int rep = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (rep >= nrep)
+        return;
+    long long cx = 0, cy = 0;
+    for (int it = 0; it < tlen; it++) {
+        const long long *d = delta + 2 * ((long long)rep * (tlen + 1) + it);
+        const long long *a = sum + 2 * ((long long)rep * tlen + it);
+        cx += d[0];
+        cy += d[1];
+        out[(long long)rep * tlen + it] = (double)(a[0] + cx) / (double)(a[1] + cy);
+    }
+}
+
 __global__ void k_run_init(unsigned long long *err_key, unsigned long long *transitions,
                            unsigned long long *frozen_cols, int *active_count,
                            unsigned *gsync, unsigned long long *rep_err, int nrep)
@@ -956,12 +1001,22 @@ static int plan_enqueue_phase(SimInf_hip_plan *P, const KParams &base, bool capt
         else
             P->model->aem_nodes(q, P->has_e1, P->canon, grid, P->block, st);
     };
+    const bool classify = nodes && p.frozen_from;
     if (phase == 0) {
-        *filled = nodes && p.frozen_from;
+        /* A fused-prevalence run has the classify and active passes but no
+         * fill: frozen nodes go into the sums. */
+        *filled = classify && !p.fp_sum;
         mark(P->ev_start, st);
         hipLaunchKernelGGL(k_run_init, dim3(1), dim3(256), 0, st, p.err_key, p.transitions,
                            p.frozen_cols, p.active_count, p.gsync, p.rep_err, P->Nrep);
-        if (*filled) {
+        if (p.fp_sum) {
+            const long long n1 = 2LL * P->Nrep * P->tlen, n2 = 2LL * P->Nrep * (P->tlen + 1);
+            hipLaunchKernelGGL(k_clear_i64, dim3((unsigned)std::min<long long>((n1 + 255) / 256, 4096)),
+                               dim3(256), 0, st, p.fp_sum, n1);
+            hipLaunchKernelGGL(k_clear_i64, dim3((unsigned)std::min<long long>((n2 + 255) / 256, 4096)),
+                               dim3(256), 0, st, p.fp_delta, n2);
+        }
+        if (classify) {
             /* Classify every lane; nodes frozen from the start go to the
              * fill kernel on the second stream while the active pass runs
              * the others, packed so that they spread over the SIMDs (the
@@ -969,7 +1024,7 @@ static int plan_enqueue_phase(SimInf_hip_plan *P, const KParams &base, bool capt
             p.mode = NODES_CLASSIFY;
             launch(p, P->grid);
             mark(P->ev_classified, st);
-            if (hipEventRecord(P->dep_fork, st) != hipSuccess)
+            if (*filled && hipEventRecord(P->dep_fork, st) != hipSuccess)
                 return SIMINF_ERR_DEVICE;
         }
     } else if (phase == 1) {
@@ -1002,7 +1057,7 @@ static int plan_enqueue_phase(SimInf_hip_plan *P, const KParams &base, bool capt
             break;
         }
     } else {
-        if (nodes && p.frozen_from) {
+        if (classify && !p.fp_sum) {
             hipStream_t fs = P->fill_stream;
             if (hipStreamWaitEvent(fs, P->dep_fork, 0) != hipSuccess ||
                 (after_fill && hipStreamWaitEvent(fs, after_fill, 0) != hipSuccess))
@@ -1070,11 +1125,14 @@ static int plan_collect(SimInf_hip_plan *P, bool filled)
          * stream. */
         float ms = 0.0f, ms_solver = 0.0f, ms_active = 0.0f, ms_fill = 0.0f;
         (void)hipEventElapsedTime(&ms, P->ev_start, P->ev_stop);
-        if (filled) {
+        const bool classified =
+            (P->kernel == K_SSM_NODES || P->kernel == K_AEM_NODES) && p.frozen_from;
+        if (classified) {
             (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_classified);
             (void)hipEventElapsedTime(&ms_active, P->ev_mid, P->ev_active1);
             ms_solver += ms_active;
-            (void)hipEventElapsedTime(&ms_fill, P->ev_fill0, P->ev_fill1);
+            if (filled)
+                (void)hipEventElapsedTime(&ms_fill, P->ev_fill0, P->ev_fill1);
         } else {
             (void)hipEventElapsedTime(&ms_solver, P->ev_start, P->ev_mid);
         }
@@ -1109,6 +1167,8 @@ extern "C" int SimInf_hip_plan_download(SimInf_hip_plan *P, int *U, double *V,
 {
     if (!P)
         return SIMINF_ERR_INVALID_MODEL;
+    if (P->fp_sum && (U || V || prU || prV))
+        return SIMINF_ERR_INVALID_SUMMARY;   /* a fused plan keeps no trajectory */
     const size_t ncols = (size_t)P->Nrep * P->tlen;
     hipError_t e = hipSuccess;
     hipStream_t st = P->stream;
@@ -2447,11 +2507,74 @@ static unsigned compartment_bits(int Nc)
     return Nc >= 32 ? 0xffffffffu : ((1u << Nc) - 1u);
 }
 
+extern "C" int SimInf_hip_plan_fuse_prevalence(SimInf_hip_plan *P, unsigned int cases_mask,
+                                               unsigned int pop_mask)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    const unsigned all = compartment_bits(P->Nc);
+    if (!cases_mask || !pop_mask || (cases_mask & ~all) || (pop_mask & ~all))
+        return SIMINF_ERR_INVALID_SUMMARY;
+    if (!P->fp_sum) {
+        if ((P->kernel != K_SSM_NODES && P->kernel != K_AEM_NODES) || !P->dense_U)
+            return 1;   /* not fusable: the plan keeps its trajectory */
+        const size_t n1 = 2 * (size_t)P->Nrep * P->tlen;
+        const size_t n2 = 2 * (size_t)P->Nrep * (P->tlen + 1);
+        long long *sum = P->buf.alloc<long long>(n1);
+        long long *delta = P->buf.alloc<long long>(n2);
+        if (P->buf.status != hipSuccess || hipStreamSynchronize(P->stream) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        P->buf.free_one(P->p.U);
+        P->p.U = nullptr;
+        P->fp_had_V = P->dense_V;
+        if (P->p.V)
+            P->buf.free_one(P->p.V);
+        P->p.V = nullptr;
+        P->dense_U = P->dense_V = false;
+        if (hipStreamSynchronize(P->stream) != hipSuccess)
+            return SIMINF_ERR_DEVICE;
+        P->fp_sum = P->p.fp_sum = sum;
+        P->fp_delta = P->p.fp_delta = delta;
+    }
+    P->p.fp_cases = cases_mask;
+    P->p.fp_pop = pop_mask;
+    P->has_run = false;
+    P->param_version++;   /* a batch holding the plan captures again */
+    return 0;
+}
+
+/* Level-1 prevalence of a fused plan: the formula it was fused with, every
+ * node, no condition. */
+static int fused_prevalence(SimInf_hip_plan *P, unsigned cases_mask, unsigned pop_mask,
+                            int cond_len, int level, const int *index, double *out)
+{
+    if (!P->has_run)
+        return SIMINF_ERR_EMPTY_TRAJECTORY;
+    if (!out || level != 1 || cond_len != 0 || index || cases_mask != P->p.fp_cases ||
+        pop_mask != P->p.fp_pop)
+        return SIMINF_ERR_INVALID_SUMMARY;
+    const size_t nout = (size_t)P->Nrep * P->tlen;
+    DeviceBuffers T;
+    T.stream = P->stream;
+    T.pooled = P->buf.pooled;
+    double *out_d = T.alloc<double>(nout);
+    if (T.status != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    hipLaunchKernelGGL(k_fused_finalize, dim3((unsigned)((P->Nrep + 63) / 64)), dim3(64), 0,
+                       P->stream, P->fp_sum, P->fp_delta, P->Nrep, P->tlen, out_d);
+    if (hipGetLastError() != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    T.copy(out, out_d, nout * sizeof(double), hipMemcpyDeviceToHost);
+    return T.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
+}
+
 extern "C" int SimInf_hip_plan_prevalence(SimInf_hip_plan *P, unsigned int cases_mask,
                                           unsigned int pop_mask, const int *cond_ops,
                                           const double *cond_args, int cond_len, int level,
                                           const int *index, int n_index, double *out)
 {
+    if (P && P->fp_sum)
+        return fused_prevalence(P, cases_mask, pop_mask, cond_len, level, index, out);
     int err = summary_checks(P, index, n_index);
     if (err)
         return err;

@@ -288,7 +288,7 @@ __device__ __forceinline__ void ssm_lane_free(const KParams &p, long long g, int
              * store, beside the next trip's work; the rest (a node that
              * freezes or finishes, several columns in one day, sparse
              * output) takes the general code below. */
-            if (p.U)
+            if (p.U || p.fp_sum)
                 slow = dend && !day_end_fast<NC, TSL>(p, rep, node, dend, s.a0 > 0.0, s.u,
                                                        tt, next, U_it, ts, ts1, ts_sh);
             const bool ended = dend && !slow;
@@ -378,6 +378,7 @@ __global__ __launch_bounds__(1024) void k_classify(KParams p)
         const long long g = g0 + threadIdx.x;
         const bool live = g < p.lanes;
         bool push = false;
+        long long fx = 0, fy = 0;   /* a frozen node's fused counts */
         if (live) {
             const int rep = (int)(g / p.Nn);
             const int node = (int)(g - (long long)rep * p.Nn);
@@ -418,7 +419,10 @@ __global__ __launch_bounds__(1024) void k_classify(KParams p)
             if (err) {
                 record_error(p, rep, 0, PH_INIT, node, err);
             } else if (pts_noop<M>::value && idle) {
-                record_frozen<NC, ND>(p, g, 0, u, v);
+                if (p.fp_sum)
+                    fused_counts<NC>(p, u, fx, fy);
+                else
+                    record_frozen<NC, ND>(p, g, 0, u, v);
                 frozen_cols += (unsigned long long)p.tlen;
                 frz = true;
             } else {
@@ -426,6 +430,34 @@ __global__ __launch_bounds__(1024) void k_classify(KParams p)
             }
             if (!frz)
                 p.frozen_from[g] = p.tlen;
+        }
+        if (p.fp_sum) {
+            /* Frozen nodes' counts into column 0 of their replicate's
+             * difference array: summed over the wavefront's lanes of the
+             * first lane's replicate and, separately, of the others (a
+             * wavefront of consecutive lanes spans two replicates at most
+             * when Nn >= 64; with smaller Nn each lane adds its own). */
+            const long long gl = live ? g : p.lanes - 1;
+            const int rep = (int)(gl / p.Nn);
+            const int rep0 = __shfl(rep, 0);
+            if (p.Nn >= 64) {
+                const bool first = rep == rep0;
+                long long a0 = first ? fx : 0, b0 = first ? fy : 0;
+                long long a1 = first ? 0 : fx, b1 = first ? 0 : fy;
+                for (int off = 32; off > 0; off >>= 1) {
+                    a0 += __shfl_xor(a0, off);
+                    b0 += __shfl_xor(b0, off);
+                    a1 += __shfl_xor(a1, off);
+                    b1 += __shfl_xor(b1, off);
+                }
+                const unsigned long long other = __ballot(!first);
+                if (lane == 0 && (a0 || b0))
+                    fused_add(p.fp_delta + 2 * ((long long)rep0 * (p.tlen + 1)), a0, b0);
+                if (other && lane == __ffsll((long long)other) - 1 && (a1 || b1))
+                    fused_add(p.fp_delta + 2 * ((long long)rep * (p.tlen + 1)), a1, b1);
+            } else if (fx || fy) {
+                fused_add(p.fp_delta + 2 * ((long long)rep * (p.tlen + 1)), fx, fy);
+            }
         }
         const unsigned long long m = __ballot(push);
         if (lane == 0)
@@ -764,7 +796,7 @@ __device__ __forceinline__ void aem_lane_free(const KParams &p, long long g, int
         bool slow = dend;
         if constexpr (!EVENTS && pts_noop<M>::value && ND == 0) {
             /* The fast day end of ssm_lane_free. */
-            if (p.U) {
+            if (p.U || p.fp_sum) {
                 bool live_rates = false;
                 static_for<NT>([&](auto J) {
                     live_rates = live_rates || s.rate[decltype(J)::value] != 0.0;
@@ -1061,7 +1093,7 @@ __device__ __forceinline__ void aem_lane_quad(const KParams &p, long long g, int
         const bool dend = going && !fires;
         bool slow = dend;
         if constexpr (!EVENTS && pts_noop<M>::value && ND == 0) {
-            if (p.U) {
+            if (p.U || p.fp_sum) {
                 const bool alive = quad_any(chan && rate != 0.0);
                 slow = dend && !day_end_fast<NC, TSL>(p, 0, node, dend, alive, u, tt, next,
                                                        U_it, ts, ts1, ts_sh, lead);

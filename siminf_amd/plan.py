@@ -137,6 +137,20 @@ class Plan:
     def _columns(self):
         return self.model.replicates * len(self.model.tspan)
 
+    def fuse_prevalence(self, formula):
+        """Keep, from the next run on, only what prevalence(formula,
+        level=1) needs (exact sums per replicate and column) instead of U
+        and V; returns False (and changes nothing) for plans that cannot
+        fuse.  A fused plan's download and trajectories raise."""
+        cases, pop, ops, _, _ = summaries.prevalence_spec(formula, self.model.compartments, 1)
+        if len(ops):
+            _raise(self.lib, -105)
+        err = self.lib.SimInf_hip_plan_fuse_prevalence(self._handle, cases, pop)
+        if err == 1:
+            return False
+        self._check(err)
+        return True
+
     def prevalence(self, formula, level=1, index=None):
         """prevalence(model, formula, level, index, format = "matrix") of the
         last run, computed on the device.  index is one-based like the
