@@ -151,7 +151,9 @@ typedef struct SimInf_solver_args {
 typedef struct SimInf_hip_model SimInf_hip_model;
 
 /* Built-in device plugins, by the reference's model names: "SIR",
- * "SEIR", "SIS", "SISe", "SISe3" (src/models/SIR.c etc.) and "SIR_mparse" (the
+ * "SEIR", "SIS", "SISe", "SISe3", "SISe_sp", "SISe3_sp" (src/models/SIR.c
+ * etc.; the spatial models' post-time-step functions read the
+ * neighbours listed in ldata, SISe_sp.c:90-138) and "SIR_mparse" (the
  * code mparse emits for S -> beta*S*I/N -> I, I -> gamma*I -> R,
  * N <- S+I+R).  NULL if the name is unknown. */
 const SimInf_hip_model *SimInf_hip_model_builtin(const char *name);

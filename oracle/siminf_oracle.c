@@ -1209,6 +1209,12 @@ int oracle_set_model(SimInf_solver_args *a, const char *name)
     } else if (strcmp(name, "SISe3") == 0) {
         a->tr_fun = SISe3_tr;
         a->pts_fun = SISe3_pts;
+    } else if (strcmp(name, "SISe_sp") == 0) {
+        a->tr_fun = SISe_tr;
+        a->pts_fun = SISe_sp_pts;
+    } else if (strcmp(name, "SISe3_sp") == 0) {
+        a->tr_fun = SISe3_tr;
+        a->pts_fun = SISe3_sp_pts;
     } else if (strcmp(name, "SIR_mparse") == 0) {
         a->tr_fun = SIR_mparse_tr;
         a->pts_fun = siminf_pts_noop;

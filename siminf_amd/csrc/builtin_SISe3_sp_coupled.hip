@@ -1,0 +1,7 @@
+/*
+ * builtin_SISe3_sp_coupled.hip -- the coupled kernels instantiated for
+ * the built-in spatial model SISe3_sp.
+ */
+#include "siminf_builtin_models.h"
+
+SIMINF_COUPLED_INSTANCE(, siminf::ModelSISe3_sp);

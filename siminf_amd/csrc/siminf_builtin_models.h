@@ -1,7 +1,8 @@
 /*
  * siminf_builtin_models.h -- the reference's built-in models as
- * compile-time plugins of the solver kernels: SIR, SEIR, SIS, SISe, SISe3
- * (the files under src/models) and the code mparse emits for the README's SIR model
+ * compile-time plugins of the solver kernels: SIR, SEIR, SIS, SISe, SISe3,
+ * SISe_sp, SISe3_sp (the files under src/models) and the code mparse
+ * emits for the README's SIR model
  * (S -> beta*S*I/N -> I, I -> gamma*I -> R, N <- S+I+R).  Each model's
  * kernels are instantiated in a translation unit of its own
  * (builtin_*.hip), which exports the model's SimInf_hip_model record.
@@ -104,6 +105,54 @@ struct ModelSISe3 {
     SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
     {
         return SISe3_pts(vn, u, v, ld, gd, node, t);
+    }
+};
+
+/* The spatial models: SISe's and SISe3's rates, a post-time-step function
+ * that reads the neighbours listed in ldata after end_t1..end_t4 (from
+ * row NEIGHBOR_ROW).  NLD covers the four interval ends the rates'
+ * context holds; the function receives the node's full ldata column. */
+struct ModelSISe_sp {
+    static constexpr int NC = 2, NT = 2, ND = 1, NLD = 4, NGD = 8;
+    static constexpr bool NEIGHBORS = true;
+    static constexpr int NEIGHBOR_ROW = 4;
+    static constexpr int S[NT][NC] =
+        {{-1, 1}, {1, -1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1}, {1, 1}};
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
+    {
+        if constexpr (J == 0) return SISe_S_to_I(u, v, ld, gd, t);
+        else return SISe_I_to_S(u, v, ld, gd, t);
+    }
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
+    {
+        return SISe_sp_pts(vn, u, v, ld, gd, node, t);
+    }
+};
+
+struct ModelSISe3_sp {
+    static constexpr int NC = 6, NT = 6, ND = 1, NLD = 4, NGD = 12;
+    static constexpr bool NEIGHBORS = true;
+    static constexpr int NEIGHBOR_ROW = 4;
+    static constexpr int S[NT][NC] =
+        {{-1, 1, 0, 0, 0, 0}, {1, -1, 0, 0, 0, 0}, {0, 0, -1, 1, 0, 0},
+         {0, 0, 1, -1, 0, 0}, {0, 0, 0, 0, -1, 1}, {0, 0, 0, 0, 1, -1}};
+    static constexpr int G[NT][NT] =
+        {{1, 1, 0, 0, 0, 0}, {1, 1, 0, 0, 0, 0}, {0, 0, 1, 1, 0, 0},
+         {0, 0, 1, 1, 0, 0}, {0, 0, 0, 0, 1, 1}, {0, 0, 0, 0, 1, 1}};
+    template <int J> SIMINF_MODEL_FN double rate(SIMINF_RATE_ARGS)
+    {
+        if constexpr (J == 0) return SISe3_S_1_to_I_1(u, v, ld, gd, t);
+        else if constexpr (J == 1) return SISe3_I_1_to_S_1(u, v, ld, gd, t);
+        else if constexpr (J == 2) return SISe3_S_2_to_I_2(u, v, ld, gd, t);
+        else if constexpr (J == 3) return SISe3_I_2_to_S_2(u, v, ld, gd, t);
+        else if constexpr (J == 4) return SISe3_S_3_to_I_3(u, v, ld, gd, t);
+        else return SISe3_I_3_to_S_3(u, v, ld, gd, t);
+    }
+    SIMINF_MODEL_FN int pts(SIMINF_PTS_ARGS)
+    {
+        return SISe3_sp_pts(vn, u, v, ld, gd, node, t);
     }
 };
 

@@ -65,6 +65,9 @@ __host__ __device__ inline uint64_t error_key(int rep, int day, int phase,
            (uint64_t)(-code & 63);
 }
 
+/* Barrier words per replicate of a grid-wide coupled launch. */
+#define SIMINF_GSYNC_WORDS 4
+
 struct KParams {
     /* Dimensions. */
     int Nn, Nrep, Nld, tlen, len, Nshift;
@@ -172,6 +175,10 @@ struct KParams {
      * exit decision never depends on another's errors. */
     int *u_work;                     /* Nc x (Nn*Nrep) */
     int *upd_work;                   /* Nn*Nrep */
+    /* Models whose post-time-step function reads other nodes: every
+     * node's v at the start of the day, published with its u (Nd x
+     * (Nn*Nrep)); null otherwise. */
+    double *v_work;
     int rep0;
     int rep_blocks;
     unsigned long long *rep_err;
@@ -184,8 +191,9 @@ struct KParams {
     const int *lane_node;
     unsigned long long *node_fired;
     /* Grid launch: the barrier words of each replicate of the launch
-     * (arrivals, last day whose E2 work is done), cleared before each
-     * launch. */
+     * (SIMINF_GSYNC_WORDS each: arrivals, last day whose E2 work is done,
+     * arrivals after the post-time-step phase of models that read other
+     * nodes), cleared before each launch. */
     unsigned *gsync;
 
     /* Node kernels with a dense trajectory: a node whose state can no
@@ -1849,6 +1857,16 @@ struct pts_noop : std::false_type {};
 template <class M>
 struct pts_noop<M, std::void_t<decltype(M::PTS_NOOP)>>
     : std::bool_constant<M::PTS_NOOP> {};
+
+/* Does the model's post-time-step function read other nodes' u and v
+ * (NEIGHBORS, the spatial models)?  Such a model runs in the coupled
+ * kernel, which passes the function pointers into the replicate's
+ * node-major state arrays in global memory. */
+template <class M, class = void>
+struct model_neighbors : std::false_type {};
+template <class M>
+struct model_neighbors<M, std::void_t<decltype(M::NEIGHBORS)>>
+    : std::bool_constant<M::NEIGHBORS> {};
 
 /* A node whose state can no longer change (all rates zero, no events
  * left, a post-time-step function that does nothing) holds the same

@@ -57,11 +57,14 @@ const SimInf_hip_model *siminf_builtin_SEIR();
 const SimInf_hip_model *siminf_builtin_SIS();
 const SimInf_hip_model *siminf_builtin_SISe();
 const SimInf_hip_model *siminf_builtin_SISe3();
+const SimInf_hip_model *siminf_builtin_SISe_sp();
+const SimInf_hip_model *siminf_builtin_SISe3_sp();
 const SimInf_hip_model *siminf_builtin_SIR_mparse();
 
 static const SimInf_hip_model *(*const builtin_models[])() = {
     siminf_builtin_SIR, siminf_builtin_SEIR, siminf_builtin_SIS,
-    siminf_builtin_SISe, siminf_builtin_SISe3, siminf_builtin_SIR_mparse,
+    siminf_builtin_SISe, siminf_builtin_SISe3, siminf_builtin_SISe_sp,
+    siminf_builtin_SISe3_sp, siminf_builtin_SIR_mparse,
 };
 
 extern "C" const SimInf_hip_model *SimInf_hip_model_builtin(const char *name)
@@ -354,6 +357,29 @@ static size_t coupled_lds_layout(KParams &p, int Nc, int Nselect, int Nshift,
 /* filter_plan: u0/v0 hold one replicate (Nn columns), copied on the
  * device to all a->Nrep replicates, and the caller runs every lane in one
  * pass (no frozen-node buffers). */
+/* A neighbour-reading model's lists in ldata (Nld rows, ncol columns):
+ * every column's (index, value) pairs from row `row` must end with an
+ * index below zero inside the column and name nodes 0..Nn-1 (the index
+ * converted to int as the plugin converts it), or the post-time-step
+ * function would read outside the state arrays. */
+static bool neighbor_lists_ok(const double *ldata, int Nld, long long ncol, int row, int Nn)
+{
+    for (long long j = 0; j < ncol; j++) {
+        const double *col = ldata + j * (long long)Nld;
+        int r = row;
+        for (;; r += 2) {
+            if (r >= Nld || !(col[r] > -2147483648.0 && col[r] < 2147483648.0))
+                return false;
+            const int idx = (int)col[r];
+            if (idx < 0)
+                break;
+            if (idx >= Nn || r + 1 >= Nld)
+                return false;
+        }
+    }
+    return true;
+}
+
 static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
                             const SimInf_hip_model *model, bool filter_plan)
 {
@@ -380,6 +406,9 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         return SIMINF_ERR_SPARSE_MODEL;
     if (!a->V && (!a->irV || !a->jcV || !a->prV))
         return SIMINF_ERR_SPARSE_MODEL;
+    if (model->neighbors > 0 &&
+        (!a->ldata || !neighbor_lists_ok(a->ldata, a->Nld, Nn, model->neighbors - 1, Nn)))
+        return SIMINF_ERR_INVALID_MODEL;
 
     SimInf_hip_plan *P = new SimInf_hip_plan();
     P->model = model;
@@ -623,12 +652,14 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         P->canon = same;
     }
 
-    /* Kernel choice. */
+    /* Kernel choice.  External transfers couple the nodes, and so does a
+     * post-time-step function that reads other nodes' state. */
+    const bool coupled = n_e2 > 0 || model->neighbors;
     if (a->solver == 1) {
-        /* aem runs one replicate; with external transfers the nodes are
-         * coupled and it runs in the grid-wide coupled kernel. */
-        P->kernel = n_e2 > 0 ? K_COUPLED_GRID : K_AEM_NODES;
-    } else if (n_e2 == 0) {
+        /* aem runs one replicate; with coupled nodes it runs in the
+         * grid-wide coupled kernel. */
+        P->kernel = coupled ? K_COUPLED_GRID : K_AEM_NODES;
+    } else if (!coupled) {
         P->kernel = K_SSM_NODES;
     } else {
         P->kernel = (Nrep == 1 || Nn > SIMINF_REP_BLOCK) ? K_COUPLED_GRID
@@ -768,6 +799,8 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
     if (P->kernel == K_COUPLED_GRID || P->kernel == K_COUPLED_REPS) {
         p.u_work = B.alloc<int>((size_t)lanes * Nc);
         p.upd_work = B.alloc<int>((size_t)lanes);
+        if (model->neighbors)
+            p.v_work = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
         /* LDS budget: at most 128 KiB, and small enough that the
          * workgroups a grid-wide launch needs fit on the CUs together
          * (every workgroup gets the same allocation). */
@@ -851,7 +884,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
             p.node_fired = B.alloc<unsigned long long>((size_t)Nn);
             /* Barrier words for the most replicates one launch can hold,
              * and each replicate's error key. */
-            p.gsync = B.alloc<unsigned>(2 * (size_t)std::min(Nrep, max_blocks));
+            p.gsync = B.alloc<unsigned>(SIMINF_GSYNC_WORDS * (size_t)std::min(Nrep, max_blocks));
             if (Nrep > 1)
                 p.rep_err = B.alloc<unsigned long long>((size_t)Nrep);
         } else {
@@ -967,7 +1000,7 @@ __global__ void k_run_init(unsigned long long *err_key, unsigned long long *tran
         if (active_count)
             *active_count = 0;
     }
-    if (gsync && threadIdx.x < 2)
+    if (gsync && threadIdx.x < SIMINF_GSYNC_WORDS)
         gsync[threadIdx.x] = 0u;
     if (rep_err)
         for (int i = (int)threadIdx.x; i < nrep; i += (int)blockDim.x)
@@ -1291,6 +1324,10 @@ extern "C" int SimInf_hip_plan_set_replicate_data(SimInf_hip_plan *P, const doub
     if (ldata && nl == 0)
         return SIMINF_ERR_INVALID_MODEL;
     if (gdata && ng == 0)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (ldata && P->model->neighbors > 0 &&
+        !neighbor_lists_ok(ldata, P->Nld, (long long)ldata_nodes * P->Nrep,
+                           P->model->neighbors - 1, P->Nn))
         return SIMINF_ERR_INVALID_MODEL;
     DeviceBuffers &B = P->buf;
     if (ldata) {

@@ -1485,13 +1485,18 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     const int local = GRID ? (int)blockIdx.x - rb * G : 0;
     const int rep = GRID ? p.rep0 + rb : (int)blockIdx.x;
     const int lane_g = GRID ? (int)(local * blockDim.x + threadIdx.x) : (int)threadIdx.x;
-    unsigned *gsync = GRID ? p.gsync + 2 * rb : nullptr;
+    unsigned *gsync = GRID ? p.gsync + SIMINF_GSYNC_WORDS * rb : nullptr;
+    /* A post-time-step function that reads other nodes gets pointers into
+     * the replicate's published u and v (node-major) and the node's whole
+     * ldata column. */
+    constexpr bool NB = model_neighbors<M>::value;
     const int node = GRID && p.lane_node ? p.lane_node[lane_g] : lane_g;
     const bool active = node >= 0 && node < p.Nn;
     const long long base = (long long)rep * p.Nn;
     const long long g = base + node;
     int *uw = p.u_work + base * NC;
     int *updw = p.upd_work + base;
+    double *vw = NB ? p.v_work + base * ND : nullptr;
     unsigned long long fired = 0;
     double tt = p.tt0, next = p.next0;
     int U_it = 0;
@@ -1537,10 +1542,10 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
      * gsync[1] is the last day whose E2 work is done.  A wavefront
      * publishes its writes device-wide before it arrives or signals, and
      * observes the others' after its wait. */
-    auto gs_arrive = [&]() {
+    auto gs_arrive = [&](unsigned *at) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (threadIdx.x == 0)
-            __hip_atomic_fetch_add(gsync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(at, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
     auto gs_wait = [&](const unsigned *at, unsigned target) {
         if (threadIdx.x == 0)
@@ -1550,7 +1555,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
     };
     auto barrier = [&]() {
         if constexpr (GRID) {
-            gs_arrive();
+            gs_arrive(gsync);
             gs_wait(gsync, (unsigned)G);
         } else {
             __syncthreads();
@@ -1654,10 +1659,16 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                     myerr = 1;
                 }
             }
-            if (touched) {
+            if (touched || NB) {
                 static_for<NC>([&](auto C) {
                     constexpr int c = decltype(C)::value;
                     uw[node * NC + c] = s.u[c];
+                });
+            }
+            if constexpr (NB) {
+                static_for<ND>([&](auto C) {
+                    constexpr int c = decltype(C)::value;
+                    vw[node * ND + c] = s.v[c];
                 });
             }
         }
@@ -1677,7 +1688,7 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                     __hip_atomic_store(gsync + 1, (unsigned)(day + 1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                gs_arrive();
+                gs_arrive(gsync);
                 gs_wait(gsync + 1, (unsigned)(day + 1));
             }
         } else {
@@ -1705,7 +1716,15 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                 hcur++;
                 hnext = hcur < hend ? p.e2_hday[hcur] : INT_MAX;
             }
-            const int rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+            int rc;
+            if constexpr (NB) {
+                const long long li = p.ld_rep == 2 ? (long long)rep
+                                     : (p.ld_rep ? base : 0LL) + node;
+                rc = M::pts(s.vnew, uw + node * NC, vw + node * ND,
+                            p.ldata + li * (long long)p.Nld, cx.gd, node, tt);
+            } else {
+                rc = M::pts(s.vnew, s.u, s.v, cx.ld, cx.gd, node, tt);
+            }
             if (rc < 0) {
                 record_error(p, rep, day, PH_PTS, node, rc);
                 myerr = 1;
@@ -1730,6 +1749,20 @@ __global__ __launch_bounds__(BLOCK) void k_coupled(KParams p)
                     record_error(p, rep, day, PH_PTS, node, err);
                     myerr = 1;
                 }
+            }
+        }
+        if constexpr (NB) {
+            /* Every node's post-time-step function has read the day's
+             * published state before any node publishes the next day's
+             * (the E2 workgroup of a grid launch writes herd state only
+             * after the next day's arrivals, which come after this). */
+            if constexpr (GRID) {
+                if (!e2wg) {
+                    gs_arrive(gsync + 2);
+                    gs_wait(gsync + 2, (unsigned)(day + 1) * (unsigned)(G - 1));
+                }
+            } else {
+                __syncthreads();
             }
         }
         tt = next;
@@ -1853,7 +1886,7 @@ inline unsigned fill_grid(long long lanes, int tlen)
 }
 
 /* Layout version of SimInf_hip_model, checked when a plugin is used. */
-#define SIMINF_HIP_PLUGIN_ABI 2
+#define SIMINF_HIP_PLUGIN_ABI 3
 
 /* Signature of the KParams layout a plugin was compiled against: the
  * library hands its KParams to the plugin's kernels, so a plugin built
@@ -1881,6 +1914,13 @@ struct SimInf_hip_model {
                       hipStream_t);
     hipError_t (*coupled)(const siminf::KParams &, bool, bool, unsigned,
                           unsigned, size_t, hipStream_t, int *);
+    /* Non-zero when the post-time-step function reads other nodes'
+     * state: 1 + the ldata row where each node's (neighbour index,
+     * value) pairs start, ended by an index below zero.  Every plan of
+     * such a model runs the coupled kernel (ssm_nodes and aem_nodes are
+     * null), and the host checks the lists before they reach the
+     * device. */
+    int neighbors;
 };
 
 namespace siminf {
@@ -1994,6 +2034,32 @@ static const void *coupled_fn()
                 : (const void *)k_coupled<M, CANON, false, SIMINF_REP_BLOCK>;
 }
 
+/* The node kernels of a model's record: none for a model whose
+ * post-time-step function reads other nodes (it never runs them, and
+ * their register-resident state could not serve such reads). */
+typedef void (*NodeLaunch)(const KParams &, bool, bool, unsigned, unsigned, hipStream_t);
+template <class M, bool NB = model_neighbors<M>::value>
+struct NodeLaunchers {
+    static constexpr NodeLaunch ssm = launch_ssm_nodes<M>;
+    static constexpr NodeLaunch aem = launch_aem_nodes<M>;
+};
+template <class M>
+struct NodeLaunchers<M, true> {
+    static constexpr NodeLaunch ssm = nullptr;
+    static constexpr NodeLaunch aem = nullptr;
+};
+
+/* The ldata row where a neighbour-reading model's pairs start; -1 for
+ * other models. */
+template <class M>
+constexpr int neighbor_row()
+{
+    if constexpr (model_neighbors<M>::value)
+        return M::NEIGHBOR_ROW;
+    else
+        return -1;
+}
+
 /* The all-events method runs one replicate only: grid launches. */
 template <class M, bool CANON>
 static const void *coupled_aem_fn()
@@ -2037,9 +2103,10 @@ hipError_t launch_coupled(const KParams &p, bool canon, bool grid_mode,
     if (grid_mode) {
         if (!p.gsync)
             return hipErrorInvalidValue;
-        /* Two barrier words per replicate of the launch. */
+        /* The barrier words of each replicate of the launch. */
         const int nrep = p.rep_blocks > 0 ? (int)grid / p.rep_blocks : 1;
-        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, st, p.gsync, 2 * nrep);
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(256), 0, st, p.gsync,
+                           SIMINF_GSYNC_WORDS * nrep);
         KParams q = p;
         void *args[] = {&q};
         return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args,
@@ -2072,7 +2139,7 @@ inline constexpr CanonTables<M> canon_tables{};
     {SIMINF_HIP_PLUGIN_ABI, SIMINF_KPARAMS_SIG, NAME, M::NC, M::NT, M::ND, \
      NGDATA,                                                              \
      siminf::canon_tables<M>.S, siminf::canon_tables<M>.G,               \
-     siminf::launch_ssm_nodes<M>, siminf::launch_aem_nodes<M>,            \
-     siminf::launch_coupled<M>}
+     siminf::NodeLaunchers<M>::ssm, siminf::NodeLaunchers<M>::aem,        \
+     siminf::launch_coupled<M>, siminf::neighbor_row<M>() + 1}
 
 } /* namespace siminf */

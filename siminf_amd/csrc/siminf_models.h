@@ -13,8 +13,9 @@
  * the CPU and the GPU (no contraction: -ffp-contract=off).
  *
  * Models: SIR (src/models/SIR.c), SEIR (src/models/SEIR.c), SIS
- * (src/models/SIS.c), SISe (src/models/SISe.c), SISe3 (src/models/SISe3.c)
- * and the model that mparse emits for
+ * (src/models/SIS.c), SISe (src/models/SISe.c), SISe3 (src/models/SISe3.c),
+ * SISe_sp and SISe3_sp (src/models/SISe_sp.c, SISe3_sp.c, with
+ * src/misc/SimInf_local_spread.c) and the model that mparse emits for
  *     transitions = c("S -> beta*S*I/N -> I", "I -> gamma*I -> R",
  *                     "N <- S+I+R"), gdata = c(beta, gamma)
  * (R/mparse.R:647-724, R/C-generator.R:170-297), named "SIR_mparse".
@@ -270,6 +271,85 @@ SIMINF_FN int SISe3_pts(double *v_new, const int *u, const double *v,
         v_new[0] += gdata[6] * I_n / n + gdata[11];
     else
         v_new[0] += gdata[11];
+    if (!siminf_isfinite(v_new[0]))
+        return SIMINF_MODEL_ERR_V_IS_NOT_FINITE;
+    if (v_new[0] < 0.0)
+        return SIMINF_MODEL_ERR_V_IS_NEGATIVE;
+    return phi != v_new[0];
+}
+
+/* ----------------------------------------------- spatial models (_sp) */
+/* The neighbours' share of a node's phi (SimInf_local_spread.c:45-75).
+ * nb lists (index, distance) pairs of replicate-local node indices and
+ * ends at an index below zero; phi0 and u0 point at the first node's
+ * continuous and compartment state, so that node j's sit at phi0[j] and
+ * u0[j * Nc].  A neighbour without individuals adds nothing. */
+SIMINF_FN double siminf_local_spread(const double *nb, const double *phi0,
+                                     const int *u0, double N_i, double phi_i,
+                                     int Nc, double D)
+{
+    const double own = phi_i * N_i;
+    double ls = 0.0;
+    for (int j = (int)nb[0]; j >= 0; nb += 2, j = (int)nb[0]) {
+        double N_j = 0;
+        for (int k = 0; k < Nc; k++)
+            N_j += u0[j * Nc + k];
+        if (N_j > 0.0)
+            ls += ((phi0[j] * N_j - own) * D) / (N_i * nb[1]);
+    }
+    return ls;
+}
+
+/* SISe_sp: u = (S, I); v = (phi); ldata = (end_t1..end_t4, then the
+ * neighbour pairs); gdata = (upsilon, gamma, alpha, beta_t1..beta_t4,
+ * coupling).  The rates are SISe's (SISe_sp.c:43-75); the post-time-step
+ * function adds the local spread and reads the other nodes' u and v
+ * through negative offsets from its own (SISe_sp.c:90-138), so u and v
+ * must point into the replicate's node-major arrays. */
+SIMINF_FN int SISe_sp_pts(double *v_new, const int *u, const double *v,
+                          const double *ldata, const double *gdata,
+                          int node, double t)
+{
+    const int day = (int)t % 365;
+    const double I_i = u[1];
+    const double N_i = u[0] + I_i;
+    const double phi = v[0];
+    v_new[0] = siminf_euler_linear_decay(phi, day,
+                                         (int)ldata[0], (int)ldata[1],
+                                         (int)ldata[2], (int)ldata[3],
+                                         gdata[3], gdata[4], gdata[5],
+                                         gdata[6]);
+    if (N_i > 0.0)
+        v_new[0] += gdata[2] * I_i / N_i +
+                    siminf_local_spread(ldata + 4, v - node, u - 2 * node,
+                                        N_i, phi, 2, gdata[7]);
+    if (!siminf_isfinite(v_new[0]))
+        return SIMINF_MODEL_ERR_V_IS_NOT_FINITE;
+    if (v_new[0] < 0.0)
+        return SIMINF_MODEL_ERR_V_IS_NEGATIVE;
+    return phi != v_new[0];
+}
+
+/* SISe3_sp: SISe3's compartments and rates; gdata = (upsilon_1..3,
+ * gamma_1..3, alpha, beta_t1..beta_t4, coupling); ldata as SISe_sp's
+ * (SISe3_sp.c:206-255). */
+SIMINF_FN int SISe3_sp_pts(double *v_new, const int *u, const double *v,
+                           const double *ldata, const double *gdata,
+                           int node, double t)
+{
+    const int day = (int)t % 365;
+    const double I_i = u[1] + u[3] + u[5];
+    const double N_i = u[0] + u[2] + u[4] + I_i;
+    const double phi = v[0];
+    v_new[0] = siminf_euler_linear_decay(phi, day,
+                                         (int)ldata[0], (int)ldata[1],
+                                         (int)ldata[2], (int)ldata[3],
+                                         gdata[7], gdata[8], gdata[9],
+                                         gdata[10]);
+    if (N_i > 0.0)
+        v_new[0] += gdata[6] * I_i / N_i +
+                    siminf_local_spread(ldata + 4, v - node, u - 6 * node,
+                                        N_i, phi, 6, gdata[11]);
     if (!siminf_isfinite(v_new[0]))
         return SIMINF_MODEL_ERR_V_IS_NOT_FINITE;
     if (v_new[0] < 0.0)

@@ -11,6 +11,8 @@
                       split 10/15/75 % by age, 1 % of herds seeded, with
                       scheduled enter / exit / internal / external events
                       (parameters of man/examples/SISe3.R)
+  sise3_sp_herds      the sise3_herds herds and events with local spread
+                      of phi among herds on a jittered grid (SISe3_sp)
   seir_million        SEIR on Nn ~ Poisson(500) nodes, 0.5 % seeded with
                       5 infected, beta 0.16, epsilon 0.25, gamma 0.077
   pfilter_requests    particle-filter requests: SIR on node counts
@@ -227,6 +229,31 @@ def sise3_herds(Nn=1600, days=1460, seed=11, events=True, replicates=1,
     return models.SISe3(u0=u0, tspan=np.arange(1, days + 1, dtype=float),
                         events=ev, phi=0.0, end_t1=91, end_t2=182, end_t3=273,
                         end_t4=365, replicates=replicates, **SISE3_GDATA)
+
+
+def sise3_sp_herds(Nn=1600, days=1460, seed=11, events=True, replicates=1,
+                   coupling=0.0005, **event_rates):
+    """The herds and events of sise3_herds on a grid of spacing 0.8 with
+    each point moved by up to 0.3 along each axis, each herd coupled
+    (SISe3_sp, local spread of phi) to the two to eight herds within
+    distance 1."""
+    from . import spatial
+    base = sise3_herds(Nn, days, seed, events=events, replicates=1, **event_rates)
+    rng = np.random.default_rng(seed + 1000)
+    k = int(np.ceil(np.sqrt(Nn)))
+    gx, gy = np.meshgrid(np.arange(k), np.arange(k))
+    x = 0.8 * gx.ravel()[:Nn] + rng.uniform(-0.3, 0.3, Nn)
+    y = 0.8 * gy.ravel()[:Nn] + rng.uniform(-0.3, 0.3, Nn)
+    distance = spatial.distance_matrix(x, y, 1.0)
+    g = dict(zip(base.gdata_names, base.gdata))
+    return models.SISe3_sp(
+        u0=base.u0[:, :Nn].T, tspan=base.tspan, events=base.events if events else None,
+        phi=0.0, upsilon_1=g["upsilon_1"], upsilon_2=g["upsilon_2"],
+        upsilon_3=g["upsilon_3"], gamma_1=g["gamma_1"], gamma_2=g["gamma_2"],
+        gamma_3=g["gamma_3"], alpha=g["alpha"], beta_t1=g["beta_t1"],
+        beta_t2=g["beta_t2"], beta_t3=g["beta_t3"], beta_t4=g["beta_t4"],
+        end_t1=91, end_t2=182, end_t3=273, end_t4=365, distance=distance,
+        coupling=coupling, replicates=replicates)
 
 
 class FilterSpec:
