@@ -170,21 +170,27 @@ def bench_batch(args):
     tuned = {}
     if args.shape:
         # A launch shape given on the command line (as a profiled run
-        # repeats the bench's autotuned one): lanes-per-wavefront shifts
-        # and "graph" or "streams".
+        # repeats the bench's autotuned one): lanes-per-wavefront shifts,
+        # columns per fill block ("fill": {solver: cols}) and "graph" or
+        # "streams".
         spec = json.loads(args.shape)
+        fills = spec.get("fill", {})
         for s, p in zip(solvers, plans):
             p.set_lanes_per_wave(int(spec[s]))
+            p.set_fill_block(int(fills.get(s, 0)))
         batch.close()
         batch = si.Batch(plans)
         batch.set_graph(spec.get("launch", "streams") == "graph")
         tuned = {"lanes_per_wave_shift": {s: int(spec[s]) for s in solvers},
+                 "fill_block": {s: p.fill_block() for s, p in zip(solvers, plans)},
                  "launch": spec.get("launch", "streams"), "given": True}
     elif not args.no_autotune:
         # The launch shape is tuned for the plans together: each plan's
-        # lanes per wavefront and the graph against the direct enqueue.
+        # lanes per wavefront and columns per fill block, and the graph
+        # against the direct enqueue.
         ms, shifts, graph = batch.autotune(seed=200, reps=5)
         tuned = {"lanes_per_wave_shift": dict(zip(solvers, shifts)),
+                 "fill_block": {s: p.fill_block() for s, p in zip(solvers, plans)},
                  "launch": "graph" if graph else "streams", "tuned_ms": ms}
     for i in range(args.warmup):
         batch.run([300 + 2 * i, 301 + 2 * i])
@@ -259,6 +265,7 @@ def bench_batch(args):
     if not args.no_extras:
         for key, fn in (("pcie_inclusive", lambda: pcie_inclusive(model)),
                         ("sise3_herds", lambda: bench_sise3(args)),
+                        ("sise3_sp_herds", lambda: bench_sise3(args, spatial=True)),
                         ("sise3_replicates", bench_sise3_replicates),
                         ("pfilter_request", lambda: bench_pfilter_request(args))):
             try:
@@ -300,13 +307,15 @@ def pcie_inclusive(model):
     return out
 
 
-def bench_sise3(args, steps=10):
+def bench_sise3(args, steps=10, spatial=False):
     """SISe3, 1600 herds x 1460 days with scheduled events (one replicate,
     grid-wide coupled kernel): mean of `steps` resident runs after the
-    autotuner."""
+    autotuner.  spatial: the same herds and events under SISe3_sp, each
+    herd's phi also spreading to the two to eight herds within distance 1
+    (every herd's state published each day, a second barrier per day)."""
     import siminf_amd as si
     from siminf_amd import workloads as W
-    model = W.sise3_herds()
+    model = W.sise3_sp_herds() if spatial else W.sise3_herds()
     with si.Plan(model) as p:
         p.run(100)
         if not args.no_autotune:
@@ -321,8 +330,8 @@ def bench_sise3(args, steps=10):
             nd += b
             ms.append(c)
         wall = time.perf_counter() - t0
-    return {"workload": "SISe3 1600 herds x 1460 days, %d scheduled events"
-                        % len(model.events["event"]),
+    return {"workload": "%s 1600 herds x 1460 days, %d scheduled events"
+                        % ("SISe3_sp" if spatial else "SISe3", len(model.events["event"])),
             "node_days_per_s": nd / wall, "transitions_per_s": tr / wall,
             "ms_per_run": wall / steps * 1e3, "kernel_ms_mean": statistics.mean(ms),
             "herd_map": {"solo_herds": solo, "herds_per_wavefront": pack},

@@ -247,6 +247,16 @@ int SimInf_hip_plan_set_replicate_data(SimInf_hip_plan *plan, const double *ldat
  * (no reference counterpart). */
 int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *plan, int shift);
 
+/* The trajectory-flush schedule of a plan whose node kernels leave the
+ * columns of nodes whose state stopped changing to a fill kernel: each
+ * of its workgroups writes `cols` columns of 256 nodes (1..4096; 0 for
+ * the default, 8).  Results do not depend on it (no reference
+ * counterpart). */
+int SimInf_hip_plan_set_fill_block(SimInf_hip_plan *plan, int cols);
+
+/* The columns per fill block the plan uses. */
+int SimInf_hip_plan_get_fill_block(const SimInf_hip_plan *plan, int *cols);
+
 /* For a plan whose coupled kernel spans the grid (one replicate with
  * external transfers, or more herds than one workgroup holds): give each
  * of the n herds `nodes` (zero-based, distinct, n <= 16) a wavefront of
@@ -273,8 +283,10 @@ int SimInf_hip_plan_get_herd_map(const SimInf_hip_plan *plan, int *n_solo,
  * 64, 32, 16 or 8 to a wavefront; then, with the best of those, 0, 2, 8
  * or 16 herds on their own (an infeasible candidate's time is +Inf);
  * *best_shift gets the kept solo count (SimInf_hip_plan_get_herd_map
- * gives both).  Other plans: -1, nothing run.  The last run's trajectory
- * is the last candidate's. */
+ * gives both).  Other plans: -1, nothing run.  Node kernels with a fill
+ * kernel then try 4, 8, 16, 32 and 64 columns per fill block and keep
+ * the least mean device time of the run (not reported in ms).  The last
+ * run's trajectory is the last candidate's. */
 int SimInf_hip_plan_autotune(SimInf_hip_plan *plan, unsigned long int seed, int reps,
                              int *best_shift, double *ms);
 
@@ -312,10 +324,11 @@ int SimInf_hip_batch_set_graph(SimInf_hip_batch *batch, int use_graph);
 
 /* Autotuner of the batch's launch shape (no reference counterpart): the
  * plans share the device, so each plan's lanes per wavefront (see
- * SimInf_hip_plan_set_lanes_per_wave) is chosen by coordinate descent on
- * the batch's launch time (two passes, every shift of one plan with the
- * others held, reps launches each, seeds from seed), then the graph or
- * the direct enqueue, whichever is faster, is kept.  best_ms (optional)
+ * SimInf_hip_plan_set_lanes_per_wave) and columns per fill block (see
+ * SimInf_hip_plan_set_fill_block) are chosen by coordinate descent on
+ * the batch's launch time (two passes, every candidate of one plan with
+ * the others held, reps launches each, seeds from seed), then the graph
+ * or the direct enqueue, whichever is faster, is kept.  best_ms (optional)
  * gets the kept shape's mean launch time.  Results do not depend on it. */
 int SimInf_hip_batch_autotune(SimInf_hip_batch *batch, unsigned long int seed, int reps,
                               double *best_ms);

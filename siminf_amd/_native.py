@@ -99,6 +99,8 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_plan_trajectory_v",
     "SimInf_hip_abc_weights",
     "SimInf_hip_plan_set_lanes_per_wave",
+    "SimInf_hip_plan_set_fill_block",
+    "SimInf_hip_plan_get_fill_block",
     "SimInf_hip_plan_set_herd_map",
     "SimInf_hip_plan_get_herd_map",
     "SimInf_hip_plan_autotune",
@@ -174,6 +176,10 @@ def load(path=None):
     lib.SimInf_hip_plan_set_replicate_data.argtypes = [vp, dp, ctypes.c_int, dp]
     lib.SimInf_hip_plan_set_lanes_per_wave.restype = ctypes.c_int
     lib.SimInf_hip_plan_set_lanes_per_wave.argtypes = [vp, ctypes.c_int]
+    lib.SimInf_hip_plan_set_fill_block.restype = ctypes.c_int
+    lib.SimInf_hip_plan_set_fill_block.argtypes = [vp, ctypes.c_int]
+    lib.SimInf_hip_plan_get_fill_block.restype = ctypes.c_int
+    lib.SimInf_hip_plan_get_fill_block.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     lib.SimInf_hip_plan_set_herd_map.restype = ctypes.c_int
     lib.SimInf_hip_plan_set_herd_map.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                                  ctypes.c_int]

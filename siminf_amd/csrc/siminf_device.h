@@ -222,6 +222,9 @@ struct KParams {
     int simds;
     int *active;
     int *active_count;
+    /* The fill kernel's trajectory-flush schedule: the columns one of its
+     * workgroups writes per 256 lanes (0: SIMINF_FILL_COLS). */
+    int fill_block;
 
     /* Results. */
     unsigned long long *err_key;

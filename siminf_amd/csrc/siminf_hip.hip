@@ -236,6 +236,9 @@ struct SimInf_hip_plan {
     /* Lanes per wavefront of the active pass as a power of two, or -1
      * for the rule of about two wavefronts per SIMD. */
     int lpw_fixed = -1;
+    /* Columns per fill block (the trajectory-flush schedule), 0 for the
+     * default; tuned by the autotuners. */
+    int fill_block = 0;
     /* Grid-wide coupled plans: herds given a wavefront of their own, the
      * other herds per wavefront, and the workgroups a grid launch may
      * hold. */
@@ -1096,7 +1099,8 @@ static int plan_enqueue_phase(SimInf_hip_plan *P, const KParams &base, bool capt
                 (after_fill && hipStreamWaitEvent(fs, after_fill, 0) != hipSuccess))
                 return SIMINF_ERR_DEVICE;
             mark(P->ev_fill0, fs);
-            const dim3 fg(fill_grid(P->p.lanes, P->tlen)), fb(256);
+            p.fill_block = P->fill_block;
+            const dim3 fg(fill_grid(p)), fb(256);
             const bool v = P->Nd > 0;
 #define SIMINF_FILL(NCV)                                                          \
     do {                                                                          \
@@ -1354,11 +1358,36 @@ extern "C" int SimInf_hip_plan_set_replicate_data(SimInf_hip_plan *P, const doub
     return B.status == hipSuccess ? 0 : SIMINF_ERR_DEVICE;
 }
 
+/* Columns per fill block that the autotuners try. */
+static const int fill_block_candidates[] = {4, 8, 16, 32, 64};
+
+extern "C" int SimInf_hip_plan_set_fill_block(SimInf_hip_plan *P, int cols)
+{
+    if (!P || cols < 0 || cols > 4096)
+        return SIMINF_ERR_INVALID_MODEL;
+    if (cols != P->fill_block) {
+        P->fill_block = cols;
+        P->param_version++;      /* a captured batch graph holds the old one */
+    }
+    return 0;
+}
+
+extern "C" int SimInf_hip_plan_get_fill_block(const SimInf_hip_plan *P, int *cols)
+{
+    if (!P || !cols)
+        return SIMINF_ERR_INVALID_MODEL;
+    *cols = P->fill_block > 0 ? P->fill_block : SIMINF_FILL_COLS;
+    return 0;
+}
+
 extern "C" int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *P, int shift)
 {
     if (!P || shift < -1 || shift > 6)
         return SIMINF_ERR_INVALID_MODEL;
-    P->lpw_fixed = shift;
+    if (shift != P->lpw_fixed) {
+        P->lpw_fixed = shift;
+        P->param_version++;
+    }
     return 0;
 }
 
@@ -1453,6 +1482,30 @@ extern "C" int SimInf_hip_plan_autotune(SimInf_hip_plan *P, unsigned long int se
     P->lpw_fixed = best;
     if (best_shift)
         *best_shift = best;
+    /* Then the fill kernel's columns per block, by the run's device time
+     * (the fill runs beside the active pass). */
+    if (P->p.frozen_from) {
+        int best_fb = P->fill_block;
+        double best_run = INFINITY;
+        for (int fb : fill_block_candidates) {
+            P->fill_block = fb;
+            double sum = 0.0;
+            for (int r = 0; r < reps; r++) {
+                const int err = SimInf_hip_plan_run(P, seed + (unsigned long)r);
+                if (err) {
+                    P->fill_block = best_fb;
+                    return err;
+                }
+                sum += P->kernel_ms;
+            }
+            if (sum / reps < best_run) {
+                best_run = sum / reps;
+                best_fb = fb;
+            }
+        }
+        P->fill_block = best_fb;
+    }
+    P->param_version++;
     return 0;
 }
 
@@ -1744,13 +1797,14 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
 }
 
 /* Autotuner of a batch's launch (no reference counterpart): the plans
- * share the device, so each one's lanes per wavefront is tuned in the
- * batch rather than alone.  Coordinate descent, two passes: for each plan
- * with an active pass in turn, every shift (-1 for the default rule, 0
- * to 6) with the others held, keeping the least mean launch time over
- * reps launches (seeds from seed); then the graph against the direct
- * enqueue, the faster kept (the graph recaptured with the kept shapes).
- * Results do not depend on any of it. */
+ * share the device, so each one's lanes per wavefront and fill block
+ * are tuned in the batch rather than alone.  Coordinate descent, two
+ * passes: for each plan with an active pass and a fill kernel in turn,
+ * every shift (-1 for the default rule, 0 to 6), then every columns-per-
+ * fill-block candidate, with the others held, keeping the least mean
+ * launch time over reps launches (seeds from seed); then the graph
+ * against the direct enqueue, the faster kept (the graph recaptured with
+ * the kept shapes).  Results do not depend on any of it. */
 extern "C" int SimInf_hip_batch_autotune(SimInf_hip_batch *B, unsigned long int seed, int reps,
                                          double *best_ms)
 {
@@ -1794,6 +1848,22 @@ extern "C" int SimInf_hip_batch_autotune(SimInf_hip_batch *B, unsigned long int 
                 }
             }
             P->lpw_fixed = keep;
+            /* The plan's trajectory-flush schedule: columns per fill
+             * block. */
+            const int fb0 = P->fill_block;
+            int keep_fb = fb0;
+            for (int fb : fill_block_candidates) {
+                if (err || fb == fb0)
+                    continue;
+                P->fill_block = fb;
+                double t = 0.0;
+                err = measure(&t);
+                if (!err && t < best) {
+                    best = t;
+                    keep_fb = fb;
+                }
+            }
+            P->fill_block = keep_fb;
         }
     }
     if (!err)

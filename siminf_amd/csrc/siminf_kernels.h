@@ -1818,14 +1818,16 @@ __device__ __forceinline__ void store_nt(int *dst, const int (&u)[NC])
 
 /* The columns frozen nodes left unwritten (node kernels, dense U/V).
  * Workgroup b covers 256 lanes (replicate * Nn + node) of lane block
- * b % nb and the SIMINF_FILL_COLS columns of column block b / nb, so the
+ * b % nb and the fb columns of column block b / nb (fb = the plan's
+ * fill block, SIMINF_FILL_COLS by default), so the
  * workgroups resident together write neighbouring stretches of the same
  * few columns (U column-major, node-major within a column: one
  * wavefront's store is 64 consecutive nodes' counts, contiguous).  NC > 0
  * gives compile-time compartment counts; NC == 0 reads the count from
  * nc.  HASV adds V (nd doubles per node-column). */
 template <int NC, bool HASV>
-__device__ __forceinline__ void fill_block(const KParams &p, int nc, int nd, long long b)
+__device__ __forceinline__ void fill_block(const KParams &p, int nc, int nd, int fb,
+                                           long long b)
 {
     const long long nb = (p.lanes + 255) / 256;
     const long long lb = b % nb, cb = b / nb;
@@ -1833,8 +1835,8 @@ __device__ __forceinline__ void fill_block(const KParams &p, int nc, int nd, lon
     if (g >= p.lanes)
         return;
     const int from = p.frozen_from[g];
-    const int c0 = (int)(cb * SIMINF_FILL_COLS);
-    const int c1 = min(c0 + SIMINF_FILL_COLS, p.tlen);
+    const int c0 = (int)(cb * fb);
+    const int c1 = min(c0 + fb, p.tlen);
     const int first = max(from, c0);
     if (first >= c1)
         return;
@@ -1864,25 +1866,33 @@ __device__ __forceinline__ void fill_block(const KParams &p, int nc, int nd, lon
     }
 }
 
-/* The fill's work is (lanes / 256) x (tlen / SIMINF_FILL_COLS) blocks;
- * a launch of at most 2^20 workgroups takes them in order, each
- * workgroup b, b + gridDim.x, ... */
-__host__ __device__ inline long long fill_blocks(long long lanes, int tlen)
+/* Columns per fill block: the plan's, or the default. */
+__host__ __device__ inline int fill_cols_of(const KParams &p)
 {
-    return ((lanes + 255) / 256) * ((tlen + SIMINF_FILL_COLS - 1) / SIMINF_FILL_COLS);
+    return p.fill_block > 0 ? p.fill_block : SIMINF_FILL_COLS;
+}
+
+/* The fill's work is (lanes / 256) x (tlen / fb) blocks; a launch of at
+ * most 2^20 workgroups takes them in order, each workgroup b,
+ * b + gridDim.x, ... */
+__host__ __device__ inline long long fill_blocks(long long lanes, int tlen, int fb)
+{
+    return ((lanes + 255) / 256) * ((tlen + fb - 1) / fb);
 }
 
 template <int NC, bool HASV>
 __global__ __launch_bounds__(256) void k_fill_frozen(KParams p, int nc, int nd)
 {
-    const long long n = fill_blocks(p.lanes, p.tlen);
+    const int fb = fill_cols_of(p);
+    const long long n = fill_blocks(p.lanes, p.tlen, fb);
     for (long long b = blockIdx.x; b < n; b += gridDim.x)
-        fill_block<NC, HASV>(p, nc, nd, b);
+        fill_block<NC, HASV>(p, nc, nd, fb, b);
 }
 
-inline unsigned fill_grid(long long lanes, int tlen)
+inline unsigned fill_grid(const KParams &p)
 {
-    return (unsigned)std::min<long long>(fill_blocks(lanes, tlen), 1LL << 20);
+    return (unsigned)std::min<long long>(fill_blocks(p.lanes, p.tlen, fill_cols_of(p)),
+                                         1LL << 20);
 }
 
 /* Layout version of SimInf_hip_model, checked when a plugin is used. */

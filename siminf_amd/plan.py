@@ -88,6 +88,17 @@ class Plan:
         the default rule)."""
         self._check(self.lib.SimInf_hip_plan_set_lanes_per_wave(self._handle, int(shift)))
 
+    def set_fill_block(self, cols):
+        """Columns each fill-kernel workgroup writes per 256 nodes (the
+        trajectory-flush schedule; 0: the default)."""
+        self._check(self.lib.SimInf_hip_plan_set_fill_block(self._handle, int(cols)))
+
+    def fill_block(self):
+        """Columns per fill block in use."""
+        c = ctypes.c_int()
+        self._check(self.lib.SimInf_hip_plan_get_fill_block(self._handle, ctypes.byref(c)))
+        return c.value
+
     def set_herd_map(self, solo=(), herds_per_wavefront=64):
         """Grid-wide coupled plans: each herd in solo (zero-based, at most
         16) on a wavefront of its own, the others herds_per_wavefront (8 to
@@ -235,9 +246,10 @@ class Batch:
             _raise(self.lib, err)
 
     def autotune(self, seed=1, reps=2):
-        """Tune each plan's lanes per wavefront in the batch and the graph
-        against the direct enqueue; returns (mean launch ms of the kept
-        shape, [shift per plan], uses_graph)."""
+        """Tune each plan's lanes per wavefront and columns per fill block
+        in the batch, and the graph against the direct enqueue; returns
+        (mean launch ms of the kept shape, [shift per plan], uses_graph);
+        Plan.fill_block() gives each plan's kept fill block."""
         ms = ctypes.c_double()
         err = self.lib.SimInf_hip_batch_autotune(self._handle, int(seed), int(reps),
                                                  ctypes.byref(ms))

@@ -27,6 +27,27 @@ def test_lanes_per_wave_do_not_change_results():
         np.testing.assert_array_equal(plan.download().U, ref)
 
 
+def test_fill_blocks_do_not_change_results():
+    """The fill kernel's columns per block (the trajectory-flush schedule)
+    change time, never results: blocks of 1, 3, 8, 64 and more columns
+    than the run has, then the autotuner's choice."""
+    m = W.seir_million(Nn=20000, days=60, seed=4, seeded_frac=0.02)
+    print("case: SEIR 20000 nodes x 60 days, fill blocks", flush=True)
+    with si.Plan(m) as plan:
+        assert plan.fill_block() == 8
+        plan.run(6)
+        ref = plan.download().U.copy()
+        for cols in (1, 3, 64, 4096):
+            plan.set_fill_block(cols)
+            assert plan.fill_block() == cols
+            plan.run(6)
+            np.testing.assert_array_equal(plan.download().U, ref)
+        plan.autotune(seed=5, reps=1)
+        assert plan.fill_block() in (4, 8, 16, 32, 64)
+        plan.run(6)
+        np.testing.assert_array_equal(plan.download().U, ref)
+
+
 def test_herd_maps_do_not_change_results(oracle):
     """A grid-wide coupled plan (one replicate with external transfers):
     herds moved to wavefronts of their own, or packed fewer to a

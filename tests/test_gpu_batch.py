@@ -136,6 +136,7 @@ def test_batch_autotune_keeps_results(oracle):
             print("autotuned", ms, shifts, graph, flush=True)
             assert ms > 0 and len(shifts) == 3
             assert all(-1 <= x <= 6 for x in shifts)
+            assert all(p.fill_block() in (4, 8, 16, 32, 64) for p in plans)
             for use_graph in (graph, not graph):
                 b.set_graph(use_graph)
                 b.run([21, 22, 23])
