@@ -81,6 +81,10 @@ def make_workload(name):
         solver = "aem" if name.endswith("aem") else "ssm"
         return W.sise3_herds(), {"workload": "SISe3 1600 herds x 1460 days, events",
                                  "solver": solver}
+    if name.startswith("sise3_sp_herds"):
+        solver = "aem" if name.endswith("aem") else "ssm"
+        return W.sise3_sp_herds(), {"workload": "SISe3_sp 1600 herds x 1460 days, events, "
+                                                "local spread of phi", "solver": solver}
     if name == "mparse_sir":
         return W.mparse_sir_readme(), {"workload": "mparse SIR 1000 nodes x 150 days",
                                        "solver": "ssm"}
@@ -241,7 +245,7 @@ def bench_batch(args):
         "dtype": "int32 state, f64 rates/time",
         "data": "synthetic (seeded generator of the configuration's shape)",
         "config": {"workload": "SEIR 1e6 nodes x 365 days, ssm+aem in one batched "
-                               "graph launch",
+                               "launch",
                    "solvers": list(solvers), "nodes": model.Nn, "days": _ndays(model),
                    "seeded_nodes": int((model.u0[2] > 0).sum())},
         "transitions_per_s": trans / wall,

@@ -114,3 +114,16 @@ def test_bad_neighbor_lists_refused():
         with pytest.raises(si.SimInfError) as e:
             si.run(m, seed=1)
         assert "Invalid model" in str(e.value)
+
+
+def test_pfilter_on_a_spatial_model(oracle):
+    """Particles of SISe_sp (one workgroup per particle, each reading its
+    own particle's neighbours) give the checker's filter bit for bit."""
+    from tests.test_gpu_pfilter import _check as pf_check
+    m = C.sise_sp_model(n=60, days=50, seed=9)
+    times = np.arange(5, 51, 5.0)
+    rng = np.random.default_rng(4)
+    obs = rng.poisson(5.0, size=(len(times), 1))
+    err, _ = pf_check(oracle, m, 1, times, obs, 48, seed=7, p=0.5, dist="poisson",
+                      offset=1.0)
+    assert err == 0
