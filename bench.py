@@ -560,6 +560,10 @@ def main():
     ap.add_argument("--requests", type=int, default=24,
                     help="pfilter_stream: requests in the timed stream")
     ap.add_argument("--particles", type=int, default=2000)
+    ap.add_argument("--generations", type=int, default=8,
+                    help="abc_stream: ABC-SMC generations")
+    ap.add_argument("--proposals", type=int, default=10_000,
+                    help="abc_stream: proposals per model and generation")
     ap.add_argument("--rate", type=float, default=4.0, help="pfilter_stream: arrivals/s")
     ap.add_argument("--inflight", type=int, default=8)
     ap.add_argument("--hbm-budget-gib", type=int, default=96)
@@ -593,7 +597,7 @@ def main():
                "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "int32 state, f64 rates/time",
                "data": "synthetic (observations simulated at the generating parameters)"}
-        r = bench_abc_stream(args)
+        r = bench_abc_stream(args, generations=args.generations, proposals=args.proposals)
         out.update({"value": r["proposals_per_s"], "steps": r["proposals"], "warmup": 0,
                     "ms_per_step": 1e3 * r["wall_s"] / r["proposals"],
                     "config": {"workload": r["workload"]}, "abc_stream": r})

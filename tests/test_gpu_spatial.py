@@ -127,3 +127,26 @@ def test_pfilter_on_a_spatial_model(oracle):
     err, _ = pf_check(oracle, m, 1, times, obs, 48, seed=7, p=0.5, dist="poisson",
                       offset=1.0)
     assert err == 0
+
+
+def test_sparse_output_of_a_spatial_model(oracle):
+    """Sparse U and V (punchcard) of SISe_sp with two replicates equal
+    the dense run's values and the checker's."""
+    from siminf_amd.punchcard import punchcard
+    from tests.test_punchcard import sp_values
+    m = C.sise_sp_model(n=150, days=60, seed=10, replicates=2)
+    print("case: SISe_sp 150 herds x 2 replicates, punchcard", flush=True)
+    si.run(m, seed=6)
+    U, V = m.U.copy(), m.V.copy()
+    rng = np.random.default_rng(2)
+    punchcard(m, {"node": rng.integers(1, 151, 400), "time": rng.integers(1, 61, 400),
+                  "S": rng.random(400) < 0.5, "I": rng.random(400) < 0.7,
+                  "phi": rng.random(400) < 0.6})
+    si.run(m, seed=6)
+    cu, cv = m.U_sparse.tocoo(), m.V_sparse.tocoo()
+    np.testing.assert_array_equal(cu.data, U[cu.row, cu.col].astype(float))
+    np.testing.assert_array_equal(cv.data, V[cv.row, cv.col])
+    ref = oracle.run(m, seed=6)
+    assert ref.err == 0
+    np.testing.assert_array_equal(cu.data, sp_values(m.U_sparse, ref.prU))
+    np.testing.assert_array_equal(cv.data, sp_values(m.V_sparse, ref.prV))
