@@ -232,6 +232,11 @@ struct SimInf_hip_plan {
     size_t nnzU, nnzV;
     int ndays;
     uint64_t host_err_key;     /* errors found while validating events */
+    /* A run's result words on the device (error key, transition count,
+     * fill node-columns: p.err_key, p.transitions, p.frozen_cols point
+     * into them) and their pinned host copy, fetched with one copy. */
+    unsigned long long *result_words = nullptr;
+    unsigned long long *host_words = nullptr;
 
     /* Lanes per wavefront of the active pass as a power of two, or -1
      * for the rule of about two wavefronts per SIMD. */
@@ -796,8 +801,15 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
         p.jcV = B.upload(a->jcV, ncols + 1);
         p.prV = B.alloc<double>(P->nnzV);
     }
-    p.err_key = B.alloc<unsigned long long>(1);
-    p.transitions = B.alloc<unsigned long long>(1);
+    P->result_words = B.alloc<unsigned long long>(3);
+    p.err_key = P->result_words;
+    p.transitions = P->result_words + 1;
+    if (hipHostMalloc((void **)&P->host_words, 3 * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
+        P->host_words = nullptr;
+        SimInf_hip_plan_destroy(P);
+        return SIMINF_ERR_ALLOC_MEMORY_BUFFER;
+    }
 
     if (P->kernel == K_COUPLED_GRID || P->kernel == K_COUPLED_REPS) {
         p.u_work = B.alloc<int>((size_t)lanes * Nc);
@@ -901,7 +913,7 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
             p.frozen_from = B.alloc<int>((size_t)lanes);
             p.frozen_u = B.alloc<int>((size_t)lanes * Nc);
             p.frozen_v = B.alloc<double>((size_t)lanes * std::max(Nd, 1));
-            p.frozen_cols = B.alloc<unsigned long long>(1);
+            p.frozen_cols = P->result_words + 2;
             p.active = B.alloc<int>((size_t)lanes);
             p.active_count = B.alloc<int>(1);
         }
@@ -1144,18 +1156,23 @@ static int plan_enqueue(SimInf_hip_plan *P, const KParams &base, bool capture, b
 
 /* Read a finished run's error key and counts (and, when timed, its
  * kernel times); returns the run's error code. */
-static int plan_collect(SimInf_hip_plan *P, bool filled)
+/* Enqueue the copy of a run's result words to the plan's pinned host
+ * copy on stream st (after the run's work on it). */
+static bool plan_fetch(SimInf_hip_plan *P, hipStream_t st)
 {
-    unsigned long long key = NO_ERROR_KEY, trans = 0, fill_cols = 0;
+    return hipMemcpyAsync(P->host_words, P->result_words, 3 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st) == hipSuccess;
+}
+
+/* fetched: the result words were already copied (plan_fetch) and the copy
+ * waited for. */
+static int plan_collect(SimInf_hip_plan *P, bool filled, bool fetched = false)
+{
     const KParams &p = P->p;
-    if (hipMemcpyAsync(&key, p.err_key, sizeof(key), hipMemcpyDeviceToHost,
-                       P->stream) != hipSuccess ||
-        hipMemcpyAsync(&trans, p.transitions, sizeof(trans),
-                       hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
-        (p.frozen_cols && hipMemcpyAsync(&fill_cols, p.frozen_cols, sizeof(fill_cols),
-                                         hipMemcpyDeviceToHost, P->stream) != hipSuccess) ||
-        hipStreamSynchronize(P->stream) != hipSuccess)
+    if (!fetched && (!plan_fetch(P, P->stream) || hipStreamSynchronize(P->stream) != hipSuccess))
         return SIMINF_ERR_DEVICE;
+    const unsigned long long key = P->host_words[0], trans = P->host_words[1];
+    const unsigned long long fill_cols = p.frozen_cols ? P->host_words[2] : 0;
     {
         /* Device times: the whole run; the solver (classify and active
          * passes, or the one kernel); the fill kernel on its own
@@ -1541,6 +1558,8 @@ extern "C" void SimInf_hip_plan_destroy(SimInf_hip_plan *P)
         (void)hipEventDestroy(P->ev_stop);
     if (P->stream)
         (void)hipStreamDestroy(P->stream);
+    if (P->host_words)
+        (void)hipHostFree(P->host_words);
     delete P;
 }
 
@@ -1734,8 +1753,13 @@ static int batch_run_streams(SimInf_hip_batch *B, const unsigned long *seeds)
             hipStreamWaitEvent(B->stream, B->join[i], 0) != hipSuccess)
             return SIMINF_ERR_DEVICE;
     }
-    if (hipEventRecord(B->ev1, B->stream) != hipSuccess ||
-        hipStreamSynchronize(B->stream) != hipSuccess)
+    if (hipEventRecord(B->ev1, B->stream) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    /* Every plan's result words in one wait. */
+    for (SimInf_hip_plan *P : B->plans)
+        if (!plan_fetch(P, B->stream))
+            return SIMINF_ERR_DEVICE;
+    if (hipStreamSynchronize(B->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
     return 0;
 }
@@ -1756,7 +1780,7 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
         B->ms = ms;
         int first = 0;
         for (int i = 0; i < n; i++) {
-            const int code = plan_collect(B->plans[i], B->filled[i] != 0);
+            const int code = plan_collect(B->plans[i], B->filled[i] != 0, true);
             if (code && !first)
                 first = code;
         }
@@ -1779,8 +1803,12 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
                        B->stream) != hipSuccess ||
         hipEventRecord(B->ev0, B->stream) != hipSuccess ||
         hipGraphLaunch(B->exec, B->stream) != hipSuccess ||
-        hipEventRecord(B->ev1, B->stream) != hipSuccess ||
-        hipStreamSynchronize(B->stream) != hipSuccess)
+        hipEventRecord(B->ev1, B->stream) != hipSuccess)
+        return SIMINF_ERR_DEVICE;
+    for (SimInf_hip_plan *P : B->plans)
+        if (!plan_fetch(P, B->stream))
+            return SIMINF_ERR_DEVICE;
+    if (hipStreamSynchronize(B->stream) != hipSuccess)
         return SIMINF_ERR_DEVICE;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, B->ev0, B->ev1);
@@ -1789,7 +1817,7 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
      * event-record nodes of the graph). */
     int first = 0;
     for (int i = 0; i < n; i++) {
-        const int code = plan_collect(B->plans[i], B->filled[i] != 0);
+        const int code = plan_collect(B->plans[i], B->filled[i] != 0, true);
         if (code && !first)
             first = code;
     }
