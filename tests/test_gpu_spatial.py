@@ -150,3 +150,52 @@ def test_sparse_output_of_a_spatial_model(oracle):
     assert ref.err == 0
     np.testing.assert_array_equal(cu.data, sp_values(m.U_sparse, ref.prU))
     np.testing.assert_array_equal(cv.data, sp_values(m.V_sparse, ref.prV))
+
+
+def _rep_ldata(n, R, seed):
+    """Per-replicate ldata of a spatial model: each replicate its own
+    interval ends and its own neighbour lists (cutoffs 1.0, 0.9, ...),
+    padded to the longest list."""
+    from siminf_amd.spatial import ldata_sp
+    rng = np.random.default_rng(seed)
+    k = int(np.ceil(np.sqrt(n)))
+    gx, gy = np.meshgrid(np.arange(k), np.arange(k))
+    x = 0.8 * gx.ravel()[:n] + rng.uniform(-0.3, 0.3, n)
+    y = 0.8 * gy.ravel()[:n] + rng.uniform(-0.3, 0.3, n)
+    cols = []
+    for r in range(R):
+        ends = np.vstack([np.full(n, e) for e in rng.permutation([91, 182, 273, 365])])
+        cols.append(ldata_sp(ends, si.distance_matrix(x, y, 1.0 - 0.1 * r), 1))
+    rows = max(c.shape[0] for c in cols)
+    return np.hstack([np.vstack([c, np.zeros((rows - c.shape[0], n))]) for c in cols])
+
+
+@pytest.mark.parametrize("n", [200, 600])
+def test_spatial_ldata_per_replicate(oracle, n):
+    """Replicates with their own neighbour lists (one workgroup per
+    replicate at 200 herds, grid launches at 600) equal the checker given
+    the same per-replicate ldata; a list naming a node outside the model
+    is refused."""
+    R = 3
+    ld = _rep_ldata(n, R, seed=n)
+    m = C.sise_sp_model(n=n, days=80, seed=11, replicates=R)
+    # The model's ldata and the per-replicate lists padded to one height.
+    rows = max(ld.shape[0], m.ldata.shape[0])
+    ld = np.vstack([ld, np.zeros((rows - ld.shape[0], ld.shape[1]))])
+    m.ldata = np.asfortranarray(np.vstack([m.ldata, np.zeros((rows - m.ldata.shape[0], n))]))
+    m.Nld = m.ldata.shape[0]
+    m.ldata_names = ["L%d" % (i + 1) for i in range(m.Nld)]
+    print("case: SISe_sp %d herds x %d replicates, per-replicate ldata" % (n, R), flush=True)
+    ref = oracle.run(m, seed=5, rep_ldata=ld)
+    assert ref.err == 0
+    with si.Plan(m) as p:
+        p.set_replicate_data(ldata=ld)
+        tr, _, _ = p.run(5)
+        out = p.download()
+        np.testing.assert_array_equal(out.U, ref.U)
+        np.testing.assert_array_equal(out.V, ref.V)
+        assert tr == ref.transitions
+        bad = ld.copy()
+        bad[4, n + 7] = float(n)
+        with pytest.raises(si.SimInfError, match="Invalid model."):
+            p.set_replicate_data(ldata=bad)
