@@ -606,6 +606,24 @@ def main():
     if args.workload == "seir_batch":
         print(json.dumps(bench_batch(args)), flush=True)
         return
+    if args.workload == "pfilter_request":
+        # One configuration-4 request, repeated: warm-up requests, then
+        # `steps` timed ones.
+        for _ in range(args.warmup):
+            bench_pfilter_request(args)
+        runs = [bench_pfilter_request(args) for _ in range(args.steps)]
+        secs = [r["seconds"] for r in runs]
+        out = {"metric": "particle-simulations/sec of one particle-filter request",
+               "value": 2000 * len(secs) / sum(secs), "unit": "particle-sims/s",
+               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": 1e3 * sum(secs) / len(secs), "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None,
+               "dtype": "int32 state, f64 rates/time", "data": "synthetic",
+               "config": {"workload": runs[0]["workload"]},
+               "p50_ms": float(np.percentile(secs, 50) * 1e3),
+               "p99_ms": float(np.percentile(secs, 99) * 1e3), "request": runs[-1]}
+        print(json.dumps(out), flush=True)
+        return
 
     from siminf_amd import _native
     model, config = make_workload(args.workload)
