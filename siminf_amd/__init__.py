@@ -10,10 +10,12 @@ from .particle_filter import pfilter, PFilterResult
 from .plan import Plan, Batch
 from .punchcard import punchcard
 from .abc_smc import abc, ABCResult
+from .pmcmc import pmcmc, PMCMCResult
 from .summaries import SummaryError
 from . import _native
 
 __all__ = ["SimInfModel", "SimInfError", "make_events", "run", "trajectory",
            "SIR", "SEIR", "SIS", "SISe", "SISe3", "SISe_sp", "SISe3_sp", "SIR_mparse", "mparse",
            "distance_matrix", "ldata_sp", "add_spatial_coupling_to_ldata",
-           "pfilter", "PFilterResult", "Plan", "Batch", "SummaryError", "punchcard", "abc", "ABCResult"]
+           "pfilter", "PFilterResult", "Plan", "Batch", "SummaryError", "punchcard", "abc", "ABCResult",
+           "pmcmc", "PMCMCResult"]
