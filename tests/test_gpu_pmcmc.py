@@ -6,7 +6,9 @@ together through the serving loop equal the same chains run alone."""
 import numpy as np
 import pytest
 
-from siminf_amd import pmcmc as P
+import importlib
+
+P = importlib.import_module("siminf_amd.pmcmc")
 from tests.test_pmcmc import IOBS, PRIORS, TIMES, reference_model
 
 pytestmark = pytest.mark.gpu

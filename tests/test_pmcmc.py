@@ -10,7 +10,9 @@ import numpy as np
 import pytest
 
 import siminf_amd as si
-from siminf_amd import pmcmc as P
+import importlib
+
+P = importlib.import_module("siminf_amd.pmcmc")
 from siminf_amd.abc_smc import parse_priors
 
 # tests/pmcmc.R:10-24: one SIR node and its observed infected.
