@@ -3,13 +3,14 @@ reference's own example (tests/pmcmc.R:10-24: one SIR node, Poisson
 observations of I + 1e-6): the chain equals, row for row, the chain the
 same seeds give with the checker's filter as its likelihood; chains run
 together through the serving loop equal the same chains run alone."""
+import importlib
+
 import numpy as np
 import pytest
 
-import importlib
+from tests.test_pmcmc import IOBS, PRIORS, TIMES, reference_model
 
 P = importlib.import_module("siminf_amd.pmcmc")
-from tests.test_pmcmc import IOBS, PRIORS, TIMES, reference_model
 
 pytestmark = pytest.mark.gpu
 

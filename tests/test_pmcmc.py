@@ -4,16 +4,16 @@ prior densities against their closed forms, the proposal square root,
 and the Metropolis-Hastings chain against an exact posterior: with a
 Gaussian log-likelihood standing in for the filter, the chain's draws
 must have the conjugate posterior's mean and variance."""
+import importlib
 import math
 
 import numpy as np
 import pytest
 
 import siminf_amd as si
-import importlib
+from siminf_amd.abc_smc import parse_priors
 
 P = importlib.import_module("siminf_amd.pmcmc")
-from siminf_amd.abc_smc import parse_priors
 
 # tests/pmcmc.R:10-24: one SIR node and its observed infected.
 TIMES = np.array([1, 4, 7, 10, 13, 16, 19, 22, 25, 28, 31, 34, 37, 40, 43, 46, 49, 52,
