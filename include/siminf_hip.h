@@ -254,6 +254,18 @@ int SimInf_hip_plan_set_lanes_per_wave(SimInf_hip_plan *plan, int shift);
  * counterpart). */
 int SimInf_hip_plan_set_fill_block(SimInf_hip_plan *plan, int cols);
 
+/* Where the plan's last run failed (no reference counterpart; the
+ * reference prints the failing node's or event's state to stderr,
+ * SimInf_solver.c:550-645, :1191-1249): *code the error code (0 if the
+ * run succeeded; the other outputs are then -1), *replicate, *day (the
+ * run's day, 0 for the day of tspan[0]), *phase (0 initial rates, 1 the
+ * SSA or aem, 2 scheduled events within a node, 3 external transfers,
+ * 4 the post-time-step function) and *where: the node (zero-based) for
+ * phases 0, 1 and 4, the event's index in the model's event list for
+ * phases 2 and 3. */
+int SimInf_hip_plan_last_error(const SimInf_hip_plan *plan, int *code, int *replicate,
+                               int *day, int *phase, long long *where);
+
 /* The columns per fill block the plan uses. */
 int SimInf_hip_plan_get_fill_block(const SimInf_hip_plan *plan, int *cols);
 

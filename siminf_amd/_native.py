@@ -100,6 +100,7 @@ EXPORTED_SYMBOLS = (
     "SimInf_hip_abc_weights",
     "SimInf_hip_plan_set_lanes_per_wave",
     "SimInf_hip_plan_set_fill_block",
+    "SimInf_hip_plan_last_error",
     "SimInf_hip_plan_get_fill_block",
     "SimInf_hip_plan_set_herd_map",
     "SimInf_hip_plan_get_herd_map",
@@ -177,6 +178,10 @@ def load(path=None):
     lib.SimInf_hip_plan_set_lanes_per_wave.restype = ctypes.c_int
     lib.SimInf_hip_plan_set_lanes_per_wave.argtypes = [vp, ctypes.c_int]
     lib.SimInf_hip_plan_set_fill_block.restype = ctypes.c_int
+    ip_ = ctypes.POINTER(ctypes.c_int)
+    lib.SimInf_hip_plan_last_error.restype = ctypes.c_int
+    lib.SimInf_hip_plan_last_error.argtypes = [vp, ip_, ip_, ip_, ip_,
+                                                ctypes.POINTER(ctypes.c_longlong)]
     lib.SimInf_hip_plan_set_fill_block.argtypes = [vp, ctypes.c_int]
     lib.SimInf_hip_plan_get_fill_block.restype = ctypes.c_int
     lib.SimInf_hip_plan_get_fill_block.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]

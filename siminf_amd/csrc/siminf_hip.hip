@@ -237,6 +237,10 @@ struct SimInf_hip_plan {
      * into them) and their pinned host copy, fetched with one copy. */
     unsigned long long *result_words = nullptr;
     unsigned long long *host_words = nullptr;
+    /* The last run's first error key, and each day's first event in the
+     * event list (-1 for days without events), to name its event. */
+    uint64_t last_key = ~0ull;
+    std::vector<int> ev_day_first;
 
     /* Lanes per wavefront of the active pass as a power of two, or -1
      * for the rule of about two wavefronts per SIMD. */
@@ -491,6 +495,12 @@ static int plan_create_impl(SimInf_hip_plan **out, const SimInf_solver_args *a,
             prev_day = ev_day[k];
         }
         ev_rank[k] = k - day_start;
+        if (ev_day[k] >= 0) {
+            if ((size_t)ev_day[k] >= P->ev_day_first.size())
+                P->ev_day_first.resize((size_t)ev_day[k] + 1, -1);
+            if (P->ev_day_first[ev_day[k]] < 0)
+                P->ev_day_first[ev_day[k]] = k;
+        }
         /* The reference checks node bounds first for every event
          * (SimInf_solver.c:674-678). */
         if (ev_node[k] < 0 || ev_node[k] >= Nn) {
@@ -1196,7 +1206,8 @@ static int plan_collect(SimInf_hip_plan *P, bool filled, bool fetched = false)
     }
     P->fill_cols = fill_cols;
     P->transitions = trans;
-    const int code = key_to_code(std::min<uint64_t>(key, P->host_err_key));
+    P->last_key = std::min<uint64_t>(key, P->host_err_key);
+    const int code = key_to_code(P->last_key);
     P->has_run = code == 0;
     return code;
 }
@@ -1377,6 +1388,32 @@ extern "C" int SimInf_hip_plan_set_replicate_data(SimInf_hip_plan *P, const doub
 
 /* Columns per fill block that the autotuners try. */
 static const int fill_block_candidates[] = {4, 8, 16, 32, 64};
+
+extern "C" int SimInf_hip_plan_last_error(const SimInf_hip_plan *P, int *code, int *replicate,
+                                          int *day, int *phase, long long *where)
+{
+    if (!P)
+        return SIMINF_ERR_INVALID_MODEL;
+    const uint64_t k = P->last_key;
+    const int c = key_to_code(k);
+    const int rep = (int)(k >> 48), d = (int)((k >> 30) & 0x3FFFF);
+    const int ph = (int)((k >> 27) & 7);
+    long long pos = (long long)((k >> 6) & 0x1FFFFF);
+    if (c && (ph == PH_E1 || ph == PH_E2))
+        pos = d < (int)P->ev_day_first.size() && P->ev_day_first[d] >= 0
+                  ? P->ev_day_first[d] + pos : -1;
+    if (code)
+        *code = c;
+    if (replicate)
+        *replicate = c ? rep : -1;
+    if (day)
+        *day = c ? d : -1;
+    if (phase)
+        *phase = c ? ph : -1;
+    if (where)
+        *where = c ? pos : -1;
+    return 0;
+}
 
 extern "C" int SimInf_hip_plan_set_fill_block(SimInf_hip_plan *P, int cols)
 {

@@ -24,11 +24,17 @@ EVENT_TYPES = {"exit": 0, "enter": 1, "intTrans": 2, "extTrans": 3}
 
 
 class SimInfError(RuntimeError):
-    """An error code of the solver core, with the reference's message."""
+    """An error code of the solver core, with the reference's message.
+    Errors of a plan's run also carry where it failed (``detail``: a dict
+    of replicate, day, phase and node or event), or None."""
 
-    def __init__(self, code, message):
+    PHASES = ("initial rates", "ssa", "scheduled events", "external transfers",
+              "post time step")
+
+    def __init__(self, code, message, detail=None):
         super().__init__(message)
         self.code = code
+        self.detail = detail
 
 
 def _csc(x, dtype=np.float64):

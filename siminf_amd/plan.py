@@ -49,7 +49,26 @@ class Plan:
 
     def _check(self, err):
         if err:
-            _raise(self.lib, err)
+            try:
+                _raise(self.lib, err)
+            except SimInfError as e:
+                e.detail = self.last_error()
+                raise
+
+    def last_error(self):
+        """Where the last run failed: {"replicate", "day", "phase" and
+        "node" or "event" (zero-based, the event's index in the model's
+        sorted event list)}, or None if it succeeded."""
+        c, r, d, ph = (ctypes.c_int() for _ in range(4))
+        w = ctypes.c_longlong()
+        if self.lib.SimInf_hip_plan_last_error(self._handle, ctypes.byref(c), ctypes.byref(r),
+                                               ctypes.byref(d), ctypes.byref(ph),
+                                               ctypes.byref(w)) or c.value == 0:
+            return None
+        phase = SimInfError.PHASES[ph.value]
+        where = "event" if ph.value in (2, 3) else "node"
+        return {"code": c.value, "replicate": r.value, "day": d.value, "phase": phase,
+                where: w.value}
 
     def run(self, seed):
         """One run with this seed; returns (transitions, node_days,

@@ -68,3 +68,32 @@ def test_enter_shift_known_answer_on_device():
         err, b, _ = _gpu(m, solver=solver, seed=3)
         assert err == 0
         np.testing.assert_array_equal(b.U, U)
+
+
+def test_where_a_plan_failed():
+    """SimInf_hip_plan_last_error names what the reference prints on
+    failure: the event (its index in the sorted event list) for event
+    errors, the node for a post-time-step error; and nothing after a run
+    that succeeded."""
+    import siminf_amd as si
+    cases = {c[0]: c for c in C.error_cases()}
+    for name, phase, key, value in (("dest_out_of_bounds", "external transfers", "event", 0),
+                                    ("exit_empty_select", "scheduled events", "event", 0),
+                                    ("sise3_v_negative", "post time step", "node", None)):
+        m = cases[name][1]()
+        print("case: last error of %s" % name, flush=True)
+        with si.Plan(m) as p:
+            with pytest.raises(si.SimInfError) as e:
+                p.run(3)
+            d = e.value.detail
+            assert d is not None and d["code"] == cases[name][2] and d["phase"] == phase, d
+            assert key in d and d["replicate"] == 0
+            if value is not None:
+                assert d[key] == value
+            else:
+                assert 0 <= d[key] < m.Nn
+            assert p.last_error() == d
+    m = C.seir_pure_death_model(50)
+    with si.Plan(m) as p:
+        p.run(1)
+        assert p.last_error() is None
