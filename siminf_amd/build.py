@@ -29,18 +29,20 @@ FLAGS = [
 ]
 
 
-# The node kernels' translation units are scheduled for instruction-level
-# parallelism: their wavefronts run mostly one per SIMD, each a long
-# dependent chain (the SSA step) beside independent work (the next
-# attempt's Philox block and logarithm), so interleaving the two matters
-# more than the register pressure the default scheduler minimises.
+# The models' translation units (node and coupled kernels) are scheduled
+# for instruction-level parallelism: their wavefronts run one or two per
+# SIMD, each a long dependent chain (the SSA step) beside independent work
+# (the next attempt's Philox block and logarithm), so interleaving the two
+# matters more than the register pressure the default scheduler minimises
+# (measured on the coupled kernel: SISe3 1600 herds x 1460 days 138.0 to
+# 132.2 ms per run).
 ILP_FLAGS = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
 
 
 def unit_flags(src):
     """Compile flags of one translation unit."""
     base = os.path.basename(src)
-    if base.startswith("builtin_") and not base.endswith("_coupled.hip"):
+    if base.startswith("builtin_"):
         return FLAGS + ILP_FLAGS
     return FLAGS
 
