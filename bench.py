@@ -334,12 +334,19 @@ def bench_sise3(args, steps=10, spatial=False):
             nd += b
             ms.append(c)
         wall = time.perf_counter() - t0
-    return {"workload": "%s 1600 herds x 1460 days, %d scheduled events"
-                        % ("SISe3_sp" if spatial else "SISe3", len(model.events["event"])),
-            "node_days_per_s": nd / wall, "transitions_per_s": tr / wall,
-            "ms_per_run": wall / steps * 1e3, "kernel_ms_mean": statistics.mean(ms),
-            "herd_map": {"solo_herds": solo, "herds_per_wavefront": pack},
-            "algorithmic_bytes_per_run": algorithmic_bytes(model)}
+    out = {"workload": "%s 1600 herds x 1460 days, %d scheduled events"
+                       % ("SISe3_sp" if spatial else "SISe3", len(model.events["event"])),
+           "node_days_per_s": nd / wall, "transitions_per_s": tr / wall,
+           "ms_per_run": wall / steps * 1e3, "kernel_ms_mean": statistics.mean(ms),
+           "herd_map": {"solo_herds": solo, "herds_per_wavefront": pack},
+           "algorithmic_bytes_per_run": algorithmic_bytes(model)}
+    if not spatial and not args.no_cpu_baseline:
+        # The checker at the reference's partition (node blocks over the
+        # host's cores; the external transfers on one thread) and on one
+        # core.
+        out["cpu_baseline"] = cpu_baseline(model, ["ssm"], host_cores(), budget_s=6.0)
+        out["cpu_baseline_1core"] = cpu_baseline(model, ["ssm"], 1, budget_s=4.0)
+    return out
 
 
 def bench_sise3_replicates(reps=(1, 2, 4, 8, 16, 32), runs=2):
