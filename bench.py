@@ -192,7 +192,7 @@ def bench_batch(args):
         # The launch shape is tuned for the plans together: each plan's
         # lanes per wavefront and columns per fill block, and the graph
         # against the direct enqueue.
-        ms, shifts, graph = batch.autotune(seed=200, reps=5)
+        ms, shifts, graph = batch.autotune(seed=200, reps=7)
         tuned = {"lanes_per_wave_shift": dict(zip(solvers, shifts)),
                  "fill_block": {s: p.fill_block() for s, p in zip(solvers, plans)},
                  "launch": "graph" if graph else "streams", "tuned_ms": ms}

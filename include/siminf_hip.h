@@ -338,9 +338,9 @@ int SimInf_hip_batch_set_graph(SimInf_hip_batch *batch, int use_graph);
  * plans share the device, so each plan's lanes per wavefront (see
  * SimInf_hip_plan_set_lanes_per_wave) and columns per fill block (see
  * SimInf_hip_plan_set_fill_block) are chosen by coordinate descent on
- * the batch's launch time (two passes, every candidate of one plan with
- * the others held, reps launches each, seeds from seed), then the graph
- * or the direct enqueue, whichever is faster, is kept.  best_ms (optional)
+ * the batch's median launch time (two passes, every candidate of one
+ * plan with the others held, reps launches each, seeds from seed), then
+ * the graph or the direct enqueue, whichever is faster, is kept.  best_ms (optional)
  * gets the kept shape's mean launch time.  Results do not depend on it. */
 int SimInf_hip_batch_autotune(SimInf_hip_batch *batch, unsigned long int seed, int reps,
                               double *best_ms);

@@ -1866,7 +1866,7 @@ extern "C" int SimInf_hip_batch_run(SimInf_hip_batch *B, const unsigned long *se
  * are tuned in the batch rather than alone.  Coordinate descent, two
  * passes: for each plan with an active pass and a fill kernel in turn,
  * every shift (-1 for the default rule, 0 to 6), then every columns-per-
- * fill-block candidate, with the others held, keeping the least mean
+ * fill-block candidate, with the others held, keeping the least median
  * launch time over reps launches (seeds from seed); then the graph
  * against the direct enqueue, the faster kept (the graph recaptured with
  * the kept shapes).  Results do not depend on any of it. */
@@ -1878,17 +1878,21 @@ extern "C" int SimInf_hip_batch_autotune(SimInf_hip_batch *B, unsigned long int 
     const int n = (int)B->plans.size();
     std::vector<unsigned long> sd((size_t)n);
     unsigned long next = seed;
-    auto measure = [&](double *mean) -> int {
-        double sum = 0.0;
+    /* The median launch time over reps launches: a launch slowed by
+     * something outside the batch (a clock dip, the host) does not decide
+     * a shape. */
+    auto measure = [&](double *median) -> int {
+        std::vector<double> t((size_t)reps);
         for (int r = 0; r < reps; r++) {
             for (int i = 0; i < n; i++)
                 sd[i] = next++;
             const int err = SimInf_hip_batch_run(B, sd.data());
             if (err)
                 return err;
-            sum += B->ms;
+            t[r] = B->ms;
         }
-        *mean = sum / reps;
+        std::sort(t.begin(), t.end());
+        *median = reps % 2 ? t[reps / 2] : 0.5 * (t[reps / 2 - 1] + t[reps / 2]);
         return 0;
     };
     B->use_graph = false;
