@@ -271,7 +271,9 @@ def bench_batch(args):
                         ("sise3_herds", lambda: bench_sise3(args)),
                         ("sise3_sp_herds", lambda: bench_sise3(args, spatial=True)),
                         ("sise3_replicates", bench_sise3_replicates),
-                        ("pfilter_request", lambda: bench_pfilter_request(args))):
+                        ("pfilter_request", lambda: bench_pfilter_request(args)),
+                        ("abc_stream", lambda: bench_abc_stream(args, generations=2,
+                                                                proposals=5000))):
             try:
                 out[key] = fn()
             except Exception as e:          # reported, not fatal to the line
