@@ -44,6 +44,8 @@ def distance_matrix(x, y, cutoff, min_dist=None, na_fail=True):
         md = float("nan") if min_dist is None else float(min_dist)
     except (TypeError, ValueError):
         md = float("nan")    # as.numeric of a non-number is NA
+    if n > 2048 and np.isfinite(x).all() and np.isfinite(y).all():
+        return _distance_matrix_tree(x, y, cutoff, md)
     rows, vals, ptr = [], [], [0]
     idx = np.arange(n)
     for i in range(n):
@@ -74,6 +76,32 @@ def distance_matrix(x, y, cutoff, min_dist=None, na_fail=True):
         ptr.append(ptr[-1] + len(dj))
     return sp.csc_matrix((np.concatenate(vals), np.concatenate(rows).astype(np.int32),
                           np.asarray(ptr, dtype=np.int32)), shape=(n, n))
+
+
+def _distance_matrix_tree(x, y, cutoff, md):
+    """distance_matrix for many finite points: a k-d tree proposes the
+    pairs within a slightly larger radius, and each pair is then kept or
+    dropped by the reference's own test on hypot (so pairs at the cutoff
+    fall as the reference's do)."""
+    from scipy.spatial import cKDTree
+    n = len(x)
+    tree = cKDTree(np.column_stack([x, y]))
+    pairs = tree.query_pairs(cutoff * (1.0 + 1e-9) + 1e-300, output_type="ndarray")
+    i = np.concatenate([pairs[:, 0], pairs[:, 1]])      # column (node)
+    j = np.concatenate([pairs[:, 1], pairs[:, 0]])      # row (neighbour)
+    d = np.hypot(x[i] - x[j], y[i] - y[j])
+    keep = d <= cutoff
+    i, j, d = i[keep], j[keep], d[keep]
+    if (d <= 0).any():
+        if not (math.isfinite(md) and md >= 0):
+            raise SpatialError("Invalid 'min_dist' argument. "
+                               "Please provide 'min_dist' > 0.")
+        d = np.where(d <= 0, md, d)
+    order = np.lexsort((j, i))
+    i, j, d = i[order], j[order], d[order]
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(np.bincount(i, minlength=n), out=ptr[1:])
+    return sp.csc_matrix((d, j.astype(np.int32), ptr), shape=(n, n))
 
 
 def ldata_sp(data, distance, metric):

@@ -259,3 +259,33 @@ def test_spread_errors(oracle, coupling, code):
     for solver in ("ssm", "aem"):
         r = oracle.run(m, solver=solver, seed=1)
         assert r.err == code, solver
+
+
+def test_distance_matrix_many_points_same_as_pairwise():
+    """Above 2048 points the k-d tree path gives the pairwise loop's
+    matrix exactly (indices, order and distances), points on the cutoff
+    included; and coincident points take min_dist."""
+    import siminf_amd.spatial as S
+    rng = np.random.default_rng(4)
+    n = 3000
+    x = np.round(rng.uniform(0, 40, n), 1)          # many exact ties at the cutoff
+    y = np.round(rng.uniform(0, 40, n), 1)
+    x[5], y[5] = x[6], y[6]
+    tree = si.distance_matrix(x, y, 1.0, min_dist=0.01)
+    loop = S.distance_matrix(x[:2048], y[:2048], 1.0, min_dist=0.01)   # pairwise path
+    assert tree.shape == (n, n)
+    sub = tree[:2048, :2048].tocsc()
+    sub.sort_indices()
+    want = sp.csc_matrix(loop)
+    want.sort_indices()
+    assert (sub != want).nnz == 0
+    # the first 2048 points' columns hold only neighbours among themselves
+    # or beyond: compare the full tree against a brute force on all pairs
+    d = np.hypot(x[:, None] - x[None, :], y[:, None] - y[None, :])
+    np.fill_diagonal(d, np.inf)
+    d[d <= 0] = 0.01
+    ii, jj = np.nonzero(d.T <= 1.0)
+    assert tree.nnz == len(ii)
+    np.testing.assert_array_equal(tree.toarray().T[ii, jj] > 0, True)
+    assert list(tree.indices[tree.indptr[0]:tree.indptr[1]]) == sorted(
+        tree.indices[tree.indptr[0]:tree.indptr[1]])
