@@ -53,3 +53,16 @@ def test_error_in_one_replicate_only(oracle):
     m._u0[4:6, 2 * Nn + 4] = (3, 2)
     ref, _ = _check(m, oracle, seed=33)
     assert ref.err == -8
+
+
+def test_coupled_size_bound(oracle):
+    """One coupled replicate's herd workgroups must be resident together:
+    40,000 herds run (bit for bit against the checker), 200,000 are
+    refused at plan creation with the reference's allocation message."""
+    m = W.sise3_herds(Nn=40_000, days=8, seed=24)
+    ref, _ = _check(m, oracle, seed=34)
+    assert ref.err == 0
+    big = W.sise3_herds(Nn=200_000, days=3, seed=25)
+    print("case: SISe3 200000 herds refused", flush=True)
+    with pytest.raises(si.SimInfError, match="Unable to allocate memory buffer."):
+        si.Plan(big)
